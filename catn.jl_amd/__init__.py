@@ -1,0 +1,469 @@
+"""catn_amd — MI355X contraction engine for CATN.jl networks (Python host mirror).
+
+Mirrors the reference's public API (xuanzhaogao/CATN.jl, src/CATN.jl:15-32)
+over the C ABI of `libcatn_hip.so` (include/catn_hip.h):
+
+  ising_network(n, edges, w, h, beta; ...)  src/ising.jl:20-227
+  TensorNetwork(tensors, ixs; ...)          src/tensor_network.jl:41-140
+  contraction(tn) -> (lnZ, error, psi)      contraction!  src/tensor_network.jl:556-717
+  contraction_batch([tn...])                many networks per persistent launch
+  free_energy(tn) -> (lnZ/n, F)             src/ising.jl:239-244
+  magnetization(...), correlation(...)      src/ising.jl:263-339 (pinning; all
+                                            pinned networks run as one batch)
+  tsvd(A; cutoff, maxdim)                   src/linalg_utils.jl:15-28
+  MPSNet: hand-built MPSNode primitives      src/mps_node.jl (swap!, cano_to!, ...)
+
+The numeric work runs only in the HIP library. There is no CPU fallback: if
+the library is missing or no GPU is present, creating a Context raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcatn_hip.so")
+
+__all__ = [
+    "Context", "TensorNetwork", "MPSNet", "ising_network", "tensor_network", "contraction",
+    "contraction_batch", "free_energy", "magnetization", "correlation", "tsvd",
+    "default_context", "LIB_PATH", "CatnError",
+]
+
+
+class CatnError(RuntimeError):
+    pass
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("Dmax", C.c_int32), ("chi", C.c_int32), ("cutoff", C.c_double),
+        ("norm_method", C.c_int32), ("select", C.c_int32), ("reverse", C.c_int32),
+        ("svdopt", C.c_int32), ("swapopt", C.c_int32), ("compress", C.c_int32),
+        ("cut_bond", C.c_int32), ("seed", C.c_int32), ("arena_elems", C.c_int64),
+        ("scratch_elems", C.c_int64),
+    ]
+
+
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+
+_SIGS = {
+    "catn_params_default": (None, [C.POINTER(_Params)]),
+    "catn_create": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "catn_destroy": (C.c_int, [_P]),
+    "catn_last_error": (C.c_char_p, []),
+    "catn_backend": (C.c_char_p, [_P]),
+    "catn_ising_network": (C.c_int, [_P, C.c_int32, C.c_int32, _I32P, _DP, _DP, C.c_double,
+                                     C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     C.POINTER(_Params), C.POINTER(_P)]),
+    "catn_tensor_network": (C.c_int, [_P, C.c_int32, _I32P, _I64P, _DP, _I64P,
+                                      C.POINTER(_Params), C.POINTER(_P)]),
+    "catn_net_create": (C.c_int, [_P, C.POINTER(_Params), C.POINTER(_P)]),
+    "catn_net_add_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I32P, _DP, _I32P, C.c_int32]),
+    "catn_net_add_tensor_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I64P, _DP, _I32P]),
+    "catn_net_free": (C.c_int, [_P]),
+    "catn_contraction": (C.c_int, [_P, _DP, _DP, _DP]),
+    "catn_contraction_batch": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _DP, _DP, _DP]),
+    "catn_net_info": (C.c_int, [_P, _DP, _I32P, _I32P, _I32P, _DP]),
+    "catn_net_edge_log": (C.c_int, [_P, _I32P, _I32P]),
+    "catn_net_mem": (C.c_int, [_P, _I64P, _I64P]),
+    "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
+    "catn_node_cano_to": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "catn_node_left_canonical": (C.c_int, [_P, C.c_int32]),
+    "catn_node_move": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
+    "catn_node_move2tail": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
+    "catn_node_move2head": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
+    "catn_node_reverse": (C.c_int, [_P, C.c_int32]),
+    "catn_node_merge": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
+    "catn_node_compress": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
+    "catn_node_eat": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _DP, _DP, _DP]),
+    "catn_cut_bondim": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
+    "catn_node_get": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32]),
+    "catn_node_site": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_int64]),
+    "catn_tsvd": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_double, C.c_int32, _I32P, _DP,
+                            _DP, _DP, _DP]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise CatnError(f"libcatn_hip not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class Context:
+    """One device context (one HIP stream). `lib` selects the shared library
+    (default: the in-tree libcatn_hip.so)."""
+
+    def __init__(self, device: int = 0, lib: Optional[str] = None):
+        self.lib = load_library(lib or LIB_PATH)
+        h = _P()
+        self._check(self.lib.catn_create(C.byref(h), int(device)))
+        self.h = h
+        self.backend = self.lib.catn_backend(h).decode()
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.catn_last_error().decode()
+            raise CatnError(f"libcatn (rc={rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.catn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEFAULT: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse=True,
+            svdopt=True, swapopt=True, compress=False, cut_bond=False, seed=1,
+            arena_elems=0, scratch_elems=0):
+    p = _Params()
+    ctx.lib.catn_params_default(C.byref(p))
+    p.Dmax, p.chi, p.cutoff = int(Dmax), int(chi), float(cutoff)
+    p.norm_method, p.select = int(norm_method), int(select)
+    p.reverse, p.svdopt, p.swapopt = int(bool(reverse)), int(bool(svdopt)), int(bool(swapopt))
+    p.compress, p.cut_bond, p.seed = int(bool(compress)), int(bool(cut_bond)), int(seed)
+    p.arena_elems, p.scratch_elems = int(arena_elems), int(scratch_elems)
+    return p
+
+
+class TensorNetwork:
+    """Handle of a device-resident network (TensorNetwork{T,AT} of the
+    reference with AT = device storage)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.h = handle
+        self.lnZ = 0.0
+        self.psi = 1.0
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.catn_net_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self):
+        sign = C.c_double()
+        md, ni, n = C.c_int32(), C.c_int32(), C.c_int32()
+        beta = C.c_double()
+        self.ctx._check(self.ctx.lib.catn_net_info(self.h, C.byref(sign), C.byref(md),
+                                                   C.byref(ni), C.byref(n), C.byref(beta)))
+        return dict(sign=sign.value, maxdim_intermediate=md.value, num_isolated=ni.value,
+                    n=n.value, beta=beta.value)
+
+    @property
+    def n(self):
+        return self.info()["n"]
+
+    @property
+    def beta(self):
+        return self.info()["beta"]
+
+    @property
+    def sign(self):
+        return self.info()["sign"]
+
+    @property
+    def maxdim_intermediate(self):
+        return self.info()["maxdim_intermediate"]
+
+    @property
+    def num_isolated(self):
+        return self.info()["num_isolated"]
+
+    def edge_log(self) -> List[Tuple[int, int]]:
+        cnt = C.c_int32(0)
+        self.ctx._check(self.ctx.lib.catn_net_edge_log(self.h, None, C.byref(cnt)))
+        buf = np.zeros(2 * max(cnt.value, 1), dtype=np.int32)
+        cap = C.c_int32(cnt.value)
+        self.ctx._check(self.ctx.lib.catn_net_edge_log(self.h, buf.ctypes.data_as(_I32P), C.byref(cap)))
+        return [(int(buf[2 * k]), int(buf[2 * k + 1])) for k in range(cnt.value)]
+
+    def mem(self):
+        u, c = C.c_int64(), C.c_int64()
+        self.ctx._check(self.ctx.lib.catn_net_mem(self.h, C.byref(u), C.byref(c)))
+        return u.value, c.value
+
+
+def ising_network(n, edges, weights, fields, beta, *, pos1=0, val1=0, pos2=0, val2=0,
+                  ctx: Optional[Context] = None, **kw) -> TensorNetwork:
+    """ising_network (src/ising.jl:20-227)."""
+    ctx = ctx or default_context()
+    e = _i32(np.asarray(edges, dtype=np.int32).reshape(-1, 2)) if len(edges) else np.zeros((0, 2), np.int32)
+    w = _f64(weights) if len(weights) else np.zeros(0)
+    h = _f64(fields)
+    p = _params(ctx, **kw)
+    out = _P()
+    ctx._check(ctx.lib.catn_ising_network(
+        ctx.h, int(n), int(e.shape[0]), e.ctypes.data_as(_I32P), _dptr(w), _dptr(h),
+        float(beta), int(pos1), int(val1), int(pos2), int(val2), C.byref(p), C.byref(out)))
+    return TensorNetwork(ctx, out)
+
+
+def tensor_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
+                   ctx: Optional[Context] = None, **kw) -> TensorNetwork:
+    """TensorNetwork(tensors, ixs; ...) (src/tensor_network.jl:41-140). Tensors
+    are taken with Julia (column-major) index order: t[i1, i2, ...]."""
+    ctx = ctx or default_context()
+    lab_ids = {}
+    labels, ndims, dims, data = [], [], [], []
+    for t, ix in zip(tensors, ixs):
+        t = np.asarray(t, dtype=np.float64)
+        if t.ndim != len(ix):
+            raise CatnError("tensor rank does not match its label list")
+        ndims.append(t.ndim)
+        dims.extend(t.shape)
+        data.append(t.ravel(order="F"))
+        for l in ix:
+            if l not in lab_ids:
+                lab_ids[l] = len(lab_ids)
+            labels.append(lab_ids[l])
+    nd = _i32(ndims)
+    dm = np.ascontiguousarray(np.asarray(dims, dtype=np.int64))
+    lb = np.ascontiguousarray(np.asarray(labels, dtype=np.int64))
+    dt = _f64(np.concatenate(data) if data else np.zeros(0))
+    p = _params(ctx, **kw)
+    out = _P()
+    ctx._check(ctx.lib.catn_tensor_network(ctx.h, len(tensors), nd.ctypes.data_as(_I32P),
+                                           dm.ctypes.data_as(_I64P), _dptr(dt),
+                                           lb.ctypes.data_as(_I64P), C.byref(p), C.byref(out)))
+    return TensorNetwork(ctx, out)
+
+
+TensorNetwork_ = tensor_network
+
+
+def contraction(tn: TensorNetwork):
+    """contraction!(tn) -> (lnZ, error, psi)."""
+    lz, err, psi = C.c_double(), C.c_double(), C.c_double()
+    tn.ctx._check(tn.ctx.lib.catn_contraction(tn.h, C.byref(lz), C.byref(err), C.byref(psi)))
+    tn.lnZ, tn.psi = lz.value, psi.value
+    return lz.value, err.value, psi.value
+
+
+def contraction_batch(tns: Sequence[TensorNetwork]):
+    """contraction! of many networks, one workgroup each per launch."""
+    if not tns:
+        return np.zeros(0), np.zeros(0), np.zeros(0)
+    ctx = tns[0].ctx
+    arr = (_P * len(tns))(*[t.h for t in tns])
+    lz, err, psi = np.zeros(len(tns)), np.zeros(len(tns)), np.zeros(len(tns))
+    ctx._check(ctx.lib.catn_contraction_batch(ctx.h, len(tns), arr, _dptr(lz), _dptr(err), _dptr(psi)))
+    for k, t in enumerate(tns):
+        t.lnZ, t.psi = float(lz[k]), float(psi[k])
+    return lz, err, psi
+
+
+def free_energy(tn: TensorNetwork):
+    """free_energy (src/ising.jl:239-244)."""
+    lnZ, _, _ = contraction(tn)
+    n, beta = tn.n, tn.beta
+    lz = lnZ / n
+    return lz, -lz / beta
+
+
+def magnetization(n, edges, weights, fields, beta, *, ctx=None, **kw):
+    """magnetization by pinning (src/ising.jl:263-290); the 1+2n networks are
+    contracted as one batch."""
+    ctx = ctx or default_context()
+    nets = [ising_network(n, edges, weights, fields, beta, ctx=ctx, **kw)]
+    for k in range(1, n + 1):
+        for v in (1, 2):
+            nets.append(ising_network(n, edges, weights, fields, beta, pos1=k, val1=v, ctx=ctx, **kw))
+    lz, _, _ = contraction_batch(nets)
+    lz0 = lz[0]
+    mag = np.zeros(n)
+    for k in range(n):
+        p = np.exp(lz[1 + 2 * k: 3 + 2 * k] - lz0)
+        mag[k] = p[1] - p[0]
+    return mag
+
+
+def correlation(n, edges, weights, fields, beta, *, ctx=None, **kw):
+    """correlation by pinning (src/ising.jl:309-339); 1+4|E| networks in one batch."""
+    ctx = ctx or default_context()
+    nets = [ising_network(n, edges, weights, fields, beta, ctx=ctx, **kw)]
+    for (m, nn) in edges:
+        for v1 in (1, 2):
+            for v2 in (1, 2):
+                nets.append(ising_network(n, edges, weights, fields, beta, pos1=m, val1=v1,
+                                          pos2=nn, val2=v2, ctx=ctx, **kw))
+    lz, _, _ = contraction_batch(nets)
+    lz0 = lz[0]
+    cor = np.zeros(len(edges))
+    for e in range(len(edges)):
+        p = np.exp(lz[1 + 4 * e: 5 + 4 * e] - lz0)
+        cor[e] = p[0] - p[1] - p[2] + p[3]
+    return cor
+
+
+def tsvd(A, cutoff=1e-15, maxdim=None, *, ctx=None):
+    """tsvd (src/linalg_utils.jl:15-28) -> (U, S, V, discarded); S is the kept
+    part; the full spectrum is returned as the fifth value."""
+    ctx = ctx or default_context()
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    m, n = A.shape
+    r = min(m, n)
+    U = np.zeros((m, r), order="F")
+    V = np.zeros((n, r), order="F")
+    S = np.zeros(r)
+    k = C.c_int32()
+    disc = C.c_double()
+    a = A.ravel(order="F")
+    Ub = np.zeros(m * r)
+    Vb = np.zeros(n * r)
+    ctx._check(ctx.lib.catn_tsvd(ctx.h, m, n, _dptr(a), float(cutoff), int(maxdim or 0),
+                                 C.byref(k), _dptr(Ub), _dptr(S), _dptr(Vb), C.byref(disc)))
+    kk = k.value
+    U = Ub[: m * kk].reshape((m, kk), order="F")
+    V = Vb[: n * kk].reshape((n, kk), order="F")
+    return U, S[:kk].copy(), V, disc.value, S
+
+
+class MPSNet:
+    """Hand-built MPSNode objects on the device, for the per-primitive API of
+    src/mps_node.jl. Node ids and positions are 1-based as in the reference."""
+
+    def __init__(self, ctx: Optional[Context] = None, **kw):
+        self.ctx = ctx or default_context()
+        p = _params(self.ctx, **kw)
+        out = _P()
+        self.ctx._check(self.ctx.lib.catn_net_create(self.ctx.h, C.byref(p), C.byref(out)))
+        self.net = TensorNetwork(self.ctx, out)
+        self.h = out
+
+    def add_node(self, id, sites: Sequence[np.ndarray], neighbor, cano=None):
+        dims, data = [], []
+        for s in sites:
+            s = np.asarray(s, dtype=np.float64)
+            assert s.ndim == 3
+            dims.extend(s.shape)
+            data.append(s.ravel(order="F"))
+        d = _i32(dims)
+        dt = _f64(np.concatenate(data) if data else np.zeros(0))
+        nb = _i32(neighbor)
+        self.ctx._check(self.ctx.lib.catn_net_add_node(
+            self.h, int(id), len(sites), d.ctypes.data_as(_I32P), _dptr(dt),
+            nb.ctypes.data_as(_I32P), int(len(sites) if cano is None else cano)))
+
+    def add_tensor(self, id, tensor, neighbor):
+        """MPSNode(tensor, neighbor; chi, cutoff) via raw2mps on the device."""
+        t = np.asarray(tensor, dtype=np.float64)
+        dm = np.ascontiguousarray(np.asarray(t.shape, dtype=np.int64))
+        dt = _f64(t.ravel(order="F"))
+        nb = _i32(neighbor)
+        self.ctx._check(self.ctx.lib.catn_net_add_tensor_node(
+            self.h, int(id), t.ndim, dm.ctypes.data_as(_I64P), _dptr(dt), nb.ctypes.data_as(_I32P)))
+
+    def _err(self, fn, *args):
+        e = C.c_double()
+        self.ctx._check(fn(self.h, *args, C.byref(e)))
+        return e.value
+
+    def swap(self, id, i, j):
+        return self._err(self.ctx.lib.catn_node_swap, int(id), int(i), int(j))
+
+    def move(self, id, a, b):
+        return self._err(self.ctx.lib.catn_node_move, int(id), int(a), int(b))
+
+    def move2tail(self, id, idx):
+        return self._err(self.ctx.lib.catn_node_move2tail, int(id), int(idx))
+
+    def move2head(self, id, idx):
+        return self._err(self.ctx.lib.catn_node_move2head, int(id), int(idx))
+
+    def merge(self, id, j, cross=False):
+        return self._err(self.ctx.lib.catn_node_merge, int(id), int(j), int(bool(cross)))
+
+    def compress(self, id, opt=True):
+        return self._err(self.ctx.lib.catn_node_compress, int(id), int(bool(opt)))
+
+    def cut_bondim(self, i, idx_j_in_i, opt=True):
+        return self._err(self.ctx.lib.catn_cut_bondim, int(i), int(idx_j_in_i), int(bool(opt)))
+
+    def cano_to(self, id, idx):
+        self.ctx._check(self.ctx.lib.catn_node_cano_to(self.h, int(id), int(idx)))
+
+    def left_canonical(self, id):
+        self.ctx._check(self.ctx.lib.catn_node_left_canonical(self.h, int(id)))
+
+    def reverse(self, id):
+        self.ctx._check(self.ctx.lib.catn_node_reverse(self.h, int(id)))
+
+    def eat(self, i, j, idx, idxi):
+        ln, err, ph = C.c_double(), C.c_double(), C.c_double()
+        self.ctx._check(self.ctx.lib.catn_node_eat(self.h, int(i), int(j), int(idx), int(idxi),
+                                                   C.byref(ln), C.byref(err), C.byref(ph)))
+        return ln.value, err.value, ph.value
+
+    def node(self, id):
+        """(order, cano, neighbor list, site dims list)"""
+        o, c = C.c_int32(), C.c_int32()
+        self.ctx._check(self.ctx.lib.catn_node_get(self.h, int(id), C.byref(o), C.byref(c),
+                                                   None, None, 0))
+        cap = max(o.value, 1)
+        nb = np.zeros(cap + 8, dtype=np.int32)
+        dims = np.zeros(3 * (cap + 8), dtype=np.int32)
+        self.ctx._check(self.ctx.lib.catn_node_get(self.h, int(id), C.byref(o), C.byref(c),
+                                                   nb.ctypes.data_as(_I32P),
+                                                   dims.ctypes.data_as(_I32P), cap + 8))
+        # neighbour count can differ from the order transiently (never at API boundaries)
+        return o.value, c.value, nb, [tuple(dims[3 * k: 3 * k + 3]) for k in range(o.value)]
+
+    def neighbor(self, id, count):
+        return [int(x) for x in self.node(id)[2][:count]]
+
+    def sites(self, id):
+        o, c, _, dims = self.node(id)
+        out = []
+        for k, (dl, d, dr) in enumerate(dims):
+            buf = np.zeros(int(dl) * int(d) * int(dr))
+            self.ctx._check(self.ctx.lib.catn_node_site(self.h, int(id), k + 1, _dptr(buf), buf.size))
+            out.append(buf.reshape((int(dl), int(d), int(dr)), order="F"))
+        return out

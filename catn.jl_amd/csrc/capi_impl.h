@@ -1,0 +1,699 @@
+// Host side of libcatn_hip: network construction (ising_network,
+// TensorNetwork), the contraction! driver loop around the planner, the
+// MPSNode primitive entry points and tsvd. Shared by every executor; the
+// executor only moves bytes and runs programs on the device.
+#pragma once
+#include <stdio.h>
+#include <string.h>
+
+#include <cmath>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/catn_hip.h"
+#include "ops.h"
+#include "planner.h"
+
+namespace catn {
+
+class Executor {
+ public:
+  virtual ~Executor() = default;
+  virtual const char* name() const = 0;
+  virtual void* dmalloc(size_t bytes) = 0;
+  virtual void dfree(void* p) = 0;
+  virtual void h2d(void* dst, const void* src, size_t bytes) = 0;
+  virtual void d2h(void* dst, const void* src, size_t bytes) = 0;
+  // Execute programs[k] on the instance whose device state is states[k]; one
+  // workgroup per instance; returns after completion.
+  virtual void run(const std::vector<InstState*>& states,
+                   const std::vector<const std::vector<Op>*>& programs) = 0;
+};
+
+Executor* make_executor(int device);   // provided by the executor TU
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& s) : std::runtime_error(s), code(c) {}
+};
+
+}  // namespace catn
+
+struct catn_ctx {
+  std::unique_ptr<catn::Executor> ex;
+};
+
+struct catn_net {
+  catn_ctx* ctx = nullptr;
+  catn::Planner pl;
+  int n = 0;             // tn.n
+  double beta = 1.0;     // tn.beta
+  int num_isolated = 0;
+  bool consumed = false;
+  // host staging of site tensors not yet on the device
+  struct Staged { int slot; int dl, d, dr; std::vector<double> data; };
+  std::vector<Staged> staged;
+  int nslots = 0;
+  // device state
+  catn::InstState hs{};          // host mirror
+  catn::InstState* ds = nullptr; // device copy
+  int slot_cap = 0, res_cap = 0;
+  int64_t arena_cap = 0, scratch_cap = 0;
+  int64_t arena_hwm = 0;
+
+  ~catn_net() { release(); }
+
+  void release() {
+    if (!ctx || !ds) return;
+    catn::Executor* ex = ctx->ex.get();
+    ex->dfree(hs.arena); ex->dfree(hs.scratch); ex->dfree(hs.sites);
+    ex->dfree(hs.results); ex->dfree(hs.dresults); ex->dfree(ds);
+    ds = nullptr;
+  }
+
+  int new_slot(int dl, int d, int dr, const double* data) {
+    Staged s;
+    s.slot = nslots++;
+    s.dl = dl; s.d = d; s.dr = dr;
+    const size_t cnt = (size_t)dl * d * dr;
+    if (data) s.data.assign(data, data + cnt);
+    else s.data.clear();   // placeholder slot, descriptor written by a device op
+    staged.push_back(std::move(s));
+    return staged.back().slot;
+  }
+
+  void ensure_device() {
+    catn::Executor* ex = ctx->ex.get();
+    if (!ds) {
+      int64_t staged_elems = 0;
+      for (auto& s : staged) staged_elems += ((int64_t)s.data.size() + 31) / 32 * 32;
+      if (arena_cap <= 0) arena_cap = std::max<int64_t>(int64_t(1) << 24, 4 * staged_elems);
+      if (scratch_cap <= 0) scratch_cap = int64_t(1) << 22;
+      arena_cap = std::max<int64_t>(arena_cap, 2 * staged_elems + 1024);
+      slot_cap = std::max(1024, 2 * nslots);
+      res_cap = 4096;
+      hs = catn::InstState{};
+      hs.psi = 1.0; hs.sign = 1.0;
+      hs.arena_cap = arena_cap;
+      hs.scratch_cap = scratch_cap;
+      hs.arena = (double*)ex->dmalloc(sizeof(double) * arena_cap);
+      hs.scratch = (double*)ex->dmalloc(sizeof(double) * scratch_cap);
+      hs.sites = (catn::SiteDesc*)ex->dmalloc(sizeof(catn::SiteDesc) * slot_cap);
+      hs.results = (int32_t*)ex->dmalloc(sizeof(int32_t) * res_cap);
+      hs.dresults = (double*)ex->dmalloc(sizeof(double) * res_cap);
+      hs.nsites = slot_cap;
+      hs.nresults = res_cap;
+      ds = (catn::InstState*)ex->dmalloc(sizeof(catn::InstState));
+      std::vector<catn::SiteDesc> zero(slot_cap);
+      for (auto& z : zero) { z.off = 0; z.dl = z.d = z.dr = 0; z.live = 0; }
+      ex->h2d(hs.sites, zero.data(), sizeof(catn::SiteDesc) * slot_cap);
+    }
+    if (nslots > slot_cap) throw catn::Error(-3, "slot table capacity exceeded");
+    if (pl.nresults > res_cap) throw catn::Error(-3, "result table capacity exceeded");
+    if (!staged.empty()) {
+      // pack staged data behind the current arena top
+      std::vector<double> buf;
+      std::vector<std::pair<int, catn::SiteDesc>> descs;
+      int64_t top = (hs.arena_top + 31) & ~int64_t(31);
+      const int64_t base = top;
+      for (auto& s : staged) {
+        catn::SiteDesc d{};
+        d.dl = s.dl; d.d = s.d; d.dr = s.dr; d.live = s.data.empty() ? 0 : 1;
+        if (!s.data.empty()) {
+          d.off = top;
+          buf.resize((size_t)(top - base), 0.0);
+          buf.insert(buf.end(), s.data.begin(), s.data.end());
+          top = (top + (int64_t)s.data.size() + 31) & ~int64_t(31);
+        }
+        descs.push_back({s.slot, d});
+      }
+      if (top > arena_cap) throw catn::Error(-4, "arena capacity exceeded while uploading sites");
+      if (!buf.empty()) ex->h2d(hs.arena + base, buf.data(), sizeof(double) * buf.size());
+      for (auto& pr : descs) ex->h2d(hs.sites + pr.first, &pr.second, sizeof(catn::SiteDesc));
+      hs.arena_top = top;
+      staged.clear();
+    }
+    ex->h2d(ds, &hs, sizeof(catn::InstState));
+  }
+
+  void pull_state() {
+    ctx->ex->d2h(&hs, ds, sizeof(catn::InstState));
+    arena_hwm = std::max<int64_t>(arena_hwm, hs.arena_top);
+  }
+
+  void check_status() {
+    if (hs.status != catn::STATUS_OK) {
+      static const char* names[] = {"ok", "arena full", "scratch full", "shape mismatch",
+                                    "SVD did not converge", "unsupported (randomized SVD branch)"};
+      const int s = hs.status;
+      std::string msg = "device op failed: ";
+      msg += (s >= 0 && s <= 5) ? names[s] : "unknown";
+      msg += " at op " + std::to_string(hs.fail_op);
+      throw catn::Error(-10 - s, msg);
+    }
+  }
+
+  void reset_acc() {
+    hs.lnZ = 0.0; hs.err = 0.0; hs.psi = 1.0; hs.sign = 1.0; hs.lntail = 0.0;
+    if (ds) ctx->ex->h2d(ds, &hs, sizeof(catn::InstState));
+  }
+
+  int read_result(int idx) {
+    int32_t v = 0;
+    ctx->ex->d2h(&v, hs.results + idx, sizeof(int32_t));
+    return v;
+  }
+
+  catn::SiteDesc read_desc(int slot) {
+    catn::SiteDesc d{};
+    ctx->ex->d2h(&d, hs.sites + slot, sizeof(d));
+    return d;
+  }
+};
+
+namespace catn {
+
+inline thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+// run the pending programs of several networks in one launch
+inline void run_nets(Executor* ex, const std::vector<catn_net*>& nets) {
+  std::vector<InstState*> st;
+  std::vector<const std::vector<Op>*> pr;
+  for (catn_net* n : nets) {
+    n->ensure_device();
+    st.push_back(n->ds);
+    pr.push_back(&n->pl.prog);
+  }
+  ex->run(st, pr);
+  for (catn_net* n : nets) {
+    n->pull_state();
+    n->pl.prog.clear();
+    n->check_status();
+  }
+}
+
+inline void run_one(catn_net* n) { run_nets(n->ctx->ex.get(), {n}); }
+
+// ising_network site construction (src/ising.jl:20-227)
+inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, const double* w,
+                        const double* h, double beta, int pos1, int val1, int pos2, int val2) {
+  std::vector<int> degree(n + 1, 0);
+  for (int e = 0; e < nedges; ++e) {
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    if (i < 1 || j < 1 || i > n || j > n) throw Error(-2, "ising_network: edge endpoint out of range");
+    degree[i]++;
+    degree[j]++;
+  }
+  for (int k = 1; k <= n; ++k) net->pl.nodes[k] = PNode{};
+  for (int e = 0; e < nedges; ++e) {
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    double si[2] = {1.0, -1.0}, sj[2] = {1.0, -1.0};
+    if (i == pos1) si[val1 - 1] = 0.0;
+    else if (i == pos2) si[val2 - 1] = 0.0;
+    if (j == pos1) sj[val1 - 1] = 0.0;
+    else if (j == pos2) sj[val2 - 1] = 0.0;
+    const double we = w[e];
+    double B[2][2], Rm[2][2] = {{1.0, 0.0}, {0.0, 1.0}};
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) B[a][b] = std::exp(beta * we * (si[a] * sj[b]));
+    for (int side = 0; side < 2; ++side) {
+      const int id = side == 0 ? i : j;
+      const int other = side == 0 ? j : i;
+      const double* s = side == 0 ? si : sj;
+      double (*mat)[2] = side == 0 ? B : Rm;
+      PNode& nd = net->pl.nodes[id];
+      const double hv = h[id - 1];
+      const double f[2] = {std::exp(beta * hv * s[0]), std::exp(beta * hv * s[1])};
+      double site[8];
+      int dl, d, dr;
+      if (degree[id] == 1) {            // leaf: column sums of field*Q
+        site[0] = f[0] * mat[0][0] + f[1] * mat[1][0];
+        site[1] = f[0] * mat[0][1] + f[1] * mat[1][1];
+        dl = 1; d = 2; dr = 1;
+      } else if (nd.nb.empty()) {       // first: (field*Q)' reshaped (1,2,2)
+        // site[1, a, b] = (fQ)[b, a]  -> linear a + 2b
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b) site[a + 2 * b] = f[b] * mat[b][a];
+        dl = 1; d = 2; dr = 2;
+      } else if ((int)nd.nb.size() == degree[id] - 1) {   // last: Q reshaped (2,2,1)
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b) site[a + 2 * b] = mat[a][b];
+        dl = 2; d = 2; dr = 1;
+      } else {                          // interior: t3[a, s, b] = delta_ab Q[a, s]
+        for (int q = 0; q < 8; ++q) site[q] = 0.0;
+        for (int a = 0; a < 2; ++a)
+          for (int sv = 0; sv < 2; ++sv) site[a + 2 * sv + 4 * a] = mat[a][sv];
+        dl = 2; d = 2; dr = 2;
+      }
+      const int slot = net->new_slot(dl, d, dr, site);
+      nd.slot.push_back(slot);
+      nd.phys.push_back(d);
+      nd.nb.push_back(other);
+    }
+  }
+  for (int k = 1; k <= n; ++k) {
+    if (degree[k] == 0) {
+      const double hv = h[k - 1];
+      const double v = std::exp(beta * hv) + std::exp(-beta * hv);
+      const int slot = net->new_slot(1, 1, 1, &v);
+      net->pl.nodes[k].slot.push_back(slot);
+      net->pl.nodes[k].phys.push_back(1);
+    }
+  }
+  int iso = 0;
+  for (auto& kv : net->pl.nodes) {
+    kv.second.cano = kv.second.order();
+    if (kv.second.nb.empty()) ++iso;
+  }
+  net->num_isolated = iso;
+  net->n = n;
+  net->beta = beta;
+}
+
+// raw2mps emitted as device split ops (src/mps_node.jl:31-50)
+inline void add_tensor_node(catn_net* net, int id, int nd, const int64_t* dims, const double* data,
+                            const std::vector<int>& nb) {
+  PNode node;
+  node.nb = nb;
+  if (nd == 0) {
+    // empty MPS (src/mps_node.jl:34)
+  } else if (nd == 1) {
+    const int slot = net->new_slot(1, (int)dims[0], 1, data);
+    node.slot.push_back(slot);
+    node.phys.push_back((int)dims[0]);
+  } else {
+    int64_t total = 1;
+    for (int k = 0; k < nd; ++k) total *= dims[k];
+    const int rem = net->new_slot(1, (int)total, 1, data);
+    for (int k = 0; k < nd - 1; ++k) {
+      const int s = net->new_slot(0, 0, 0, nullptr);
+      net->pl.emit(OP_RAW_SPLIT, rem, s, (int)dims[k], net->pl.p.chi, 0, 0, net->pl.p.cutoff);
+      node.slot.push_back(s);
+      node.phys.push_back((int)dims[k]);
+    }
+    node.slot.push_back(rem);
+    node.phys.push_back((int)dims[nd - 1]);
+  }
+  node.cano = node.order();
+  net->pl.nodes[id] = node;
+}
+
+inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
+  for (catn_net* n : nets) {
+    if (n->consumed) throw Error(-5, "contraction!: network already contracted");
+    n->ensure_device();
+    n->reset_acc();
+    n->pl.start();
+  }
+  std::vector<catn_net*> active = nets;
+  std::vector<int> need(nets.size());
+  while (!active.empty()) {
+    std::vector<catn_net*> still;
+    std::vector<int> cut;
+    for (catn_net* n : active) {
+      const Planner::StepResult r = n->pl.step();
+      cut.push_back(r == Planner::NEED_CUT ? 1 : 0);
+    }
+    run_nets(ex, active);
+    for (size_t k = 0; k < active.size(); ++k) {
+      catn_net* n = active[k];
+      if (cut[k]) {
+        const int myd = n->read_result(n->pl.pending_cut_result());
+        n->pl.apply_cut(myd);
+        still.push_back(n);
+      } else {
+        n->consumed = true;
+      }
+    }
+    active.swap(still);
+  }
+}
+
+}  // namespace catn
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define CATN_TRY try {
+#define CATN_CATCH                                                      \
+  }                                                                     \
+  catch (const catn::Error& e) { return catn::fail(e.code, e.what()); } \
+  catch (const catn::PlanError& e) { return catn::fail(-2, e.what()); } \
+  catch (const std::exception& e) { return catn::fail(-1, e.what()); }
+
+extern "C" {
+
+void catn_params_default(catn_params* p) {
+  p->Dmax = 32; p->chi = 32; p->cutoff = 1e-15; p->norm_method = 1; p->select = 1;
+  p->reverse = 1; p->svdopt = 1; p->swapopt = 1; p->compress = 0; p->cut_bond = 0;
+  p->seed = 1; p->arena_elems = 0; p->scratch_elems = 0;
+}
+
+const char* catn_last_error(void) { return catn::g_last_error.c_str(); }
+
+int catn_create(catn_ctx** ctx, int device) {
+  CATN_TRY
+  auto c = new catn_ctx;
+  c->ex.reset(catn::make_executor(device));
+  *ctx = c;
+  return 0;
+  CATN_CATCH
+}
+
+int catn_destroy(catn_ctx* ctx) {
+  delete ctx;
+  return 0;
+}
+
+const char* catn_backend(catn_ctx* ctx) { return ctx->ex->name(); }
+
+static void apply_params(catn_net* net, const catn_params* p) {
+  catn_params d;
+  catn_params_default(&d);
+  if (!p) p = &d;
+  catn::Params& q = net->pl.p;
+  q.Dmax = p->Dmax; q.chi = p->chi; q.cutoff = p->cutoff; q.norm_method = p->norm_method;
+  q.select = p->select; q.reverse = p->reverse != 0; q.svdopt = p->svdopt != 0;
+  q.swapopt = p->swapopt != 0; q.compress = p->compress != 0; q.cut_bond = p->cut_bond != 0;
+  q.seed = p->seed;
+  net->arena_cap = p->arena_elems;
+  net->scratch_cap = p->scratch_elems;
+}
+
+int catn_ising_network(catn_ctx* ctx, int32_t n, int32_t nedges, const int32_t* edges,
+                       const double* w, const double* h, double beta, int32_t pos1,
+                       int32_t val1, int32_t pos2, int32_t val2, const catn_params* p,
+                       catn_net** out) {
+  CATN_TRY
+  std::unique_ptr<catn_net> net(new catn_net);
+  net->ctx = ctx;
+  apply_params(net.get(), p);
+  catn::build_ising(net.get(), n, nedges, edges, w, h, beta, pos1, val1, pos2, val2);
+  *out = net.release();
+  return 0;
+  CATN_CATCH
+}
+
+int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
+                        const int64_t* dims, const double* data, const int64_t* labels,
+                        const catn_params* p, catn_net** out) {
+  CATN_TRY
+  std::unique_ptr<catn_net> net(new catn_net);
+  net->ctx = ctx;
+  apply_params(net.get(), p);
+  // label -> occurrences (src/tensor_network.jl:59-102)
+  std::map<int64_t, std::vector<std::pair<int, int>>> occ;
+  std::vector<int64_t> first_seen;
+  size_t li = 0;
+  for (int t = 0; t < ntensors; ++t) {
+    std::set<int64_t> seen;
+    for (int k = 0; k < ndims[t]; ++k) {
+      const int64_t l = labels[li + k];
+      if (seen.count(l))
+        throw catn::Error(-2, "Self-loop detected: label " + std::to_string(l) +
+                                  " appears twice in tensor " + std::to_string(t + 1) + "'s index list");
+      seen.insert(l);
+      if (!occ.count(l)) first_seen.push_back(l);
+      occ[l].push_back({t + 1, k});
+    }
+    li += ndims[t];
+  }
+  for (auto& kv : occ)
+    if (kv.second.size() > 2)
+      throw catn::Error(-2, "Label " + std::to_string(kv.first) + " appears in " +
+                                std::to_string(kv.second.size()) + " tensors (max 2 allowed)");
+  std::vector<std::vector<int>> nbv(ntensors);
+  for (int t = 0; t < ntensors; ++t) nbv[t].assign(ndims[t], 0);
+  int sentinel = 0;
+  for (int64_t l : first_seen) {
+    auto& o = occ[l];
+    if (o.size() == 2) {
+      nbv[o[0].first - 1][o[0].second] = o[1].first;
+      nbv[o[1].first - 1][o[1].second] = o[0].first;
+    } else {
+      nbv[o[0].first - 1][o[0].second] = --sentinel;
+    }
+  }
+  size_t di = 0, dd = 0;
+  int iso = 0;
+  for (int t = 0; t < ntensors; ++t) {
+    int64_t total = 1;
+    for (int k = 0; k < ndims[t]; ++k) total *= dims[di + k];
+    catn::add_tensor_node(net.get(), t + 1, ndims[t], dims + di, data + dd, nbv[t]);
+    bool all_neg = true;
+    for (int v : nbv[t]) all_neg = all_neg && v < 0;
+    if (all_neg) ++iso;
+    di += ndims[t];
+    dd += (size_t)total;
+  }
+  net->num_isolated = iso;
+  net->n = ntensors;
+  net->beta = 1.0;
+  if (!net->pl.prog.empty()) catn::run_one(net.get());
+  *out = net.release();
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_create(catn_ctx* ctx, const catn_params* p, catn_net** out) {
+  CATN_TRY
+  std::unique_ptr<catn_net> net(new catn_net);
+  net->ctx = ctx;
+  apply_params(net.get(), p);
+  *out = net.release();
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_add_node(catn_net* net, int32_t id, int32_t nsites, const int32_t* dims,
+                      const double* data, const int32_t* neighbor, int32_t cano) {
+  CATN_TRY
+  catn::PNode node;
+  size_t off = 0;
+  for (int k = 0; k < nsites; ++k) {
+    const int dl = dims[3 * k], d = dims[3 * k + 1], dr = dims[3 * k + 2];
+    node.slot.push_back(net->new_slot(dl, d, dr, data + off));
+    node.phys.push_back(d);
+    node.nb.push_back(neighbor[k]);
+    off += (size_t)dl * d * dr;
+  }
+  node.cano = cano;
+  net->pl.nodes[id] = node;
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_add_tensor_node(catn_net* net, int32_t id, int32_t ndim, const int64_t* dims,
+                             const double* data, const int32_t* neighbor) {
+  CATN_TRY
+  std::vector<int> nb(neighbor, neighbor + ndim);
+  catn::add_tensor_node(net, id, ndim, dims, data, nb);
+  if (!net->pl.prog.empty()) catn::run_one(net);
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_free(catn_net* net) {
+  delete net;
+  return 0;
+}
+
+int catn_contraction(catn_net* net, double* lnZ, double* error, double* psi) {
+  CATN_TRY
+  catn::contract_nets(net->ctx->ex.get(), {net});
+  if (lnZ) *lnZ = net->hs.lnZ + net->hs.lntail;
+  if (error) *error = net->hs.err;
+  if (psi) *psi = net->hs.psi;
+  return 0;
+  CATN_CATCH
+}
+
+int catn_contraction_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, double* lnZ,
+                           double* error, double* psi) {
+  CATN_TRY
+  std::vector<catn_net*> v(nets, nets + count);
+  catn::contract_nets(ctx->ex.get(), v);
+  for (int k = 0; k < count; ++k) {
+    if (lnZ) lnZ[k] = v[k]->hs.lnZ + v[k]->hs.lntail;
+    if (error) error[k] = v[k]->hs.err;
+    if (psi) psi[k] = v[k]->hs.psi;
+  }
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_info(catn_net* net, double* sign, int32_t* maxdim, int32_t* num_isolated, int32_t* n,
+                  double* beta) {
+  if (sign) *sign = net->hs.sign;
+  if (maxdim) *maxdim = net->pl.maxdim_intermediate;
+  if (num_isolated) *num_isolated = net->num_isolated;
+  if (n) *n = net->n;
+  if (beta) *beta = net->beta;
+  return 0;
+}
+
+int catn_net_edge_log(catn_net* net, int32_t* pairs, int32_t* count) {
+  const int cap = *count;
+  const int tot = (int)net->pl.edge_log.size();
+  for (int k = 0; k < tot && k < cap; ++k) {
+    pairs[2 * k] = net->pl.edge_log[k].first;
+    pairs[2 * k + 1] = net->pl.edge_log[k].second;
+  }
+  *count = tot;
+  return 0;
+}
+
+int catn_net_mem(catn_net* net, int64_t* used, int64_t* cap) {
+  if (used) *used = net->arena_hwm;
+  if (cap) *cap = net->arena_cap;
+  return 0;
+}
+
+#define NODE_OP(body)                      \
+  CATN_TRY                                 \
+  catn::PNode& nd = net->pl.node(id);      \
+  (void)nd;                                \
+  net->ensure_device();                    \
+  net->reset_acc();                        \
+  net->pl.acc_err = true;                  \
+  body;                                    \
+  catn::run_one(net);                      \
+  if (error) *error = net->hs.err;         \
+  return 0;                                \
+  CATN_CATCH
+
+int catn_node_swap(catn_net* net, int32_t id, int32_t i, int32_t j, double* error) {
+  NODE_OP(net->pl.swap(nd, i, j))
+}
+int catn_node_move(catn_net* net, int32_t id, int32_t a, int32_t b, double* error) {
+  NODE_OP(net->pl.move(nd, a, b))
+}
+int catn_node_move2tail(catn_net* net, int32_t id, int32_t idx, double* error) {
+  NODE_OP(net->pl.move2tail(nd, idx))
+}
+int catn_node_move2head(catn_net* net, int32_t id, int32_t idx, double* error) {
+  NODE_OP(net->pl.move2head(nd, idx))
+}
+int catn_node_merge(catn_net* net, int32_t id, int32_t j, int32_t cross, double* error) {
+  NODE_OP(net->pl.merge(nd, j, cross != 0))
+}
+int catn_node_compress(catn_net* net, int32_t id, int32_t opt, double* error) {
+  NODE_OP(net->pl.compress_node(nd, opt != 0))
+}
+int catn_node_cano_to(catn_net* net, int32_t id, int32_t idx) {
+  double* error = nullptr;
+  NODE_OP(net->pl.cano_to(nd, idx))
+}
+int catn_node_left_canonical(catn_net* net, int32_t id) {
+  double* error = nullptr;
+  NODE_OP(net->pl.left_canonical(nd))
+}
+int catn_node_reverse(catn_net* net, int32_t id) {
+  double* error = nullptr;
+  NODE_OP(net->pl.reverse(nd))
+}
+
+int catn_node_eat(catn_net* net, int32_t id_i, int32_t id_j, int32_t idx, int32_t idxi,
+                  double* lognorm, double* error, double* phase) {
+  CATN_TRY
+  catn::PNode& ni = net->pl.node(id_i);
+  catn::PNode& nj = net->pl.node(id_j);
+  net->ensure_device();
+  net->reset_acc();
+  net->pl.acc_err = true;
+  net->pl.eat(ni, nj, idx, idxi);
+  catn::run_one(net);
+  if (lognorm) *lognorm = net->hs.lnZ;
+  if (error) *error = net->hs.err;
+  if (phase) *phase = net->hs.psi;
+  return 0;
+  CATN_CATCH
+}
+
+int catn_cut_bondim(catn_net* net, int32_t i, int32_t idx_j_in_i, int32_t opt, double* error) {
+  CATN_TRY
+  if (net->pl.p.Dmax < 0) {   // no-op in exact mode (:343, :400)
+    if (error) *error = 0.0;
+    return 0;
+  }
+  net->ensure_device();
+  net->reset_acc();
+  net->pl.emit_cut(i, idx_j_in_i, opt != 0);
+  catn::run_one(net);
+  net->pl.apply_cut(net->read_result(net->pl.pending_cut_result()));
+  if (error) *error = net->hs.err;
+  return 0;
+  CATN_CATCH
+}
+
+int catn_node_get(catn_net* net, int32_t id, int32_t* order, int32_t* cano, int32_t* neighbor,
+                  int32_t* dims, int32_t cap) {
+  CATN_TRY
+  catn::PNode& nd = net->pl.node(id);
+  if (order) *order = nd.order();
+  if (cano) *cano = nd.cano;
+  if (neighbor)
+    for (int k = 0; k < (int)nd.nb.size() && k < cap; ++k) neighbor[k] = nd.nb[k];
+  if (dims) {
+    net->ensure_device();
+    for (int k = 0; k < nd.order() && k < cap; ++k) {
+      catn::SiteDesc d = net->read_desc(nd.slot[k]);
+      dims[3 * k] = d.dl; dims[3 * k + 1] = d.d; dims[3 * k + 2] = d.dr;
+    }
+  }
+  return 0;
+  CATN_CATCH
+}
+
+int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t cap) {
+  CATN_TRY
+  catn::PNode& nd = net->pl.node(id);
+  if (k < 1 || k > nd.order()) throw catn::Error(-2, "site index out of range");
+  net->ensure_device();
+  catn::SiteDesc d = net->read_desc(nd.slot[k - 1]);
+  const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
+  if (cnt > cap) throw catn::Error(-2, "output buffer too small");
+  net->ctx->ex->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
+  return 0;
+  CATN_CATCH
+}
+
+int catn_tsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, double cutoff, int32_t maxdim,
+              int32_t* k, double* U, double* S, double* V, double* discarded) {
+  CATN_TRY
+  catn_net net;
+  net.ctx = ctx;
+  catn_params p;
+  catn_params_default(&p);
+  apply_params(&net, &p);
+  const int64_t need = (int64_t)m * n;
+  net.arena_cap = 4 * need + 4 * ((int64_t)m + n) * std::min(m, n) + 4096;
+  net.scratch_cap = 8 * need + 8 * ((int64_t)std::max(m, n)) * std::max(m, n) + 4096;
+  const int a = net.new_slot(m, 1, n, A);
+  const int su = net.new_slot(0, 0, 0, nullptr);
+  const int sv = net.new_slot(0, 0, 0, nullptr);
+  const int ss = net.new_slot(0, 0, 0, nullptr);
+  net.pl.nresults = 1;
+  net.pl.emit(catn::OP_TSVD, a, su, sv, maxdim <= 0 ? (1 << 30) : maxdim, 0, ss, cutoff);
+  catn::run_one(&net);
+  const int kk = net.read_result(0);
+  double disc = 0.0;
+  ctx->ex->d2h(&disc, net.hs.dresults, sizeof(double));
+  catn::SiteDesc du = net.read_desc(su), dv = net.read_desc(sv), dsd = net.read_desc(ss);
+  if (U) ctx->ex->d2h(U, net.hs.arena + du.off, sizeof(double) * (size_t)m * kk);
+  if (V) ctx->ex->d2h(V, net.hs.arena + dv.off, sizeof(double) * (size_t)n * kk);
+  if (S) ctx->ex->d2h(S, net.hs.arena + dsd.off, sizeof(double) * (size_t)dsd.dl);
+  if (k) *k = kk;
+  if (discarded) *discarded = disc;
+  return 0;
+  CATN_CATCH
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// libcatn_hip: gfx950 executor of the contraction engine.
+//
+// One persistent kernel launch per planning segment: workgroup b runs the op
+// program of network b (a segment ends only where the host planner needs a
+// Dmax-cut kept count back from the device). Site tensors of each network live
+// in one device arena; descriptors (offset, dl, d, dr) are device-resident
+// because virtual bond dimensions are produced by the device SVDs.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "capi_impl.h"
+
+namespace catn {
+
+constexpr int kThreads = 256;
+
+__global__ __launch_bounds__(kThreads) void exec_programs(InstState* const* states,
+                                                          const Op* ops, const int* ranges) {
+  __shared__ double red[16];
+  __shared__ int flag;
+  WG g;
+  g.tid = threadIdx.x;
+  g.nthr = blockDim.x;
+  g.red = red;
+  g.flag = &flag;
+  const int b = blockIdx.x;
+  run_program(g, states[b], ops, ranges[2 * b], ranges[2 * b + 1]);
+}
+
+static void hipcheck(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw Error(-100, std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+class HipExecutor : public Executor {
+ public:
+  explicit HipExecutor(int device) : device_(device) {
+    int count = 0;
+    hipcheck(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (count <= 0) throw Error(-100, "no HIP device available");
+    hipcheck(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    hipcheck(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    name_ = std::string("hip-") + prop.gcnArchName;
+    hipcheck(hipStreamCreate(&stream_), "hipStreamCreate");
+  }
+  ~HipExecutor() override {
+    if (d_ops_) hipFree(d_ops_);
+    if (d_ranges_) hipFree(d_ranges_);
+    if (d_states_) hipFree(d_states_);
+    hipStreamDestroy(stream_);
+  }
+  const char* name() const override { return name_.c_str(); }
+  void* dmalloc(size_t bytes) override {
+    void* p = nullptr;
+    hipcheck(hipMalloc(&p, bytes < 8 ? 8 : bytes), "hipMalloc");
+    return p;
+  }
+  void dfree(void* p) override {
+    if (p) hipFree(p);
+  }
+  void h2d(void* dst, const void* src, size_t bytes) override {
+    hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "h2d");
+    hipcheck(hipStreamSynchronize(stream_), "h2d sync");
+  }
+  void d2h(void* dst, const void* src, size_t bytes) override {
+    hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
+    hipcheck(hipStreamSynchronize(stream_), "d2h sync");
+  }
+  void run(const std::vector<InstState*>& states,
+           const std::vector<const std::vector<Op>*>& programs) override {
+    const int nb = (int)states.size();
+    if (nb == 0) return;
+    std::vector<Op> all;
+    std::vector<int> ranges(2 * nb);
+    for (int b = 0; b < nb; ++b) {
+      ranges[2 * b] = (int)all.size();
+      all.insert(all.end(), programs[b]->begin(), programs[b]->end());
+      ranges[2 * b + 1] = (int)all.size();
+    }
+    if (all.empty()) return;
+    reserve(all.size(), nb);
+    hipcheck(hipMemcpyAsync(d_ops_, all.data(), sizeof(Op) * all.size(), hipMemcpyHostToDevice, stream_), "ops");
+    hipcheck(hipMemcpyAsync(d_ranges_, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice, stream_), "ranges");
+    hipcheck(hipMemcpyAsync(d_states_, states.data(), sizeof(InstState*) * nb, hipMemcpyHostToDevice, stream_), "states");
+    hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(kThreads), 0, stream_, d_states_, d_ops_, d_ranges_);
+    hipcheck(hipGetLastError(), "launch exec_programs");
+    hipcheck(hipStreamSynchronize(stream_), "exec_programs");
+  }
+
+ private:
+  void reserve(size_t nops, int nb) {
+    if (nops > ops_cap_) {
+      if (d_ops_) hipFree(d_ops_);
+      ops_cap_ = nops * 2;
+      hipcheck(hipMalloc(&d_ops_, sizeof(Op) * ops_cap_), "hipMalloc ops");
+    }
+    if ((size_t)nb > st_cap_) {
+      if (d_ranges_) hipFree(d_ranges_);
+      if (d_states_) hipFree(d_states_);
+      st_cap_ = (size_t)nb * 2;
+      hipcheck(hipMalloc(&d_ranges_, sizeof(int) * 2 * st_cap_), "hipMalloc ranges");
+      hipcheck(hipMalloc(&d_states_, sizeof(InstState*) * st_cap_), "hipMalloc states");
+    }
+  }
+  int device_;
+  std::string name_;
+  hipStream_t stream_ = nullptr;
+  Op* d_ops_ = nullptr;
+  int* d_ranges_ = nullptr;
+  InstState** d_states_ = nullptr;
+  size_t ops_cap_ = 0, st_cap_ = 0;
+};
+
+Executor* make_executor(int device) { return new HipExecutor(device); }
+
+}  // namespace catn

@@ -1,0 +1,1061 @@
+// Workgroup-level numerics of the MPS edge step and the op interpreter.
+//
+// Single source: compiled by hipcc for gfx950 (one workgroup executes one
+// instance's program; lanes cooperate through LDS and wavefront shuffles) and
+// by g++ for the CPU emulator used in tests (one "workgroup" of one thread,
+// barriers become no-ops). Every routine is written in terms of
+// (tid, nthr, barrier) so both builds run the same algorithm.
+//
+// Numerics (all FP64 here):
+//   * theta of swap!            ein"ijk,kab->iajb" without a materialised permute
+//   * truncated SVD (tsvd)       Householder QR for tall/wide operands, then a
+//                               one-sided (Hestenes) Jacobi SVD with round-robin
+//                               pair ordering; singular values carry absolute
+//                               error O(eps*||A||) like LAPACK gesdd, so the
+//                               reference's absolute cutoff keeps its meaning
+//   * folds, fuses, permutes, norms (streaming, coalesced over the leading index)
+#pragma once
+#include <math.h>
+#include <stdint.h>
+#include "ops.h"
+
+#if defined(__HIPCC__)
+#define CATN_D __device__ __forceinline__
+#define CATN_DN __device__ __noinline__
+#else
+#define CATN_D inline
+#define CATN_DN
+#endif
+
+namespace catn {
+
+// ---------------------------------------------------------------------------
+// workgroup context
+// ---------------------------------------------------------------------------
+struct WG {
+  int tid, nthr;
+  double* red;     // LDS, >= 16 doubles (one per wavefront)
+  int* flag;       // LDS int
+  CATN_D void sync() const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __syncthreads();
+#endif
+  }
+};
+
+CATN_D double wg_sum(const WG& g, double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = g.tid >> 6, l = g.tid & 63;
+  g.sync();
+  if (l == 0) g.red[w] = v;
+  g.sync();
+  double s = 0.0;
+  const int nw = (g.nthr + 63) >> 6;
+  for (int i = 0; i < nw; ++i) s += g.red[i];
+  return s;
+#else
+  (void)g;
+  return v;
+#endif
+}
+
+CATN_D double wg_max(const WG& g, double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int w = g.tid >> 6, l = g.tid & 63;
+  g.sync();
+  if (l == 0) g.red[w] = v;
+  g.sync();
+  double s = g.red[0];
+  const int nw = (g.nthr + 63) >> 6;
+  for (int i = 1; i < nw; ++i) s = fmax(s, g.red[i]);
+  return s;
+#else
+  (void)g;
+  return v;
+#endif
+}
+
+// Sum over an aligned group of L lanes (L a power of two <= 64). The xor
+// butterfly gives every lane the bitwise-identical sum (a+b == b+a), so
+// decisions taken on it are uniform across the group.
+CATN_D double grp_sum(double v, int L) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = L >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#else
+  (void)L;
+#endif
+  return v;
+}
+
+CATN_D int pow2floor(int x) {
+  int p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// scratch bump allocator (per op)
+// ---------------------------------------------------------------------------
+struct Scratch {
+  double* base;
+  int64_t cap, top;
+  bool fail;
+  CATN_D double* alloc(int64_t n) {
+    int64_t a = (top + 31) & ~int64_t(31);
+    if (a + n > cap) { fail = true; return base; }
+    top = a + n;
+    return base + a;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// small dense kernels
+// ---------------------------------------------------------------------------
+CATN_D void wg_copy(const WG& g, double* dst, const double* src, int64_t n) {
+  for (int64_t i = g.tid; i < n; i += g.nthr) dst[i] = src[i];
+}
+
+CATN_D void wg_fill(const WG& g, double* dst, double v, int64_t n) {
+  for (int64_t i = g.tid; i < n; i += g.nthr) dst[i] = v;
+}
+
+// C (m x n, ldc) = A (m x k, lda) * B (k x n, ldb); column-major.
+CATN_D void wg_gemm_nn(const WG& g, int m, int n, int k, const double* A, int64_t lda,
+                       const double* B, int64_t ldb, double* C, int64_t ldc) {
+  const int64_t tot = (int64_t)m * n;
+  for (int64_t idx = g.tid; idx < tot; idx += g.nthr) {
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    double s = 0.0;
+    const double* a = A + i;
+    const double* b = B + (int64_t)j * ldb;
+    for (int p = 0; p < k; ++p) s += a[(int64_t)p * lda] * b[p];
+    C[i + (int64_t)j * ldc] = s;
+  }
+}
+
+// transpose: dst (n x m) = src (m x n)^T
+CATN_D void wg_transpose(const WG& g, double* dst, const double* src, int m, int n) {
+  const int64_t tot = (int64_t)m * n;
+  for (int64_t idx = g.tid; idx < tot; idx += g.nthr) {
+    const int j = (int)(idx % n), i = (int)(idx / n);  // dst[j + n*i] = src[i + m*j]
+    dst[j + (int64_t)n * i] = src[i + (int64_t)m * j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR (LAPACK dgeqr2 reflectors: H = I - tau v v^T, v(0) = 1)
+// A (m x n, m >= n) overwritten by R (upper) and v (below the diagonal).
+// ---------------------------------------------------------------------------
+CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double* tau) {
+  for (int j = 0; j < n; ++j) {
+    double* x = A + j + (int64_t)j * lda;
+    double part = 0.0;
+    for (int i = 1 + g.tid; i < m - j; i += g.nthr) part += x[i] * x[i];
+    const double s = wg_sum(g, part);
+    const double alpha = x[0];
+    double t = 0.0, beta = alpha, scal = 0.0;
+    if (s > 0.0) {
+      const double nrm = sqrt(alpha * alpha + s);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      t = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    g.sync();
+    if (s > 0.0) {
+      for (int i = 1 + g.tid; i < m - j; i += g.nthr) x[i] *= scal;
+    }
+    if (g.tid == 0) { tau[j] = t; }
+    g.sync();
+    if (s > 0.0) {
+      if (g.tid == 0) x[0] = 1.0;   // temporarily, for the update
+      g.sync();
+      // apply H to the trailing columns: one group of lanes per column
+      const int ncol = n - j - 1;
+      if (ncol > 0) {
+        int L = pow2floor(g.nthr / ncol < 1 ? 1 : g.nthr / ncol);
+        if (L > 64) L = 64;
+        const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
+        for (int c = grp; c < ncol; c += ng) {
+          double* y = A + j + (int64_t)(j + 1 + c) * lda;
+          double w = 0.0;
+          for (int i = lane; i < m - j; i += L) w += x[i] * y[i];
+          w = grp_sum(w, L) * t;
+          for (int i = lane; i < m - j; i += L) y[i] -= w * x[i];
+        }
+      }
+      g.sync();
+      if (g.tid == 0) x[0] = beta;
+    }
+    g.sync();
+  }
+}
+
+// U (m x k) := Q * U where Q = H_0 ... H_{n-1} from wg_geqr2 (stored in A);
+// U is given as its top n x k block followed by zeros (rows n..m-1 set here).
+CATN_DN void wg_apply_q(const WG& g, const double* A, int m, int n, int64_t lda,
+                        const double* tau, double* U, int k, int64_t ldu) {
+  for (int64_t idx = g.tid; idx < (int64_t)(m - n) * k; idx += g.nthr) {
+    const int i = n + (int)(idx % (m - n)), c = (int)(idx / (m - n));
+    U[i + (int64_t)c * ldu] = 0.0;
+  }
+  g.sync();
+  int L = pow2floor(g.nthr / k < 1 ? 1 : g.nthr / k);
+  if (L > 64) L = 64;
+  const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
+  for (int j = n - 1; j >= 0; --j) {
+    const double t = tau[j];
+    if (t != 0.0) {
+      const double* v = A + j + (int64_t)j * lda;   // v[0] implicit 1
+      for (int c = grp; c < k; c += ng) {
+        double* y = U + j + (int64_t)c * ldu;
+        double w = 0.0;
+        for (int i = lane; i < m - j; i += L) w += (i == 0 ? 1.0 : v[i]) * y[i];
+        w = grp_sum(w, L) * t;
+        for (int i = lane; i < m - j; i += L) y[i] -= w * (i == 0 ? 1.0 : v[i]);
+      }
+    }
+    g.sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// one-sided Jacobi SVD: A (m x n, m >= n) -> A V with mutually orthogonal
+// columns; V (n x n) accumulates the rotations. Returns sweeps used, or -1
+// when not converged within maxsweeps.
+// ---------------------------------------------------------------------------
+CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double* V,
+                      int64_t ldv, int maxsweeps) {
+  for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    V[i + (int64_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+  }
+  g.sync();
+  if (n < 2) return 0;
+  const int N = n + (n & 1);
+  const int npairs = N / 2;
+  int L = g.nthr / npairs;
+  L = L < 1 ? 1 : pow2floor(L);
+  if (L > 64) L = 64;
+  const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
+  // pair convergence test on the cosine |a_p.a_q| / (|a_p||a_q|); a rotation
+  // below that level is below working precision and can repeat forever
+  const double tol = 4.0 * 2.220446049250313e-16 * sqrt((double)(m > 1 ? m : 1));
+  // columns below 1e-18 ||A||_F are rounding noise of a rank-deficient input:
+  // they cannot be made orthogonal (the column space is full) and only shrink
+  // towards denormals, so they are treated as zero columns
+  double fpart = 0.0;
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+    const double v = A[(idx % m) + (idx / m) * lda];
+    fpart += v * v;
+  }
+  const double thr = 1e-36 * wg_sum(g, fpart);
+  for (int sweep = 1; sweep <= maxsweeps; ++sweep) {
+    if (g.tid == 0) *g.flag = 0;
+    g.sync();
+    int rotated = 0;
+    for (int r = 0; r < N - 1; ++r) {
+      for (int k = grp; k < npairs; k += ng) {
+        int p, q;
+        if (k == 0) { p = r; q = N - 1; }
+        else { p = (r + k) % (N - 1); q = (r - k + N - 1) % (N - 1); }
+        if (p > q) { int t = p; p = q; q = t; }
+        if (q >= n) continue;
+        double* ap = A + (int64_t)p * lda;
+        double* aq = A + (int64_t)q * lda;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = lane; i < m; i += L) {
+          const double x = ap[i], y = aq[i];
+          al += x * x; be += y * y; ga += x * y;
+        }
+        al = grp_sum(al, L); be = grp_sum(be, L); ga = grp_sum(ga, L);
+        if (al <= thr || be <= thr) continue;
+        if (fabs(ga) <= tol * sqrt(al) * sqrt(be)) continue;
+        rotated = 1;
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+        for (int i = lane; i < m; i += L) {
+          const double x = ap[i], y = aq[i];
+          ap[i] = c * x - s * y;
+          aq[i] = s * x + c * y;
+        }
+        double* vp = V + (int64_t)p * ldv;
+        double* vq = V + (int64_t)q * ldv;
+        for (int i = lane; i < n; i += L) {
+          const double x = vp[i], y = vq[i];
+          vp[i] = c * x - s * y;
+          vq[i] = s * x + c * y;
+        }
+      }
+      g.sync();
+    }
+    if (rotated) *g.flag = 1;
+    g.sync();
+    const int any = *g.flag;
+    g.sync();
+    if (!any) return sweep;
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// full thin SVD: A (m x n, lda) is NOT modified. Outputs, sorted descending:
+// S[r], U (m x r, ld m), V (n x r, ld n), r = min(m, n). Returns r, or -1 on
+// scratch exhaustion / non-convergence (status set in *st).
+// ---------------------------------------------------------------------------
+struct SvdResult {
+  int r;
+  double* U;  // m x r
+  double* S;  // r
+  double* V;  // n x r
+};
+
+CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
+                        bool transpose_in, SvdResult& out, int* sweeps_used) {
+  // Works on B = A (m x n) or B = A^T when transpose_in (then the caller
+  // passes m, n of B). Requires m >= n. Produces U (m x n), S, V (n x n) of B.
+  double* W = sc.alloc((int64_t)m * n);
+  double* V = sc.alloc((int64_t)n * n);
+  double* S = sc.alloc(n);
+  double* U = sc.alloc((int64_t)m * n);
+  int* rank_tmp = nullptr;
+  (void)rank_tmp;
+  if (sc.fail) return -1;
+  // load B
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    W[idx] = transpose_in ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
+  }
+  g.sync();
+  const bool use_qr = m >= 2 * n && n > 0;
+  double* R = W;
+  int64_t ldr = m;
+  double* tau = nullptr;
+  if (use_qr) {
+    tau = sc.alloc(n);
+    R = sc.alloc((int64_t)n * n);
+    if (sc.fail) return -1;
+    wg_geqr2(g, W, m, n, m, tau);
+    for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
+      const int i = (int)(idx % n), j = (int)(idx / n);
+      R[idx] = (i <= j) ? W[i + (int64_t)j * m] : 0.0;
+    }
+    ldr = n;
+    g.sync();
+  }
+  const int rows = use_qr ? n : m;
+  const int sw = wg_jacobi(g, R, rows, n, ldr, V, n, 60);
+  if (sweeps_used) *sweeps_used = sw;
+  // singular values = column norms; stable descending order by rank sort
+  for (int j = g.tid; j < n; j += g.nthr) {
+    const double* c = R + (int64_t)j * ldr;
+    double s = 0.0;
+    for (int i = 0; i < rows; ++i) s += c[i] * c[i];
+    S[j] = sqrt(s);
+  }
+  g.sync();
+  double* Ss = sc.alloc(n);
+  double* Vs = sc.alloc((int64_t)n * n);
+  if (sc.fail) return -1;
+  // the sorted U (rows x n) is assembled into U's top block, V into Vs
+  for (int j = g.tid; j < n; j += g.nthr) {
+    const double sj = S[j];
+    int rk = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = S[i];
+      rk += (si > sj) || (si == sj && i < j);
+    }
+    Ss[rk] = sj;
+    const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+    const double* c = R + (int64_t)j * ldr;
+    double* u = U + (int64_t)rk * m;
+    if (sj > 0.0) {
+      for (int i = 0; i < rows; ++i) u[i] = c[i] * inv;
+    } else {
+      for (int i = 0; i < rows; ++i) u[i] = 0.0;
+    }
+    const double* v = V + (int64_t)j * n;
+    double* vs = Vs + (int64_t)rk * n;
+    for (int i = 0; i < n; ++i) vs[i] = v[i];
+  }
+  g.sync();
+  // columns with zero singular value: give them a unit vector orthogonal to
+  // nothing in particular (LAPACK returns some orthonormal completion; only
+  // the all-zero matrix keeps such a column)
+  for (int j = g.tid; j < n; j += g.nthr) {
+    if (Ss[j] == 0.0) {
+      double* u = U + (int64_t)j * m;
+      if (j < rows) u[j] = 1.0;
+    }
+  }
+  g.sync();
+  if (use_qr) {
+    wg_apply_q(g, W, m, n, m, tau, U, n, m);
+  }
+  out.r = n;
+  out.U = U;
+  out.S = Ss;
+  out.V = Vs;
+  return sw < 0 ? -1 : n;
+}
+
+// A (m x n, lda) -> SVD of A with U (m x r), V (n x r), r = min(m,n).
+CATN_DN int wg_svd(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
+                   SvdResult& out, int* sweeps_used) {
+  if (m >= n) return wg_svd_tall(g, sc, A, m, n, lda, false, out, sweeps_used);
+  // A^T = U' S V'^T  ->  A = V' S U'^T
+  SvdResult t;
+  const int r = wg_svd_tall(g, sc, A, n, m, lda, true, t, sweeps_used);
+  out.r = t.r;
+  out.U = t.V;   // m x r (ld m)
+  out.S = t.S;
+  out.V = t.U;   // n x r (ld n)
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// interpreter
+// ---------------------------------------------------------------------------
+struct Exec {
+  WG g;
+  InstState* st;
+  Scratch sc;
+  int64_t top;     // arena bump pointer (uniform register copy)
+  int status;
+
+  CATN_D SiteDesc site(int s) const {
+    volatile SiteDesc* p = st->sites + s;
+    SiteDesc d;
+    d.off = p->off; d.dl = p->dl; d.d = p->d; d.dr = p->dr; d.live = p->live;
+    return d;
+  }
+  CATN_D double* ptr(const SiteDesc& d) const { return st->arena + d.off; }
+  CATN_D int64_t alloc(int64_t n) {
+    int64_t a = (top + 31) & ~int64_t(31);
+    if (a + n > st->arena_cap) { status = STATUS_ARENA_FULL; return 0; }
+    top = a + n;
+    return a;
+  }
+  CATN_D void set_site(int s, int64_t off, int dl, int d, int dr) {
+    if (g.tid == 0) {
+      volatile SiteDesc* p = st->sites + s;
+      p->off = off; p->dl = dl; p->d = d; p->dr = dr; p->live = 1;
+    }
+  }
+  CATN_D void acc(volatile double* dst, double v) {
+    if (g.tid == 0) *dst = *dst + v;
+  }
+};
+
+// count kept singular values: tsvd rule (src/linalg_utils.jl:24-25)
+CATN_D int keep_tsvd(const double* S, int r, double cutoff, int maxdim) {
+  int c = 0;
+  for (int i = 0; i < r; ++i) c += (S[i] > cutoff);
+  if (c == 0) return 1;
+  if (c > maxdim) c = maxdim;
+  if (c > r) c = r;
+  return c;
+}
+
+// cano_to! rule: tsvd(cutoff=0) then keep S > node.cutoff, all if none
+// (src/mps_node.jl:79-82)
+CATN_D int keep_cano(const double* S, int r, double cutoff) {
+  int c0 = 0;
+  for (int i = 0; i < r; ++i) c0 += (S[i] > 0.0);
+  if (c0 == 0) c0 = 1;
+  int c = 0;
+  for (int i = 0; i < c0; ++i) c += (S[i] > cutoff);
+  return c == 0 ? c0 : c;
+}
+
+CATN_D double tail_sum(const double* S, int from, int r) {
+  double s = 0.0;
+  for (int i = from; i < r; ++i) s += S[i];
+  return s;
+}
+
+CATN_DN void op_cano_r(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc a = E.site(op.a), b = E.site(op.b);
+  const int m = a.dl * a.d, n = a.dr;
+  if (b.dl != n) { E.status = STATUS_SHAPE; return; }
+  SvdResult R;
+  int sw = 0;
+  if (wg_svd(g, E.sc, E.ptr(a), m, n, m, R, &sw) < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+  }
+  const int k = keep_cano(R.S, R.r, op.x);
+  const int64_t offa = E.alloc((int64_t)m * k);
+  const int64_t nb = (int64_t)b.d * b.dr;
+  const int64_t offb = E.alloc((int64_t)k * nb);
+  if (E.status) return;
+  double* na = E.st->arena + offa;
+  double* nbp = E.st->arena + offb;
+  wg_copy(g, na, R.U, (int64_t)m * k);
+  // new b[p, x] = sum_q S[p] V[q,p] b[q, x]
+  const double* bp = E.ptr(b);
+  for (int64_t idx = g.tid; idx < (int64_t)k * nb; idx += g.nthr) {
+    const int p = (int)(idx % k);
+    const int64_t x = idx / k;
+    double s = 0.0;
+    const double* v = R.V + (int64_t)p * n;
+    for (int q = 0; q < n; ++q) s += v[q] * bp[q + (int64_t)n * x];
+    nbp[idx] = R.S[p] * s;
+  }
+  E.set_site(op.a, offa, a.dl, a.d, k);
+  E.set_site(op.b, offb, k, b.d, b.dr);
+  g.sync();
+}
+
+CATN_DN void op_cano_l(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc a = E.site(op.a), b = E.site(op.b);   // b = site i-1
+  const int dl = a.dl, d = a.d, dr = a.dr;
+  if (b.dr != dl) { E.status = STATUS_SHAPE; return; }
+  const int m = d * dr, n = dl;
+  double* Mt = E.sc.alloc((int64_t)m * n);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  const double* ap = E.ptr(a);
+  // Mt[(y + d*z), x] = a[x, y, z]
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+    const int64_t row = idx % m;
+    const int x = (int)(idx / m);
+    const int y = (int)(row % d), z = (int)(row / d);
+    Mt[idx] = ap[x + (int64_t)dl * (y + (int64_t)d * z)];
+  }
+  g.sync();
+  SvdResult R;
+  int sw = 0;
+  if (wg_svd(g, E.sc, Mt, m, n, m, R, &sw) < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+  }
+  const int k = keep_cano(R.S, R.r, op.x);
+  const int64_t offa = E.alloc((int64_t)k * d * dr);
+  const int64_t offb = E.alloc((int64_t)b.dl * b.d * k);
+  if (E.status) return;
+  double* na = E.st->arena + offa;
+  double* nbp = E.st->arena + offb;
+  // new a[p, y, z] = U[y + d*z, p]
+  for (int64_t idx = g.tid; idx < (int64_t)k * m; idx += g.nthr) {
+    const int p = (int)(idx % k);
+    const int64_t row = idx / k;
+    na[idx] = R.U[row + (int64_t)m * p];
+  }
+  // new b[u, p] = sum_c b[u, c] * S[p] * V[c, p]   (b viewed (b.dl*b.d) x dl)
+  const double* bp = E.ptr(b);
+  const int64_t rb = (int64_t)b.dl * b.d;
+  for (int64_t idx = g.tid; idx < rb * k; idx += g.nthr) {
+    const int64_t u = idx % rb;
+    const int p = (int)(idx / rb);
+    double s = 0.0;
+    const double* v = R.V + (int64_t)p * n;
+    for (int c = 0; c < n; ++c) s += bp[u + rb * c] * v[c];
+    nbp[idx] = s * R.S[p];
+  }
+  E.set_site(op.a, offa, k, d, dr);
+  E.set_site(op.b, offb, b.dl, b.d, k);
+  g.sync();
+}
+
+CATN_DN void op_swap(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc L = E.site(op.a), Rr = E.site(op.b);
+  const int d0 = L.dl, d2 = L.d, dd = L.dr, d1 = Rr.d, d3 = Rr.dr;
+  if (Rr.dl != dd) { E.status = STATUS_SHAPE; return; }
+  const int m = d0 * d1, n = d2 * d3;
+  if (op.f && ((m > 7000 && n > 7000) || m > 20000 || n > 20000)) {
+    E.status = STATUS_UNSUPPORTED;   // randomized SVD branch (src/mps_node.jl:154-156)
+    return;
+  }
+  double* M = E.sc.alloc((int64_t)m * n);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  const double* tl = E.ptr(L);
+  const double* tr = E.ptr(Rr);
+  // M[(i + d0*a), (j + d2*b)] = sum_k tl[i, j, k] tr[k, a, b]
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+    const int64_t row = idx % m, col = idx / m;
+    const int i = (int)(row % d0), a = (int)(row / d0);
+    const int j = (int)(col % d2), b = (int)(col / d2);
+    const double* x = tl + i + (int64_t)d0 * j;
+    const double* y = tr + (int64_t)dd * (a + (int64_t)d1 * b);
+    double s = 0.0;
+    for (int k = 0; k < dd; ++k) s += x[(int64_t)d0 * d2 * k] * y[k];
+    M[idx] = s;
+  }
+  g.sync();
+  SvdResult R;
+  int sw = 0;
+  if (wg_svd(g, E.sc, M, m, n, m, R, &sw) < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+  }
+  const int k = keep_tsvd(R.S, R.r, op.x, op.d);
+  if (op.e) E.acc(&E.st->err, tail_sum(R.S, k, R.r));
+  const int64_t offl = E.alloc((int64_t)m * k);
+  const int64_t offr = E.alloc((int64_t)k * n);
+  if (E.status) return;
+  double* nl = E.st->arena + offl;
+  double* nr = E.st->arena + offr;
+  const bool right = op.c != 0;
+  for (int64_t idx = g.tid; idx < (int64_t)m * k; idx += g.nthr) {
+    const int p = (int)(idx / m);
+    nl[idx] = right ? R.U[idx] : R.U[idx] * R.S[p];
+  }
+  // right site [p, c] = (S[p]) V[c, p]
+  for (int64_t idx = g.tid; idx < (int64_t)k * n; idx += g.nthr) {
+    const int p = (int)(idx % k);
+    const int64_t c = idx / k;
+    const double v = R.V[c + (int64_t)n * p];
+    nr[idx] = right ? R.S[p] * v : v;
+  }
+  E.set_site(op.a, offl, d0, d1, k);
+  E.set_site(op.b, offr, k, d2, d3);
+  g.sync();
+}
+
+CATN_DN void op_permute321(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc a = E.site(op.a);
+  const int64_t tot = (int64_t)a.dl * a.d * a.dr;
+  const int64_t off = E.alloc(tot);
+  if (E.status) return;
+  const double* s = E.ptr(a);
+  double* dst = E.st->arena + off;
+  // dst[z, y, x] = s[x, y, z]; dst dims (dr, d, dl)
+  for (int64_t idx = g.tid; idx < tot; idx += g.nthr) {
+    const int z = (int)(idx % a.dr);
+    const int64_t rest = idx / a.dr;
+    const int y = (int)(rest % a.d), x = (int)(rest / a.d);
+    dst[idx] = s[x + (int64_t)a.dl * (y + (int64_t)a.d * z)];
+  }
+  E.set_site(op.a, off, a.dr, a.d, a.dl);
+  g.sync();
+}
+
+CATN_DN void op_eat_a(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc a = E.site(op.a), b = E.site(op.b);
+  const int64_t n = (int64_t)a.dl * a.d * a.dr;
+  if (n != (int64_t)b.dl * b.d * b.dr) { E.status = STATUS_SHAPE; return; }
+  const double* x = E.ptr(a);
+  const double* y = E.ptr(b);
+  double part = 0.0;
+  for (int64_t i = g.tid; i < n; i += g.nthr) part += x[i] * y[i];
+  const double r = wg_sum(g, part);
+  const double ar = fabs(r);
+  if (ar > op.x) {
+    E.acc(&E.st->lnZ, log(ar));
+    if (g.tid == 0) E.st->psi = E.st->psi * (r / ar);
+  }
+  if (g.tid == 0) E.st->sites[op.a].live = 0;
+  g.sync();
+}
+
+CATN_D double site_norm(const WG& g, const double* x, int64_t n, int method) {
+  if (method == 1) {
+    double part = 0.0;
+    for (int64_t i = g.tid; i < n; i += g.nthr) part += x[i] * x[i];
+    return sqrt(wg_sum(g, part));
+  } else if (method == 2) {
+    double part = 0.0;
+    for (int64_t i = g.tid; i < n; i += g.nthr) part = fmax(part, fabs(x[i]));
+    return wg_max(g, part);
+  }
+  return 1.0;
+}
+
+CATN_D void scale_normalize(Exec& E, int slot, const SiteDesc& s, int method, double cutoff) {
+  const WG& g = E.g;
+  double* x = E.ptr(s);
+  const int64_t n = (int64_t)s.dl * s.d * s.dr;
+  const double nrm = site_norm(g, x, n, method);
+  if (nrm <= cutoff) { g.sync(); return; }   // src/mps_node.jl:522-524, :569-571
+  const double inv = 1.0 / nrm;
+  for (int64_t i = g.tid; i < n; i += g.nthr) x[i] = x[i] * inv;
+  E.acc(&E.st->lnZ, log(nrm));
+  (void)slot;
+  g.sync();
+}
+
+CATN_DN void op_eat_b(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc p = E.site(op.a), t = E.site(op.b), j = E.site(op.c);
+  const int D = t.dl, d = t.d;
+  if (t.dr != 1 || j.d != d || p.dr != D) { E.status = STATUS_SHAPE; return; }
+  double* mat = E.sc.alloc(D);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  const double* tp = E.ptr(t);
+  const double* jp = E.ptr(j);
+  for (int i = g.tid; i < D; i += g.nthr) {
+    double s = 0.0;
+    for (int q = 0; q < d; ++q) s += tp[i + (int64_t)D * q] * jp[q];
+    mat[i] = s;
+  }
+  g.sync();
+  const int64_t rows = (int64_t)p.dl * p.d;
+  const int64_t off = E.alloc(rows);
+  if (E.status) return;
+  double* np_ = E.st->arena + off;
+  const double* pp = E.ptr(p);
+  for (int64_t r = g.tid; r < rows; r += g.nthr) {
+    double s = 0.0;
+    for (int q = 0; q < D; ++q) s += pp[r + rows * q] * mat[q];
+    np_[r] = s;
+  }
+  E.set_site(op.a, off, p.dl, p.d, 1);
+  if (g.tid == 0) { E.st->sites[op.b].live = 0; E.st->sites[op.c].live = 0; }
+  g.sync();
+  SiteDesc nn = E.site(op.a);
+  scale_normalize(E, op.a, nn, op.d, op.x);
+}
+
+CATN_DN void op_eat_c(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc p = E.site(op.a), t = E.site(op.b), h = E.site(op.c);
+  const int Dl = t.dl, d = t.d, Dr = h.dr;
+  if (t.dr != 1 || h.dl != 1 || h.d != d || p.dr != Dl) { E.status = STATUS_SHAPE; return; }
+  double* mat = E.sc.alloc((int64_t)Dl * Dr);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  wg_gemm_nn(g, Dl, Dr, d, E.ptr(t), Dl, E.ptr(h), d, mat, Dl);
+  g.sync();
+  const int64_t rows = (int64_t)p.dl * p.d;
+  const int64_t off = E.alloc(rows * Dr);
+  if (E.status) return;
+  wg_gemm_nn(g, (int)rows, Dr, Dl, E.ptr(p), rows, mat, Dl, E.st->arena + off, rows);
+  E.set_site(op.a, off, p.dl, p.d, Dr);
+  if (g.tid == 0) { E.st->sites[op.b].live = 0; E.st->sites[op.c].live = 0; }
+  g.sync();
+}
+
+CATN_DN void op_normalize(Exec& E, const Op& op) {
+  SiteDesc s = E.site(op.a);
+  scale_normalize(E, op.a, s, op.d, op.x);
+}
+
+CATN_DN void op_merge_fuse(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc a = E.site(op.a), b = E.site(op.b);
+  if (a.dr != b.dl) { E.status = STATUS_SHAPE; return; }
+  const int m = a.dl * a.d, n = b.d * b.dr, k = a.dr;
+  const int64_t off = E.alloc((int64_t)m * n);
+  if (E.status) return;
+  wg_gemm_nn(g, m, n, k, E.ptr(a), m, E.ptr(b), k, E.st->arena + off, m);
+  E.set_site(op.a, off, a.dl, a.d * b.d, b.dr);
+  if (g.tid == 0) E.st->sites[op.b].live = 0;
+  g.sync();
+}
+
+// thin QR of a tall m x n (m > n) matrix held in Wk (overwritten):
+// Q (m x n, ld m) and R (n x n, ld n) into fresh scratch
+CATN_DN bool wg_thin_qr(const WG& g, Scratch& sc, double* Wk, int m, int n, double** Qo, double** Ro) {
+  double* tau = sc.alloc(n);
+  double* R = sc.alloc((int64_t)n * n);
+  double* Q = sc.alloc((int64_t)m * n);
+  if (sc.fail) return false;
+  wg_geqr2(g, Wk, m, n, m, tau);
+  for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    R[idx] = (i <= j) ? Wk[i + (int64_t)j * m] : 0.0;
+    Q[i + (int64_t)j * m] = (i == j) ? 1.0 : 0.0;
+  }
+  g.sync();
+  wg_apply_q(g, Wk, m, n, m, tau, Q, n, m);
+  *Qo = Q;
+  *Ro = R;
+  return true;
+}
+
+CATN_DN void op_cut(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc A = E.site(op.a), B = E.site(op.b);
+  const int dal = A.dl, d = A.d, dar = A.dr, dbl = B.dl, dbr = B.dr;
+  if (B.d != d) { E.status = STATUS_SHAPE; return; }
+  const int mi = dal * dar, mj = dbl * dbr;
+  double* Mi = E.sc.alloc((int64_t)mi * d);
+  double* Mj = E.sc.alloc((int64_t)mj * d);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  // mat[a + dal*c, b] = A[a, b, c]   (permutedims(A,(1,3,2)) reshaped)
+  const double* ap = E.ptr(A);
+  const double* bp = E.ptr(B);
+  for (int64_t idx = g.tid; idx < (int64_t)mi * d; idx += g.nthr) {
+    const int64_t row = idx % mi;
+    const int b = (int)(idx / mi);
+    const int a = (int)(row % dal), c = (int)(row / dal);
+    Mi[idx] = ap[a + (int64_t)dal * (b + (int64_t)d * c)];
+  }
+  for (int64_t idx = g.tid; idx < (int64_t)mj * d; idx += g.nthr) {
+    const int64_t row = idx % mj;
+    const int b = (int)(idx / mj);
+    const int a = (int)(row % dbl), c = (int)(row / dbl);
+    Mj[idx] = bp[a + (int64_t)dbl * (b + (int64_t)d * c)];
+  }
+  g.sync();
+  const bool opt = op.f != 0;
+  const bool fl = opt && mi > d, fr = opt && mj > d;
+  double *Qi = nullptr, *Ri = Mi, *Qj = nullptr, *Rj = Mj;
+  int ri_rows = mi, rj_rows = mj;
+  if (fl) { if (!wg_thin_qr(g, E.sc, Mi, mi, d, &Qi, &Ri)) { E.status = STATUS_SCRATCH_FULL; return; } ri_rows = d; }
+  if (fr) { if (!wg_thin_qr(g, E.sc, Mj, mj, d, &Qj, &Rj)) { E.status = STATUS_SCRATCH_FULL; return; } rj_rows = d; }
+  // merged = ri * rj^T  (ri_rows x rj_rows), non-conjugating
+  double* Mg = E.sc.alloc((int64_t)ri_rows * rj_rows);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  const int ldri = fl ? d : mi, ldrj = fr ? d : mj;
+  for (int64_t idx = g.tid; idx < (int64_t)ri_rows * rj_rows; idx += g.nthr) {
+    const int i = (int)(idx % ri_rows), j = (int)(idx / ri_rows);
+    double s = 0.0;
+    for (int q = 0; q < d; ++q) s += Ri[i + (int64_t)ldri * q] * Rj[j + (int64_t)ldrj * q];
+    Mg[idx] = s;
+  }
+  g.sync();
+  SvdResult R;
+  int sw = 0;
+  if (wg_svd(g, E.sc, Mg, ri_rows, rj_rows, ri_rows, R, &sw) < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+  }
+  const int nk = keep_tsvd(R.S, R.r, op.x, 1 << 30);
+  int myd = nk < op.d ? nk : op.d;
+  if (myd == 0) myd = 1;
+  E.acc(&E.st->err, tail_sum(R.S, myd, R.r));
+  // mati' = U sqrt(S) (ri_rows x myd), matj' = V sqrt(S) (rj_rows x myd)
+  double* Ui = E.sc.alloc((int64_t)ri_rows * myd);
+  double* Uj = E.sc.alloc((int64_t)rj_rows * myd);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  for (int64_t idx = g.tid; idx < (int64_t)ri_rows * myd; idx += g.nthr) {
+    const int p = (int)(idx / ri_rows);
+    Ui[idx] = R.U[idx] * sqrt(R.S[p]);
+  }
+  for (int64_t idx = g.tid; idx < (int64_t)rj_rows * myd; idx += g.nthr) {
+    const int p = (int)(idx / rj_rows);
+    Uj[idx] = R.V[idx] * sqrt(R.S[p]);
+  }
+  g.sync();
+  double* Fi = Ui;
+  double* Fj = Uj;
+  if (fl) {
+    Fi = E.sc.alloc((int64_t)mi * myd);
+    if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+    wg_gemm_nn(g, mi, myd, d, Qi, mi, Ui, d, Fi, mi);
+  }
+  if (fr) {
+    Fj = E.sc.alloc((int64_t)mj * myd);
+    if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+    wg_gemm_nn(g, mj, myd, d, Qj, mj, Uj, d, Fj, mj);
+  }
+  g.sync();
+  const int64_t offa = E.alloc((int64_t)dal * myd * dar);
+  const int64_t offb = E.alloc((int64_t)dbl * myd * dbr);
+  if (E.status) return;
+  double* na = E.st->arena + offa;
+  double* nb = E.st->arena + offb;
+  // A'[a, p, c] = Fi[a + dal*c, p]
+  for (int64_t idx = g.tid; idx < (int64_t)dal * myd * dar; idx += g.nthr) {
+    const int a = (int)(idx % dal);
+    const int64_t rest = idx / dal;
+    const int p = (int)(rest % myd), c = (int)(rest / myd);
+    na[idx] = Fi[a + (int64_t)dal * c + (int64_t)mi * p];
+  }
+  for (int64_t idx = g.tid; idx < (int64_t)dbl * myd * dbr; idx += g.nthr) {
+    const int a = (int)(idx % dbl);
+    const int64_t rest = idx / dbl;
+    const int p = (int)(rest % myd), c = (int)(rest / myd);
+    nb[idx] = Fj[a + (int64_t)dbl * c + (int64_t)mj * p];
+  }
+  E.set_site(op.a, offa, dal, myd, dar);
+  E.set_site(op.b, offb, dbl, myd, dbr);
+  if (g.tid == 0) E.st->results[op.e] = myd;
+  g.sync();
+}
+
+// compress! / compress_opt! two-site step
+CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
+  const WG& g = E.g;
+  SiteDesc A = E.site(op.a), B = E.site(op.b);
+  const int d0 = A.dl, d1 = A.d, dd = A.dr, d2 = B.d, d3 = B.dr;
+  if (B.dl != dd) { E.status = STATUS_SHAPE; return; }
+  const int m = d0 * d1, n = d2 * d3;
+  SvdResult R;
+  int sw = 0;
+  double *Ql = nullptr, *Qr = nullptr;
+  bool fl = false, fr = false;
+  int rl = m, rr = n;
+  if (!opt) {
+    double* M = E.sc.alloc((int64_t)m * n);
+    if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+    wg_gemm_nn(g, m, n, dd, E.ptr(A), m, E.ptr(B), dd, M, m);
+    g.sync();
+    if (wg_svd(g, E.sc, M, m, n, m, R, &sw) < 0) {
+      E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+    }
+  } else {
+    // matl (m x dd), matr (dd x n)
+    double* Ml = E.sc.alloc((int64_t)m * dd);
+    double* Mr = E.sc.alloc((int64_t)n * dd);   // matr^T (n x dd)
+    if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+    wg_copy(g, Ml, E.ptr(A), (int64_t)m * dd);
+    wg_transpose(g, Mr, E.ptr(B), dd, n);
+    g.sync();
+    double* Rl = Ml;
+    double* Rr = Mr;   // Rr is n x dd (or dd x dd after QR)
+    fl = m > dd;
+    fr = dd < n;
+    if (fl) { if (!wg_thin_qr(g, E.sc, Ml, m, dd, &Ql, &Rl)) { E.status = STATUS_SCRATCH_FULL; return; } rl = dd; }
+    if (fr) { if (!wg_thin_qr(g, E.sc, Mr, n, dd, &Qr, &Rr)) { E.status = STATUS_SCRATCH_FULL; return; } rr = dd; }
+    // mat = Rl * Rr^T   (rl x rr)
+    double* M = E.sc.alloc((int64_t)rl * rr);
+    if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+    for (int64_t idx = g.tid; idx < (int64_t)rl * rr; idx += g.nthr) {
+      const int i = (int)(idx % rl), j = (int)(idx / rl);
+      double s = 0.0;
+      for (int q = 0; q < dd; ++q) s += Rl[i + (int64_t)rl * q] * Rr[j + (int64_t)rr * q];
+      M[idx] = s;
+    }
+    g.sync();
+    if (wg_svd(g, E.sc, M, rl, rr, rl, R, &sw) < 0) {
+      E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+    }
+  }
+  const int k = keep_tsvd(R.S, R.r, op.x, op.d);
+  // left factor U*S (rl x k) -> Ql*(U S); right factor V (rr x k) -> Qr*V
+  double* Fl = E.sc.alloc((int64_t)rl * k);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  for (int64_t idx = g.tid; idx < (int64_t)rl * k; idx += g.nthr) {
+    const int p = (int)(idx / rl);
+    Fl[idx] = R.U[idx] * R.S[p];
+  }
+  g.sync();
+  double* Gl = Fl;
+  const double* Gr = R.V;
+  if (fl) {
+    Gl = E.sc.alloc((int64_t)m * k);
+    if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+    wg_gemm_nn(g, m, k, dd, Ql, m, Fl, dd, Gl, m);
+  }
+  if (fr) {
+    double* t = E.sc.alloc((int64_t)n * k);
+    if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+    wg_gemm_nn(g, n, k, dd, Qr, n, R.V, dd, t, n);
+    Gr = t;
+  }
+  g.sync();
+  const int64_t offa = E.alloc((int64_t)m * k);
+  const int64_t offb = E.alloc((int64_t)k * n);
+  if (E.status) return;
+  wg_copy(g, E.st->arena + offa, Gl, (int64_t)m * k);
+  wg_transpose(g, E.st->arena + offb, Gr, n, k);   // (k x n) = V^T
+  E.set_site(op.a, offa, d0, d1, k);
+  E.set_site(op.b, offb, k, d2, d3);
+  g.sync();
+}
+
+CATN_DN void op_raw_split(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc s = E.site(op.a);   // (dleft, rest, 1)
+  const int dleft = s.dl, rest = s.d, di = op.c;
+  if (rest % di != 0) { E.status = STATUS_SHAPE; return; }
+  const int m = dleft * di, n = rest / di;
+  SvdResult R;
+  int sw = 0;
+  if (wg_svd(g, E.sc, E.ptr(s), m, n, m, R, &sw) < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+  }
+  const int k = keep_tsvd(R.S, R.r, op.x, op.d);
+  const int64_t offu = E.alloc((int64_t)m * k);
+  const int64_t offr = E.alloc((int64_t)k * n);
+  if (E.status) return;
+  wg_copy(g, E.st->arena + offu, R.U, (int64_t)m * k);
+  double* nr = E.st->arena + offr;
+  for (int64_t idx = g.tid; idx < (int64_t)k * n; idx += g.nthr) {
+    const int p = (int)(idx % k);
+    const int64_t c = idx / k;
+    nr[idx] = R.S[p] * R.V[c + (int64_t)n * p];
+  }
+  E.set_site(op.b, offu, dleft, di, k);
+  E.set_site(op.a, offr, k, n, 1);
+  g.sync();
+}
+
+CATN_DN void op_scalar_lognorm(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc s = E.site(op.a);
+  const int64_t n = (int64_t)s.dl * s.d * s.dr;
+  const double* x = E.ptr(s);
+  double part = 0.0;
+  for (int64_t i = g.tid; i < n; i += g.nthr) part += x[i];
+  const double z = wg_sum(g, part);
+  E.acc(&E.st->lntail, log(fabs(z)));
+  if (g.tid == 0) E.st->sign = E.st->sign * (z > 0.0 ? 1.0 : (z < 0.0 ? -1.0 : 0.0));
+  g.sync();
+}
+
+CATN_DN void op_tsvd(Exec& E, const Op& op) {
+  const WG& g = E.g;
+  SiteDesc s = E.site(op.a);
+  const int m = s.dl * s.d, n = s.dr;
+  SvdResult R;
+  int sw = 0;
+  if (wg_svd(g, E.sc, E.ptr(s), m, n, m, R, &sw) < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
+  }
+  const int k = keep_tsvd(R.S, R.r, op.x, op.d);
+  const int64_t offu = E.alloc((int64_t)m * k);
+  const int64_t offv = E.alloc((int64_t)n * k);
+  const int64_t offs = E.alloc(R.r);
+  if (E.status) return;
+  wg_copy(g, E.st->arena + offu, R.U, (int64_t)m * k);
+  wg_copy(g, E.st->arena + offv, R.V, (int64_t)n * k);
+  wg_copy(g, E.st->arena + offs, R.S, R.r);
+  E.set_site(op.b, offu, m, 1, k);
+  E.set_site(op.c, offv, n, 1, k);
+  E.set_site(op.f, offs, R.r, 1, 1);
+  if (g.tid == 0) {
+    E.st->results[op.e] = k;
+    E.st->dresults[op.e] = tail_sum(R.S, k, R.r);
+  }
+  g.sync();
+}
+
+// Execute ops [begin, end) of one instance. Returns the status.
+CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, int end) {
+  Exec E;
+  E.g = g;
+  E.st = st;
+  E.top = st->arena_top;
+  E.status = st->status;
+  for (int i = begin; i < end && E.status == STATUS_OK; ++i) {
+    const Op op = ops[i];
+    E.sc.base = st->scratch;
+    E.sc.cap = st->scratch_cap;
+    E.sc.top = 0;
+    E.sc.fail = false;
+    switch (op.code) {
+      case OP_CANO_R: op_cano_r(E, op); break;
+      case OP_CANO_L: op_cano_l(E, op); break;
+      case OP_SWAP: op_swap(E, op); break;
+      case OP_PERMUTE321: op_permute321(E, op); break;
+      case OP_EAT_A: op_eat_a(E, op); break;
+      case OP_EAT_B: op_eat_b(E, op); break;
+      case OP_EAT_C: op_eat_c(E, op); break;
+      case OP_NORMALIZE: op_normalize(E, op); break;
+      case OP_MERGE_FUSE: op_merge_fuse(E, op); break;
+      case OP_CUT: op_cut(E, op); break;
+      case OP_COMPRESS_OPT: op_compress(E, op, true); break;
+      case OP_COMPRESS: op_compress(E, op, false); break;
+      case OP_RAW_SPLIT: op_raw_split(E, op); break;
+      case OP_SCALAR_LOGNORM: op_scalar_lognorm(E, op); break;
+      case OP_TSVD: op_tsvd(E, op); break;
+      default: break;
+    }
+    if (E.sc.fail && E.status == STATUS_OK) E.status = STATUS_SCRATCH_FULL;
+    if (E.status != STATUS_OK && g.tid == 0) st->fail_op = i;
+    g.sync();
+  }
+  if (g.tid == 0) {
+    st->arena_top = E.top;
+    st->status = E.status;
+  }
+  g.sync();
+  return E.status;
+}
+
+}  // namespace catn

@@ -1,0 +1,104 @@
+// Op records of the per-instance programs executed by the device interpreter.
+//
+// The host planner (planner.h) restates the integer control flow of
+// contraction! (src/tensor_network.jl:556-717) and of the MPSNode primitives
+// (src/mps_node.jl) and emits one record per numeric primitive. Site tensors
+// are addressed by slot ids; a slot's descriptor (arena offset, dl, d, dr)
+// lives in device memory because virtual bond dimensions are only known on
+// the device (they come out of truncated SVDs).
+#pragma once
+#include <stdint.h>
+
+namespace catn {
+
+enum OpCode : int32_t {
+  OP_NOP = 0,
+  // cano_to! single step to the right: SVD of site a as (dl*d) x dr, keep
+  // S > cutoff (all if none), U -> a, S*V' folded into site b
+  // (src/mps_node.jl:77-87)
+  OP_CANO_R = 1,
+  // cano_to! single step to the left: SVD of permutedims(a,(2,3,1)) as
+  // (d*dr) x dl; U -> a, (S*V')^T folded into site b (src/mps_node.jl:90-100)
+  OP_CANO_L = 2,
+  // swap! numerics (src/mps_node.jl:144-170): a = left slot (tl), b = right
+  // slot (tr), c = 1 if going right (S folded right) else 0, d = chi,
+  // e = 1 if the discarded weight counts into the returned error, f = swapopt
+  OP_SWAP = 3,
+  // reverse! per site: permutedims(t, (3,2,1)) (src/mps_node.jl:240)
+  OP_PERMUTE321 = 4,
+  // eat! case (a): both leaves, non-conjugating dot (src/mps_node.jl:469-489)
+  OP_EAT_A = 5,
+  // eat! case (b): a = site end-1, b = site end (contracted), c = leaf j
+  // (src/mps_node.jl:492-527); norm_method in d
+  OP_EAT_B = 6,
+  // eat! case (c) fold: a = site end-1, b = site end of node i, c = head of j
+  // (src/mps_node.jl:533-539)
+  OP_EAT_C = 7,
+  // normalise site a per norm_method d, accumulate log(norm) into lnZ
+  // (src/mps_node.jl:558-574)
+  OP_NORMALIZE = 8,
+  // merge! fuse: a = ein"ijk,kab->ijab"(a,b) reshaped (dl, d1*d2, dr2)
+  // (src/mps_node.jl:610-616)
+  OP_MERGE_FUSE = 9,
+  // cut_bondim_opt! numerics after both cano_to!: a = site of node i,
+  // b = site of node j, d = Dmax, e = result index, f = 1 to use the QR
+  // pre-reduction (opt) (src/tensor_network.jl:414-476)
+  OP_CUT = 10,
+  // compress_opt! two-site step: a = site i (left), b = site j (right),
+  // d = chi (src/mps_node.jl:401-443)
+  OP_COMPRESS_OPT = 11,
+  // compress! two-site step (src/mps_node.jl:363-372)
+  OP_COMPRESS = 12,
+  // raw2mps split step: a = remainder slot viewed (dleft, rest, 1), b = new
+  // site slot, c = physical dim of the split leg, d = chi
+  // (src/mps_node.jl:40-47)
+  OP_RAW_SPLIT = 13,
+  // lognorm of a scalar node: z = sum(site a); lnZ += log|z|; sign *= sign(z)
+  // (src/mps_node.jl:319-328, src/tensor_network.jl:489-501)
+  OP_SCALAR_LOGNORM = 14,
+  // standalone truncated SVD of the matrix held in slot a viewed as
+  // (dl*d) x dr: U -> slot b (m x k), S*V' kept in slot c (k x n), S -> result
+  // (src/linalg_utils.jl:15-28); d = maxdim, e = result index
+  OP_TSVD = 15,
+};
+
+struct Op {
+  int32_t code;
+  int32_t a, b, c, d, e, f;
+  int32_t pad;
+  double x;  // cutoff
+};
+
+struct SiteDesc {
+  int64_t off;   // element offset into the instance arena
+  int32_t dl, d, dr, live;
+};
+
+// Per-instance device state.
+struct InstState {
+  double lnZ;      // sum of log-norms (src/tensor_network.jl:657, :714)
+  double err;      // sum of discarded singular values (returned `error`)
+  double psi;      // product of eat! phases (src/tensor_network.jl:656)
+  double sign;     // product of scalar-node signs (tn.sign, :713)
+  int64_t arena_top, arena_cap, scratch_cap;
+  int32_t status;  // 0 ok, else an error code (see STATUS_*)
+  int32_t fail_op;
+  double* arena;
+  double* scratch;
+  SiteDesc* sites;
+  int32_t* results;    // per-op integer results (cut myd, tsvd k)
+  double* dresults;    // per-op floating results (tsvd discarded)
+  int32_t nsites, nresults;
+  double lntail;       // network_lognorm sum, added to lnZ once at the end
+};
+
+enum Status : int32_t {
+  STATUS_OK = 0,
+  STATUS_ARENA_FULL = 1,
+  STATUS_SCRATCH_FULL = 2,
+  STATUS_SHAPE = 3,
+  STATUS_NOCONV = 4,
+  STATUS_UNSUPPORTED = 5,
+};
+
+}  // namespace catn

@@ -1,0 +1,520 @@
+// Host planner: the integer control flow of CATN's contraction engine.
+//
+// Restates, line by line, the bookkeeping of src/tensor_network.jl (edge
+// selection, edge_count buckets, reverse/duplicate logic, Dmax cut decision)
+// and of src/mps_node.jl (swap chains, canonical-centre moves, eat!/merge!
+// leg bookkeeping). Every numeric step is emitted as an Op for the device
+// interpreter; physical leg dimensions are tracked here (they only change by
+// merge! products and by Dmax cuts, whose kept count is read back from the
+// device), virtual bond dimensions live only on the device.
+//
+// Documented fixed choices where the reference depends on Julia Dict order:
+//   * network node iteration (select=2 scan, network_lognorm sum) is in
+//     ascending node id (src/tensor_network.jl:303, :494).
+#pragma once
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ops.h"
+
+namespace catn {
+
+struct Params {
+  int Dmax = 32;
+  int chi = 32;
+  double cutoff = 1e-15;
+  int norm_method = 1;
+  int select = 1;
+  bool reverse = true;
+  bool svdopt = true;
+  bool swapopt = true;
+  bool compress = false;
+  bool cut_bond = false;
+  int seed = 1;
+};
+
+struct PNode {
+  std::vector<int> slot;   // site slot per MPS position
+  std::vector<int> nb;     // neighbour id per MPS position (<0: open leg)
+  std::vector<int> phys;   // physical dim per MPS position
+  int cano = 0;            // 1-based canonical centre (reference convention)
+  int order() const { return (int)slot.size(); }
+};
+
+struct PlanError : std::runtime_error {
+  explicit PlanError(const std::string& s) : std::runtime_error(s) {}
+};
+
+class Planner {
+ public:
+  Params p;
+  std::map<int, PNode> nodes;   // ascending id
+  std::map<int, std::vector<std::pair<int, int>>> edge_count;
+  int maxdim_intermediate = -1;
+  int nresults = 0;
+  std::vector<Op> prog;
+  bool acc_err = true;          // whether swap discards count into `error`
+  std::vector<std::pair<int, int>> edge_log;   // selected (i, j) per step
+
+  // ---- emission helpers ---------------------------------------------------
+  void emit(int code, int a = 0, int b = 0, int c = 0, int d = 0, int e = 0, int f = 0,
+            double x = 0.0) {
+    Op o;
+    o.code = code; o.a = a; o.b = b; o.c = c; o.d = d; o.e = e; o.f = f; o.pad = 0; o.x = x;
+    prog.push_back(o);
+  }
+
+  PNode& node(int id) {
+    auto it = nodes.find(id);
+    if (it == nodes.end()) throw PlanError("no node " + std::to_string(id));
+    return it->second;
+  }
+
+  // ---- src/mps_node.jl ----------------------------------------------------
+  // cano_to! (src/mps_node.jl:72-104); idx 1-based, 0 = last
+  void cano_to(PNode& n, int idx) {
+    if (idx == 0) idx = n.order();
+    if (n.cano == idx) return;
+    if (n.cano < idx) {
+      for (int i = n.cano; i <= idx - 1; ++i) {
+        emit(OP_CANO_R, n.slot[i - 1], n.slot[i], 0, 0, 0, 0, p.cutoff);
+        n.cano = i + 1;
+      }
+    } else {
+      for (int i = n.cano; i >= idx + 1; --i) {
+        emit(OP_CANO_L, n.slot[i - 1], n.slot[i - 2], 0, 0, 0, 0, p.cutoff);
+        n.cano = i - 1;
+      }
+    }
+  }
+
+  void left_canonical(PNode& n) {   // src/mps_node.jl:106
+    n.cano = 1;
+    cano_to(n, n.order());
+  }
+
+  // swap! (src/mps_node.jl:126-178)
+  void swap(PNode& n, int i, int j) {
+    if (std::abs(i - j) != 1) throw PlanError("swap!: i and j must be consecutive indices");
+    if (i < 1 || j < 1 || i > n.order() || j > n.order()) throw PlanError("swap!: indices out of range");
+    if (n.cano != i && n.cano != j) {
+      const int target = std::abs(n.cano - i) < std::abs(n.cano - j) ? i : j;
+      cano_to(n, target);
+    }
+    const int lo = std::min(i, j), hi = std::max(i, j);
+    emit(OP_SWAP, n.slot[lo - 1], n.slot[hi - 1], i < j ? 1 : 0, p.chi, acc_err ? 1 : 0,
+         p.swapopt ? 1 : 0, p.cutoff);
+    std::swap(n.phys[lo - 1], n.phys[hi - 1]);
+    n.cano = j;
+    std::swap(n.nb[i - 1], n.nb[j - 1]);
+  }
+
+  void move(PNode& n, int a, int b) {   // src/mps_node.jl:186-199
+    if (a == b) return;
+    if (b > a) {
+      for (int idx = a; idx <= b - 1; ++idx) swap(n, idx, idx + 1);
+    } else {
+      for (int idx = a; idx >= b + 1; --idx) swap(n, idx, idx - 1);
+    }
+  }
+
+  void move2tail(PNode& n, int idx) {   // src/mps_node.jl:208-217
+    const int len = n.order();
+    if (idx == len) { cano_to(n, len); return; }
+    for (int k = idx; k <= len - 1; ++k) swap(n, k, k + 1);
+    cano_to(n, len);
+  }
+
+  void move2head(PNode& n, int idx) {   // src/mps_node.jl:225-229
+    move(n, idx, 1);
+    cano_to(n, 1);
+  }
+
+  void reverse(PNode& n) {   // src/mps_node.jl:238-244
+    if (n.order() <= 1) return;
+    std::reverse(n.slot.begin(), n.slot.end());
+    std::reverse(n.nb.begin(), n.nb.end());
+    std::reverse(n.phys.begin(), n.phys.end());
+    for (int s : n.slot) emit(OP_PERMUTE321, s);
+    n.cano = n.order() + 1 - n.cano;
+  }
+
+  static int find_neighbor(const PNode& n, int j) {   // src/mps_node.jl:257-260
+    for (size_t k = 0; k < n.nb.size(); ++k)
+      if (n.nb[k] == j) return (int)k + 1;
+    return 0;
+  }
+
+  static void add_neighbor(PNode& n, int v, int pos) {   // src/mps_node.jl:267-274
+    if (pos == 0) n.nb.push_back(v);
+    else n.nb.insert(n.nb.begin() + (pos - 1), v);
+  }
+
+  static int delete_neighbor(PNode& n, int v) {   // src/mps_node.jl:282-287
+    const int idx = find_neighbor(n, v);
+    if (idx == 0) throw PlanError("delete_neighbor!: neighbor " + std::to_string(v) + " not found");
+    n.nb.erase(n.nb.begin() + (idx - 1));
+    return idx;
+  }
+
+  static double logdim(const PNode& n) {   // src/mps_node.jl:299-302
+    double s = 0.0;
+    for (int d : n.phys) s += log2((double)d);
+    return s;
+  }
+  static double logdim(const PNode& n, int idx) { return log2((double)n.phys[idx - 1]); }
+
+  // eat! (src/mps_node.jl:463-575)
+  void eat(PNode& ni, PNode& nj, int idx, int idxi) {
+    if (ni.order() == 1) {                       // case (a)
+      if (nj.order() != 1) throw PlanError("eat!: case (a) requires a leaf nodej");
+      emit(OP_EAT_A, ni.slot[0], nj.slot[0], 0, 0, 0, 0, p.cutoff);
+      ni.slot.clear(); ni.phys.clear();
+      ni.nb.erase(ni.nb.begin() + (idx - 1));
+      return;
+    }
+    move2tail(ni, idx);
+    if (nj.order() == 1) {                       // case (b)
+      const int L = ni.order();
+      emit(OP_EAT_B, ni.slot[L - 2], ni.slot[L - 1], nj.slot[0], p.norm_method, 0, 0, p.cutoff);
+      ni.cano -= 1;
+      ni.slot.pop_back(); ni.phys.pop_back();
+      ni.nb.pop_back();
+      return;
+    }
+    move2head(nj, idxi);                         // case (c)
+    const int L = ni.order();
+    emit(OP_EAT_C, ni.slot[L - 2], ni.slot[L - 1], nj.slot[0]);
+    ni.slot.pop_back(); ni.phys.pop_back();
+    ni.cano = ni.order();
+    ni.slot.insert(ni.slot.end(), nj.slot.begin() + 1, nj.slot.end());
+    ni.phys.insert(ni.phys.end(), nj.phys.begin() + 1, nj.phys.end());
+    ni.nb.pop_back();
+    ni.nb.insert(ni.nb.end(), nj.nb.begin() + 1, nj.nb.end());
+    cano_to(ni, ni.order());
+    emit(OP_NORMALIZE, ni.slot[ni.cano - 1], 0, 0, p.norm_method, 0, 0, p.cutoff);
+  }
+
+  // merge! (src/mps_node.jl:595-622)
+  void merge(PNode& n, int j, bool cross) {
+    std::vector<int> pos;
+    for (size_t k = 0; k < n.nb.size(); ++k)
+      if (n.nb[k] == j) pos.push_back((int)k + 1);
+    if (pos.size() != 2)
+      throw PlanError("merge!: expected exactly 2 sites with neighbor " + std::to_string(j));
+    const int idx1 = pos[0], idx2 = pos[1];
+    const int target = cross ? idx1 : idx1 + 1;
+    move(n, idx2, target);
+    n.nb.erase(n.nb.begin() + (target - 1));
+    cano_to(n, idx1);
+    emit(OP_MERGE_FUSE, n.slot[idx1 - 1], n.slot[idx1]);
+    n.phys[idx1 - 1] *= n.phys[idx1];
+    n.slot.erase(n.slot.begin() + idx1);
+    n.phys.erase(n.phys.begin() + idx1);
+    // the reference keeps `cano` as is after deleteat! (cano == idx1 here)
+    cano_to(n, idx1);
+  }
+
+  // compress! / compress_opt! (src/mps_node.jl:356-447)
+  void compress_node(PNode& n, bool opt) {
+    if (n.order() == 0) return;
+    left_canonical(n);
+    for (int j = n.order(); j >= 2; --j) {
+      const int i = j - 1;
+      emit(opt ? OP_COMPRESS_OPT : OP_COMPRESS, n.slot[i - 1], n.slot[j - 1], 0, p.chi, 0, 0,
+           p.cutoff);
+    }
+    n.cano = 1;
+  }
+
+  // ---- src/tensor_network.jl ----------------------------------------------
+  int dim_after_merge(int i, int j) {   // :152-155 (round: ties to even)
+    const PNode& a = node(i);
+    const int idx = find_neighbor(a, j);
+    const double v = logdim(a) + logdim(node(j)) - 2.0 * logdim(a, idx);
+    return (int)nearbyint(v);
+  }
+
+  void count_add_edges(const std::vector<std::pair<int, int>>& edges) {   // :167-176
+    for (auto& e : edges) edge_count[dim_after_merge(e.first, e.second)].push_back(e);
+  }
+
+  void count_add_nodes(const std::vector<int>& ids) {   // :185-197
+    std::set<std::pair<int, int>> edges;
+    for (int i : ids) {
+      if (i <= 0) continue;
+      for (int j : node(i).nb) {
+        if (j <= 0) continue;
+        edges.insert({std::min(i, j), std::max(i, j)});
+      }
+    }
+    count_add_edges(std::vector<std::pair<int, int>>(edges.begin(), edges.end()));
+  }
+
+  void count_remove_nodes(const std::vector<int>& ids) {   // :213-225
+    for (int id : ids) {
+      if (id <= 0) continue;
+      const std::vector<int> nbs = node(id).nb;
+      for (int nb : nbs) {
+        if (nb <= 0) continue;
+        const int i = std::min(id, nb), j = std::max(id, nb);
+        const int c = dim_after_merge(i, j);
+        auto it = edge_count.find(c);
+        if (it != edge_count.end()) {
+          auto& v = it->second;
+          v.erase(std::remove(v.begin(), v.end(), std::make_pair(i, j)), v.end());
+        }
+      }
+    }
+  }
+
+  void select_edge_init() {   // :233-236
+    edge_count.clear();
+    std::vector<int> ids;
+    for (auto& kv : nodes) ids.push_back(kv.first);
+    count_add_nodes(ids);
+  }
+
+  int min_cost() {
+    bool found = false;
+    int best = 0;
+    for (auto& kv : edge_count)
+      if (!kv.second.empty()) {
+        if (!found || kv.first < best) best = kv.first;
+        found = true;
+      }
+    if (!found) throw PlanError("select: no candidate edges");
+    return best;
+  }
+
+  std::pair<int, int> select_edge_min_dim() {   // :248-253
+    const int c = min_cost();
+    maxdim_intermediate = std::max(maxdim_intermediate, c);
+    return edge_count[c][0];
+  }
+
+  std::pair<int, int> select_edge_min_dim_triangle() {   // :262-288
+    const int c = min_cost();
+    maxdim_intermediate = std::max(maxdim_intermediate, c);
+    const auto& cands = edge_count[c];
+    std::pair<int, int> best = cands[0];
+    int bestc = -1;
+    for (auto& pr : cands) {
+      const int i = pr.first, j = pr.second;
+      std::set<int> neigh_j;
+      for (int v : node(j).nb)
+        if (v > 0 && v != i) neigh_j.insert(v);
+      int both = 0;
+      for (int k : neigh_j)
+        if (find_neighbor(node(k), i) > 0) ++both;
+      if (both > bestc) { bestc = both; best = pr; }
+    }
+    return best;
+  }
+
+  std::pair<int, int> select_edge_sequentially() {   // :300-316
+    bool found = false;
+    int bs = 0;
+    std::pair<int, int> best;
+    for (auto& kv : nodes) {
+      const int i = kv.first;
+      for (int v : kv.second.nb) {
+        if (v <= 0 || !(i < v)) continue;
+        if (!found || i + v < bs) { bs = i + v; best = {i, v}; found = true; }
+      }
+    }
+    if (!found) throw PlanError("select_edge_sequentially: no edges remain");
+    return best;
+  }
+
+  bool has_real_edges() const {   // :535-542
+    for (auto& kv : nodes)
+      for (int v : kv.second.nb)
+        if (v > 0) return true;
+    return false;
+  }
+
+  // cut_bondim(_opt)! up to the device op (:342-477). Emits the canonical
+  // moves and the OP_CUT; returns the result index that receives `myd`.
+  int emit_cut(int i, int idx_j_in_i, bool opt) {
+    PNode& ni = node(i);
+    const int j = ni.nb[idx_j_in_i - 1];
+    if (j <= 0)
+      throw PlanError("cut_bondim_opt!: leg " + std::to_string(idx_j_in_i) + " of node " +
+                      std::to_string(i) + " is an open leg (no neighbor)");
+    PNode& nj = node(j);
+    const int idx_i_in_j = find_neighbor(nj, i);
+    if (idx_i_in_j == 0)
+      throw PlanError("cut_bondim_opt!: node " + std::to_string(i) + " not found as neighbor of node " +
+                      std::to_string(j));
+    if (opt) {
+      cano_to(ni, idx_j_in_i);
+      cano_to(nj, idx_i_in_j);
+    }
+    const int r = nresults++;
+    emit(OP_CUT, ni.slot[idx_j_in_i - 1], nj.slot[idx_i_in_j - 1], 0, p.Dmax, r, opt ? 1 : 0,
+         p.cutoff);
+    pending_cut_[0] = i; pending_cut_[1] = idx_j_in_i; pending_cut_[2] = j;
+    pending_cut_[3] = idx_i_in_j; pending_cut_[4] = r;
+    return r;
+  }
+
+  // apply the device's kept count to both physical legs of the cut bond
+  void apply_cut(int myd) {
+    node(pending_cut_[0]).phys[pending_cut_[1] - 1] = myd;
+    node(pending_cut_[2]).phys[pending_cut_[3] - 1] = myd;
+  }
+  int pending_cut_result() const { return pending_cut_[4]; }
+
+  // ---- contraction! as a resumable state machine (:556-717) ----------------
+  // step() runs until the program needs a device result (after an OP_CUT:
+  // returns NEED_CUT) or the contraction is finished (returns DONE).
+  enum StepResult { NEED_CUT = 1, DONE = 2 };
+
+  void start() {
+    if ((p.select == 0 || p.select == 1) && edge_count.empty()) select_edge_init();
+    state_ = ST_EDGE;
+  }
+
+  StepResult step() {
+    for (;;) {
+      if (state_ == ST_DUPS) {
+        if (cut_pending_) {
+          cut_pending_ = false;
+          // result was applied by the caller via apply_cut
+        }
+        while (dup_pos_ < (int)dups_.size()) {
+          const int k = dups_[dup_pos_++];
+          PNode& ni = node(cur_i_);
+          const bool save = acc_err;
+          acc_err = false;                       // merge!(node_i, k) result is dropped (:662)
+          merge(ni, k, false);
+          acc_err = save;
+          const int idx_k_in_i = find_neighbor(ni, k);
+          bool cut = false;
+          if (p.cut_bond) cut = true;
+          else if (p.Dmax > 0 && ni.phys[idx_k_in_i - 1] > p.Dmax) cut = true;
+          if (cut && p.Dmax >= 0) {             // cut_bondim_opt! is a no-op for Dmax < 0
+            emit_cut(cur_i_, idx_k_in_i, true);
+            cut_pending_ = true;
+            return NEED_CUT;
+          }
+        }
+        finish_edge();
+        state_ = ST_EDGE;
+      }
+      if (state_ == ST_EDGE) {
+        if (!has_real_edges()) {
+          network_lognorm();
+          state_ = ST_DONE;
+          return DONE;
+        }
+        begin_edge();
+        state_ = ST_DUPS;
+        continue;
+      }
+      return DONE;
+    }
+  }
+
+  // network_lognorm (:489-501): emitted as scalar lognorm ops
+  void network_lognorm() {
+    for (auto& kv : nodes) {
+      PNode& n = kv.second;
+      bool scalar = true;
+      for (int d : n.phys) scalar = scalar && d == 1;
+      if (!scalar) continue;
+      if (n.order() == 0) continue;            // lognorm of an empty MPS = (0, 1)
+      if (n.order() != 1)
+        throw PlanError("lognorm: computing norm of a non-scalar MPS is not supported in contraction");
+      emit(OP_SCALAR_LOGNORM, n.slot[0]);
+    }
+  }
+
+ private:
+  enum State { ST_INIT, ST_EDGE, ST_DUPS, ST_DONE };
+  State state_ = ST_INIT;
+  int cur_i_ = 0, cur_j_ = 0;
+  std::vector<int> dups_;
+  int dup_pos_ = 0;
+  bool cut_pending_ = false;
+  int pending_cut_[5] = {0, 0, 0, 0, 0};
+
+  static int cld2(int a) { return (a + 1) / 2; }   // cld(a, 2) for a >= 0
+
+  void begin_edge() {
+    std::pair<int, int> e;
+    if (p.select == 0) e = select_edge_min_dim();
+    else if (p.select == 1) e = select_edge_min_dim_triangle();
+    else e = select_edge_sequentially();
+    int i = e.first, j = e.second;
+    edge_log.push_back(e);
+    if (node(j).order() > node(i).order()) std::swap(i, j);   // :585-590
+    PNode& ni = node(i);
+    PNode& nj = node(j);
+    if (p.select == 0 || p.select == 1) {                       // :594-597
+      std::vector<int> affected = {i, j};
+      affected.insert(affected.end(), ni.nb.begin(), ni.nb.end());
+      affected.insert(affected.end(), nj.nb.begin(), nj.nb.end());
+      count_remove_nodes(affected);
+    }
+    std::vector<int> neigh_j = nj.nb;
+    int idx_j_in_i = find_neighbor(ni, j);
+    int idx_i_in_j = find_neighbor(nj, i);
+    if (p.reverse) {                                            // :608-619
+      if (idx_j_in_i < cld2((int)ni.nb.size())) {
+        reverse(ni);
+        idx_j_in_i = find_neighbor(ni, j);
+      }
+      if (idx_i_in_j >= cld2((int)nj.nb.size())) {
+        reverse(nj);
+        neigh_j = nj.nb;
+        idx_i_in_j = find_neighbor(nj, i);
+      }
+    }
+    dups_.clear();
+    dup_pos_ = 0;
+    acc_err = true;
+    for (int l = 1; l <= (int)neigh_j.size(); ++l) {            // :627-650
+      if (l == idx_i_in_j) continue;
+      const int k = neigh_j[l - 1];
+      if (k <= 0) continue;
+      const int idx_k_in_i = find_neighbor(ni, k);
+      PNode& nk = node(k);
+      const int idx_i_in_k = find_neighbor(nk, i);
+      const int idx_j_in_k = delete_neighbor(nk, j);
+      add_neighbor(nk, i, idx_j_in_k);
+      if (idx_k_in_i > 0) {
+        dups_.push_back(k);
+        const bool cross = idx_i_in_k > idx_j_in_k;
+        merge(nk, i, cross);
+      }
+    }
+    eat(ni, nj, idx_j_in_i, idx_i_in_j);                       // :654-657
+    cur_i_ = i;
+    cur_j_ = j;
+  }
+
+  void finish_edge() {
+    PNode& ni = node(cur_i_);
+    PNode& nj = node(cur_j_);
+    nj.slot.clear(); nj.nb.clear(); nj.phys.clear();           // clear!(node_j) :684
+    if (p.compress) compress_node(ni, p.svdopt);               // :688-694
+    if (p.select == 0 || p.select == 1) {                       // :697-699
+      std::vector<int> ids = {cur_i_};
+      ids.insert(ids.end(), ni.nb.begin(), ni.nb.end());
+      count_add_nodes(ids);
+    }
+    for (int d : ni.phys) maxdim_intermediate = std::max(maxdim_intermediate, d);   // :702-707
+  }
+};
+
+}  // namespace catn

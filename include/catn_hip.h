@@ -1,0 +1,125 @@
+/*
+ * libcatn_hip — C ABI of the MI355X contraction engine for CATN.jl networks.
+ *
+ * The reference (xuanzhaogao/CATN.jl) is pure Julia and has no FFI; its only
+ * extension point is the storage type AT of TensorNetwork{T,AT} / MPSNode{T,AT}
+ * (src/tensor_network.jl:1, src/mps_node.jl:1) plus Adapt.adapt_structure
+ * (src/adapt.jl:6-24). This header is what a Julia backend (ccall) binds in
+ * place of that dispatch; each entry point names the reference function whose
+ * contract it keeps. All arrays are column-major (Julia layout); node ids and
+ * leg positions are 1-based like the reference. Every call returns 0 on
+ * success or a negative status; catn_last_error() gives the message.
+ *
+ * Threading: a context is thread-compatible, not thread-safe (one host
+ * thread, one HIP stream), as the reference is single-threaded.
+ */
+#ifndef CATN_HIP_H
+#define CATN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct catn_ctx catn_ctx;
+typedef struct catn_net catn_net;
+
+/* Keyword arguments of TensorNetwork / ising_network
+ * (src/tensor_network.jl:41-54, src/ising.jl:20-40). */
+typedef struct catn_params {
+  int32_t Dmax;        /* < 0: exact mode */
+  int32_t chi;
+  double cutoff;
+  int32_t norm_method; /* 0 none, 1 L2, 2 max-abs */
+  int32_t select;      /* 0 min-dim, 1 min-dim+triangle, 2 sequential */
+  int32_t reverse, svdopt, swapopt, compress, cut_bond;
+  int32_t seed;
+  int64_t arena_elems;   /* device arena per network (0 = default) */
+  int64_t scratch_elems; /* device scratch per network (0 = default) */
+} catn_params;
+
+/* Fill `p` with the reference defaults (Dmax=32, chi=32, cutoff=1e-15, ...). */
+void catn_params_default(catn_params* p);
+
+/* Create a context on HIP device `device` (one stream). */
+int catn_create(catn_ctx** ctx, int device);
+int catn_destroy(catn_ctx* ctx);
+const char* catn_last_error(void);
+/* name of the executor backing the context ("hip-gfx950") */
+const char* catn_backend(catn_ctx* ctx);
+
+/* ising_network (src/ising.jl:20-227): n spins, edges (2*nedges ints,
+ * 1-based, in list order), couplings w[nedges], fields h[n], inverse
+ * temperature beta; pinning arguments pos1/val1/pos2/val2 (0 = none). */
+int catn_ising_network(catn_ctx* ctx, int32_t n, int32_t nedges, const int32_t* edges,
+                       const double* w, const double* h, double beta, int32_t pos1,
+                       int32_t val1, int32_t pos2, int32_t val2, const catn_params* p,
+                       catn_net** out);
+
+/* TensorNetwork(tensors, ixs; ...) (src/tensor_network.jl:41-140): ntensors
+ * tensors, tensor t has ndims[t] legs with sizes dims[...] (concatenated)
+ * and data (concatenated, column-major); labels[...] one int64 label per leg.
+ * Labels shared by two tensors become bonds; a label seen once is an open leg. */
+int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
+                        const int64_t* dims, const double* data, const int64_t* labels,
+                        const catn_params* p, catn_net** out);
+
+/* Empty network holding hand-built MPS nodes (for the primitive API below). */
+int catn_net_create(catn_ctx* ctx, const catn_params* p, catn_net** out);
+/* MPSNode(mps, neighbor, cano, ...) (src/mps_node.jl:17-21): nsites sites,
+ * site k has dims (3k..3k+2) = (dl, d, dr) and data (concatenated). */
+int catn_net_add_node(catn_net* net, int32_t id, int32_t nsites, const int32_t* dims,
+                      const double* data, const int32_t* neighbor, int32_t cano);
+/* MPSNode(tensor, neighbor; chi, cutoff) via raw2mps (src/mps_node.jl:23-50) */
+int catn_net_add_tensor_node(catn_net* net, int32_t id, int32_t ndim, const int64_t* dims,
+                             const double* data, const int32_t* neighbor);
+int catn_net_free(catn_net* net);
+
+/* contraction!(tn) -> (lnZ, error, psi) (src/tensor_network.jl:556-717).
+ * Consumes the network like the reference. */
+int catn_contraction(catn_net* net, double* lnZ, double* error, double* psi);
+/* Same for `count` independent networks in ONE persistent launch per planning
+ * segment (one workgroup per network). Outputs are arrays of length count. */
+int catn_contraction_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, double* lnZ,
+                           double* error, double* psi);
+
+/* Per-network results after contraction. */
+int catn_net_info(catn_net* net, double* sign, int32_t* maxdim_intermediate,
+                  int32_t* num_isolated, int32_t* n, double* beta);
+/* The selected edge sequence (pairs, 2*count ints); *count in: capacity. */
+int catn_net_edge_log(catn_net* net, int32_t* pairs, int32_t* count);
+/* Device memory statistics: arena high-water mark and capacity (elements). */
+int catn_net_mem(catn_net* net, int64_t* arena_used, int64_t* arena_cap);
+
+/* ---- MPSNode primitives (src/mps_node.jl); node ids / positions 1-based ---- */
+int catn_node_swap(catn_net* net, int32_t id, int32_t i, int32_t j, double* error); /* swap! :126 */
+int catn_node_cano_to(catn_net* net, int32_t id, int32_t idx);                      /* cano_to! :72 */
+int catn_node_left_canonical(catn_net* net, int32_t id);                           /* :106 */
+int catn_node_move(catn_net* net, int32_t id, int32_t a, int32_t b, double* error); /* move! :186 */
+int catn_node_move2tail(catn_net* net, int32_t id, int32_t idx, double* error);    /* :208 */
+int catn_node_move2head(catn_net* net, int32_t id, int32_t idx, double* error);    /* :225 */
+int catn_node_reverse(catn_net* net, int32_t id);                                  /* reverse! :238 */
+int catn_node_merge(catn_net* net, int32_t id, int32_t j, int32_t cross, double* error); /* merge! :595 */
+int catn_node_compress(catn_net* net, int32_t id, int32_t opt, double* error);     /* compress(_opt)! :356/:394 */
+/* eat!(node_i, node_j, idx, idxi) -> (lognorm, error, phase) (:463-575) */
+int catn_node_eat(catn_net* net, int32_t id_i, int32_t id_j, int32_t idx, int32_t idxi,
+                  double* lognorm, double* error, double* phase);
+/* cut_bondim! / cut_bondim_opt! (src/tensor_network.jl:342, :399) */
+int catn_cut_bondim(catn_net* net, int32_t i, int32_t idx_j_in_i, int32_t opt, double* error);
+/* Node structure: order, cano, neighbours (cap entries), per-site dims (3*cap) */
+int catn_node_get(catn_net* net, int32_t id, int32_t* order, int32_t* cano, int32_t* neighbor,
+                  int32_t* dims, int32_t cap);
+/* Download site k (1-based) data (dl*d*dr doubles). */
+int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t cap);
+
+/* tsvd(A; cutoff, maxdim) (src/linalg_utils.jl:15-28) on the device.
+ * A is m x n column-major. Outputs U (m x k), S (min(m,n), full spectrum,
+ * descending), V (n x k), k, and the discarded sum. */
+int catn_tsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, double cutoff,
+              int32_t maxdim, int32_t* k, double* U, double* S, double* V, double* discarded);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CATN_HIP_H */

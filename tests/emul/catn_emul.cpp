@@ -1,0 +1,43 @@
+// CPU emulator of libcatn_hip — TEST BUILD ONLY.
+//
+// Compiles the product's engine (engine.h), planner and C ABI with g++ and runs
+// each "workgroup" as a single CPU thread (barriers become no-ops). It lets the
+// CPU test suite exercise the exact device algorithm and host bookkeeping
+// without a GPU. It is never loaded by the product path (catn_amd), which
+// requires the HIP library.
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../catn.jl_amd/csrc/engine.h"
+#include "../../catn.jl_amd/csrc/capi_impl.h"
+
+namespace catn {
+
+class CpuExecutor : public Executor {
+ public:
+  const char* name() const override { return "cpu-emulator"; }
+  void* dmalloc(size_t bytes) override { return calloc(1, bytes < 8 ? 8 : bytes); }
+  void dfree(void* p) override { free(p); }
+  void h2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
+  void d2h(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
+  void run(const std::vector<InstState*>& states,
+           const std::vector<const std::vector<Op>*>& programs) override {
+    double red[16];
+    int flag = 0;
+    WG g;
+    g.tid = 0;
+    g.nthr = 1;
+    g.red = red;
+    g.flag = &flag;
+    for (size_t b = 0; b < states.size(); ++b) {
+      const std::vector<Op>& p = *programs[b];
+      run_program(g, states[b], p.data(), 0, (int)p.size());
+    }
+  }
+};
+
+Executor* make_executor(int) { return new CpuExecutor(); }
+
+}  // namespace catn
