@@ -67,12 +67,16 @@ _SIGS = {
     "catn_net_create": (C.c_int, [_P, C.POINTER(_Params), C.POINTER(_P)]),
     "catn_net_add_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I32P, _DP, _I32P, C.c_int32]),
     "catn_net_add_tensor_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I64P, _DP, _I32P]),
+    "catn_net_upload": (C.c_int, [_P]),
     "catn_net_free": (C.c_int, [_P]),
     "catn_contraction": (C.c_int, [_P, _DP, _DP, _DP]),
     "catn_contraction_batch": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _DP, _DP, _DP]),
     "catn_net_info": (C.c_int, [_P, _DP, _I32P, _I32P, _I32P, _DP]),
     "catn_net_edge_log": (C.c_int, [_P, _I32P, _I32P]),
     "catn_net_mem": (C.c_int, [_P, _I64P, _I64P]),
+    "catn_net_profile": (C.c_int, [_P, _I64P, _I32P, _I64P, _I32P, _DP]),
+    "catn_net_work": (C.c_int, [_P, _DP, _DP, _I64P]),
+    "catn_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
     "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "catn_node_cano_to": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "catn_node_left_canonical": (C.c_int, [_P, C.c_int32]),
@@ -131,6 +135,12 @@ class Context:
         if rc != 0:
             msg = self.lib.catn_last_error().decode()
             raise CatnError(f"libcatn (rc={rc}): {msg}")
+
+    def kernel_stats(self, reset=False):
+        """(launches, seconds) of the interpreter kernel on this context."""
+        n, sec = C.c_int64(), C.c_double()
+        self._check(self.lib.catn_kernel_stats(self.h, C.byref(n), C.byref(sec), int(bool(reset))))
+        return n.value, sec.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -224,6 +234,31 @@ class TensorNetwork:
         cap = C.c_int32(cnt.value)
         self.ctx._check(self.ctx.lib.catn_net_edge_log(self.h, buf.ctypes.data_as(_I32P), C.byref(cap)))
         return [(int(buf[2 * k]), int(buf[2 * k + 1])) for k in range(cnt.value)]
+
+    OPNAMES = ["nop", "cano_r", "cano_l", "swap", "permute321", "eat_a", "eat_b", "eat_c",
+               "normalize", "merge_fuse", "cut", "compress_opt", "compress", "raw_split",
+               "scalar_lognorm", "tsvd"]
+
+    def profile(self):
+        """Device op profile: {opname: (count, seconds)} and per SVD size bucket."""
+        t = np.zeros(16, np.int64)
+        c = np.zeros(16, np.int32)
+        st = np.zeros(8, np.int64)
+        sc = np.zeros(8, np.int32)
+        hz = C.c_double()
+        self.ctx._check(self.ctx.lib.catn_net_profile(
+            self.h, t.ctypes.data_as(_I64P), c.ctypes.data_as(_I32P), st.ctypes.data_as(_I64P),
+            sc.ctypes.data_as(_I32P), C.byref(hz)))
+        ops = {self.OPNAMES[k]: (int(c[k]), float(t[k]) / hz.value) for k in range(16) if c[k]}
+        buckets = {f"<={4 << k}" if k < 7 else ">256": (int(sc[k]), float(st[k]) / hz.value)
+                   for k in range(8) if sc[k]}
+        return ops, buckets
+
+    def work(self):
+        """(algorithmic flops, algorithmic bytes, scratch high-water mark)."""
+        f, b, h = C.c_double(), C.c_double(), C.c_int64()
+        self.ctx._check(self.ctx.lib.catn_net_work(self.h, C.byref(f), C.byref(b), C.byref(h)))
+        return f.value, b.value, h.value
 
     def mem(self):
         u, c = C.c_int64(), C.c_int64()

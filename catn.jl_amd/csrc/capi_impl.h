@@ -30,6 +30,12 @@ class Executor {
   // workgroup per instance; returns after completion.
   virtual void run(const std::vector<InstState*>& states,
                    const std::vector<const std::vector<Op>*>& programs) = 0;
+  // frequency of the device wall clock read by the op timers (Hz)
+  virtual double tick_hz() const { return 1.0; }
+  // launches of the interpreter kernel and their summed duration (seconds,
+  // measured with events on the stream the kernel runs on)
+  int64_t launches = 0;
+  double kernel_seconds = 0.0;
 };
 
 Executor* make_executor(int device);   // provided by the executor TU
@@ -500,6 +506,13 @@ int catn_net_add_tensor_node(catn_net* net, int32_t id, int32_t ndim, const int6
   CATN_CATCH
 }
 
+int catn_net_upload(catn_net* net) {
+  CATN_TRY
+  net->ensure_device();
+  return 0;
+  CATN_CATCH
+}
+
 int catn_net_free(catn_net* net) {
   delete net;
   return 0;
@@ -547,6 +560,37 @@ int catn_net_edge_log(catn_net* net, int32_t* pairs, int32_t* count) {
     pairs[2 * k + 1] = net->pl.edge_log[k].second;
   }
   *count = tot;
+  return 0;
+}
+
+int catn_net_profile(catn_net* net, int64_t* op_ticks, int32_t* op_count, int64_t* svd_ticks,
+                     int32_t* svd_count, double* tick_hz) {
+  for (int k = 0; k < 16; ++k) {
+    if (op_ticks) op_ticks[k] = net->hs.op_ticks[k];
+    if (op_count) op_count[k] = net->hs.op_count[k];
+  }
+  for (int k = 0; k < 8; ++k) {
+    if (svd_ticks) svd_ticks[k] = net->hs.svd_ticks[k];
+    if (svd_count) svd_count[k] = net->hs.svd_count[k];
+  }
+  if (tick_hz) *tick_hz = net->ctx->ex->tick_hz();
+  return 0;
+}
+
+int catn_net_work(catn_net* net, double* flops, double* bytes, int64_t* scratch_hwm) {
+  if (flops) *flops = net->hs.alg_flops;
+  if (bytes) *bytes = net->hs.alg_bytes;
+  if (scratch_hwm) *scratch_hwm = net->hs.scratch_hwm;
+  return 0;
+}
+
+int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t reset) {
+  if (launches) *launches = ctx->ex->launches;
+  if (seconds) *seconds = ctx->ex->kernel_seconds;
+  if (reset) {
+    ctx->ex->launches = 0;
+    ctx->ex->kernel_seconds = 0.0;
+  }
   return 0;
 }
 

@@ -46,22 +46,30 @@ class HipExecutor : public Executor {
     hipDeviceProp_t prop;
     hipcheck(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     name_ = std::string("hip-") + prop.gcnArchName;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+      tick_hz_ = 1000.0 * khz;
     hipcheck(hipStreamCreate(&stream_), "hipStreamCreate");
+    hipcheck(hipEventCreate(&ev0_), "hipEventCreate");
+    hipcheck(hipEventCreate(&ev1_), "hipEventCreate");
   }
   ~HipExecutor() override {
-    if (d_ops_) hipFree(d_ops_);
-    if (d_ranges_) hipFree(d_ranges_);
-    if (d_states_) hipFree(d_states_);
-    hipStreamDestroy(stream_);
+    if (d_ops_) (void)hipFree(d_ops_);
+    if (d_ranges_) (void)hipFree(d_ranges_);
+    if (d_states_) (void)hipFree(d_states_);
+    (void)hipEventDestroy(ev0_);
+    (void)hipEventDestroy(ev1_);
+    (void)hipStreamDestroy(stream_);
   }
   const char* name() const override { return name_.c_str(); }
+  double tick_hz() const override { return tick_hz_; }
   void* dmalloc(size_t bytes) override {
     void* p = nullptr;
     hipcheck(hipMalloc(&p, bytes < 8 ? 8 : bytes), "hipMalloc");
     return p;
   }
   void dfree(void* p) override {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
   void h2d(void* dst, const void* src, size_t bytes) override {
     hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "h2d");
@@ -87,21 +95,27 @@ class HipExecutor : public Executor {
     hipcheck(hipMemcpyAsync(d_ops_, all.data(), sizeof(Op) * all.size(), hipMemcpyHostToDevice, stream_), "ops");
     hipcheck(hipMemcpyAsync(d_ranges_, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice, stream_), "ranges");
     hipcheck(hipMemcpyAsync(d_states_, states.data(), sizeof(InstState*) * nb, hipMemcpyHostToDevice, stream_), "states");
+    hipcheck(hipEventRecord(ev0_, stream_), "event record");
     hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(kThreads), 0, stream_, d_states_, d_ops_, d_ranges_);
     hipcheck(hipGetLastError(), "launch exec_programs");
+    hipcheck(hipEventRecord(ev1_, stream_), "event record");
     hipcheck(hipStreamSynchronize(stream_), "exec_programs");
+    float ms = 0.0f;
+    hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+    launches += 1;
+    kernel_seconds += 1e-3 * ms;
   }
 
  private:
   void reserve(size_t nops, int nb) {
     if (nops > ops_cap_) {
-      if (d_ops_) hipFree(d_ops_);
+      if (d_ops_) (void)hipFree(d_ops_);
       ops_cap_ = nops * 2;
       hipcheck(hipMalloc(&d_ops_, sizeof(Op) * ops_cap_), "hipMalloc ops");
     }
     if ((size_t)nb > st_cap_) {
-      if (d_ranges_) hipFree(d_ranges_);
-      if (d_states_) hipFree(d_states_);
+      if (d_ranges_) (void)hipFree(d_ranges_);
+      if (d_states_) (void)hipFree(d_states_);
       st_cap_ = (size_t)nb * 2;
       hipcheck(hipMalloc(&d_ranges_, sizeof(int) * 2 * st_cap_), "hipMalloc ranges");
       hipcheck(hipMalloc(&d_states_, sizeof(InstState*) * st_cap_), "hipMalloc states");
@@ -110,10 +124,12 @@ class HipExecutor : public Executor {
   int device_;
   std::string name_;
   hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   Op* d_ops_ = nullptr;
   int* d_ranges_ = nullptr;
   InstState** d_states_ = nullptr;
   size_t ops_cap_ = 0, st_cap_ = 0;
+  double tick_hz_ = 1e8;
 };
 
 Executor* make_executor(int device) { return new HipExecutor(device); }

@@ -89,6 +89,20 @@ CATN_D double grp_sum(double v, int L) {
   return v;
 }
 
+CATN_D int64_t now_ticks() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (int64_t)wall_clock64();
+#else
+  return 0;
+#endif
+}
+
+CATN_D int size_bucket(int r) {
+  int b = 0, lim = 4;
+  while (b < 7 && r > lim) { ++b; lim *= 2; }
+  return b;
+}
+
 CATN_D int pow2floor(int x) {
   int p = 1;
   while (p * 2 <= x) p *= 2;
@@ -424,6 +438,9 @@ struct Exec {
   Scratch sc;
   int64_t top;     // arena bump pointer (uniform register copy)
   int status;
+  int bucket;      // size bucket of the op's main SVD (-1: none)
+  double flops, bytes;   // algorithmic work of this launch segment
+  CATN_D void work(double f, double elems) { flops += f; bytes += 8.0 * elems; }
 
   CATN_D SiteDesc site(int s) const {
     volatile SiteDesc* p = st->sites + s;
@@ -450,6 +467,14 @@ struct Exec {
 };
 
 // count kept singular values: tsvd rule (src/linalg_utils.jl:24-25)
+// textbook operation counts (Golub & Van Loan): thin SVD (R-SVD) of an
+// m x n matrix, Householder QR of a tall m x n matrix
+CATN_D double svd_flops(double m, double n) {
+  const double M = m > n ? m : n, N = m > n ? n : m;
+  return 6.0 * M * N * N + 20.0 * N * N * N;
+}
+CATN_D double qr_flops(double m, double n) { return 2.0 * m * n * n - 2.0 / 3.0 * n * n * n; }
+
 CATN_D int keep_tsvd(const double* S, int r, double cutoff, int maxdim) {
   int c = 0;
   for (int i = 0; i < r; ++i) c += (S[i] > cutoff);
@@ -486,6 +511,7 @@ CATN_DN void op_cano_r(Exec& E, const Op& op) {
   if (wg_svd(g, E.sc, E.ptr(a), m, n, m, R, &sw) < 0) {
     E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
   }
+  E.bucket = size_bucket(R.r);
   const int k = keep_cano(R.S, R.r, op.x);
   const int64_t offa = E.alloc((int64_t)m * k);
   const int64_t nb = (int64_t)b.d * b.dr;
@@ -504,6 +530,8 @@ CATN_DN void op_cano_r(Exec& E, const Op& op) {
     for (int q = 0; q < n; ++q) s += v[q] * bp[q + (int64_t)n * x];
     nbp[idx] = R.S[p] * s;
   }
+  E.work(svd_flops(m, n) + 2.0 * k * n * (double)nb,
+         (double)m * n + (double)n * nb + (double)m * k + (double)k * nb);
   E.set_site(op.a, offa, a.dl, a.d, k);
   E.set_site(op.b, offb, k, b.d, b.dr);
   g.sync();
@@ -531,6 +559,7 @@ CATN_DN void op_cano_l(Exec& E, const Op& op) {
   if (wg_svd(g, E.sc, Mt, m, n, m, R, &sw) < 0) {
     E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
   }
+  E.bucket = size_bucket(R.r);
   const int k = keep_cano(R.S, R.r, op.x);
   const int64_t offa = E.alloc((int64_t)k * d * dr);
   const int64_t offb = E.alloc((int64_t)b.dl * b.d * k);
@@ -554,6 +583,8 @@ CATN_DN void op_cano_l(Exec& E, const Op& op) {
     for (int c = 0; c < n; ++c) s += bp[u + rb * c] * v[c];
     nbp[idx] = s * R.S[p];
   }
+  E.work(svd_flops(m, n) + 2.0 * (double)rb * n * k,
+         (double)m * n + (double)rb * n + (double)k * m + (double)rb * k);
   E.set_site(op.a, offa, k, d, dr);
   E.set_site(op.b, offb, b.dl, b.d, k);
   g.sync();
@@ -590,6 +621,7 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   if (wg_svd(g, E.sc, M, m, n, m, R, &sw) < 0) {
     E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
   }
+  E.bucket = size_bucket(R.r);
   const int k = keep_tsvd(R.S, R.r, op.x, op.d);
   if (op.e) E.acc(&E.st->err, tail_sum(R.S, k, R.r));
   const int64_t offl = E.alloc((int64_t)m * k);
@@ -609,6 +641,8 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
     const double v = R.V[c + (int64_t)n * p];
     nr[idx] = right ? R.S[p] * v : v;
   }
+  E.work(2.0 * m * n * (double)dd + svd_flops(m, n),
+         (double)d0 * d2 * dd + (double)dd * d1 * d3 + (double)m * k + (double)k * n);
   E.set_site(op.a, offl, d0, d1, k);
   E.set_site(op.b, offr, k, d2, d3);
   g.sync();
@@ -629,6 +663,7 @@ CATN_DN void op_permute321(Exec& E, const Op& op) {
     const int y = (int)(rest % a.d), x = (int)(rest / a.d);
     dst[idx] = s[x + (int64_t)a.dl * (y + (int64_t)a.d * z)];
   }
+  E.work(0.0, 2.0 * tot);
   E.set_site(op.a, off, a.dr, a.d, a.dl);
   g.sync();
 }
@@ -644,6 +679,7 @@ CATN_DN void op_eat_a(Exec& E, const Op& op) {
   for (int64_t i = g.tid; i < n; i += g.nthr) part += x[i] * y[i];
   const double r = wg_sum(g, part);
   const double ar = fabs(r);
+  E.work(2.0 * n, 2.0 * n);
   if (ar > op.x) {
     E.acc(&E.st->lnZ, log(ar));
     if (g.tid == 0) E.st->psi = E.st->psi * (r / ar);
@@ -670,6 +706,7 @@ CATN_D void scale_normalize(Exec& E, int slot, const SiteDesc& s, int method, do
   double* x = E.ptr(s);
   const int64_t n = (int64_t)s.dl * s.d * s.dr;
   const double nrm = site_norm(g, x, n, method);
+  E.work(3.0 * n, 2.0 * n);
   if (nrm <= cutoff) { g.sync(); return; }   // src/mps_node.jl:522-524, :569-571
   const double inv = 1.0 / nrm;
   for (int64_t i = g.tid; i < n; i += g.nthr) x[i] = x[i] * inv;
@@ -703,6 +740,7 @@ CATN_DN void op_eat_b(Exec& E, const Op& op) {
     for (int q = 0; q < D; ++q) s += pp[r + rows * q] * mat[q];
     np_[r] = s;
   }
+  E.work(2.0 * D * d + 2.0 * rows * D + 3.0 * rows, (double)D * d + d + rows * D + rows);
   E.set_site(op.a, off, p.dl, p.d, 1);
   if (g.tid == 0) { E.st->sites[op.b].live = 0; E.st->sites[op.c].live = 0; }
   g.sync();
@@ -723,6 +761,8 @@ CATN_DN void op_eat_c(Exec& E, const Op& op) {
   const int64_t off = E.alloc(rows * Dr);
   if (E.status) return;
   wg_gemm_nn(g, (int)rows, Dr, Dl, E.ptr(p), rows, mat, Dl, E.st->arena + off, rows);
+  E.work(2.0 * Dl * Dr * d + 2.0 * rows * Dl * Dr,
+         (double)Dl * d + (double)d * Dr + rows * Dl + rows * Dr);
   E.set_site(op.a, off, p.dl, p.d, Dr);
   if (g.tid == 0) { E.st->sites[op.b].live = 0; E.st->sites[op.c].live = 0; }
   g.sync();
@@ -741,6 +781,7 @@ CATN_DN void op_merge_fuse(Exec& E, const Op& op) {
   const int64_t off = E.alloc((int64_t)m * n);
   if (E.status) return;
   wg_gemm_nn(g, m, n, k, E.ptr(a), m, E.ptr(b), k, E.st->arena + off, m);
+  E.work(2.0 * m * n * (double)k, (double)m * k + (double)k * n + (double)m * n);
   E.set_site(op.a, off, a.dl, a.d * b.d, b.dr);
   if (g.tid == 0) E.st->sites[op.b].live = 0;
   g.sync();
@@ -813,6 +854,7 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
   if (wg_svd(g, E.sc, Mg, ri_rows, rj_rows, ri_rows, R, &sw) < 0) {
     E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
   }
+  E.bucket = size_bucket(R.r);
   const int nk = keep_tsvd(R.S, R.r, op.x, 1 << 30);
   int myd = nk < op.d ? nk : op.d;
   if (myd == 0) myd = 1;
@@ -861,6 +903,10 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
     const int p = (int)(rest % myd), c = (int)(rest / myd);
     nb[idx] = Fj[a + (int64_t)dbl * c + (int64_t)mj * p];
   }
+  E.work((fl ? 2.0 * qr_flops(mi, d) + 2.0 * mi * d * (double)myd : 0.0) +
+             (fr ? 2.0 * qr_flops(mj, d) + 2.0 * mj * d * (double)myd : 0.0) +
+             2.0 * ri_rows * rj_rows * (double)d + svd_flops(ri_rows, rj_rows),
+         (double)mi * d + (double)mj * d + (double)mi * myd + (double)mj * myd);
   E.set_site(op.a, offa, dal, myd, dar);
   E.set_site(op.b, offb, dbl, myd, dbr);
   if (g.tid == 0) E.st->results[op.e] = myd;
@@ -943,6 +989,11 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
   if (E.status) return;
   wg_copy(g, E.st->arena + offa, Gl, (int64_t)m * k);
   wg_transpose(g, E.st->arena + offb, Gr, n, k);   // (k x n) = V^T
+  E.work(opt ? ((fl ? 2.0 * qr_flops(m, dd) + 2.0 * m * dd * (double)k : 0.0) +
+                 (fr ? 2.0 * qr_flops(n, dd) + 2.0 * n * dd * (double)k : 0.0) +
+                 2.0 * rl * rr * (double)dd + svd_flops(rl, rr))
+              : 2.0 * m * n * (double)dd + svd_flops(m, n),
+         (double)m * dd + (double)dd * n + (double)m * k + (double)k * n);
   E.set_site(op.a, offa, d0, d1, k);
   E.set_site(op.b, offb, k, d2, d3);
   g.sync();
@@ -959,6 +1010,7 @@ CATN_DN void op_raw_split(Exec& E, const Op& op) {
   if (wg_svd(g, E.sc, E.ptr(s), m, n, m, R, &sw) < 0) {
     E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
   }
+  E.bucket = size_bucket(R.r);
   const int k = keep_tsvd(R.S, R.r, op.x, op.d);
   const int64_t offu = E.alloc((int64_t)m * k);
   const int64_t offr = E.alloc((int64_t)k * n);
@@ -970,6 +1022,7 @@ CATN_DN void op_raw_split(Exec& E, const Op& op) {
     const int64_t c = idx / k;
     nr[idx] = R.S[p] * R.V[c + (int64_t)n * p];
   }
+  E.work(svd_flops(m, n), (double)m * n + (double)m * k + (double)k * n);
   E.set_site(op.b, offu, dleft, di, k);
   E.set_site(op.a, offr, k, n, 1);
   g.sync();
@@ -997,6 +1050,7 @@ CATN_DN void op_tsvd(Exec& E, const Op& op) {
   if (wg_svd(g, E.sc, E.ptr(s), m, n, m, R, &sw) < 0) {
     E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
   }
+  E.bucket = size_bucket(R.r);
   const int k = keep_tsvd(R.S, R.r, op.x, op.d);
   const int64_t offu = E.alloc((int64_t)m * k);
   const int64_t offv = E.alloc((int64_t)n * k);
@@ -1022,8 +1076,13 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
   E.st = st;
   E.top = st->arena_top;
   E.status = st->status;
+  E.flops = 0.0;
+  E.bytes = 0.0;
+  int64_t shwm = 0;
   for (int i = begin; i < end && E.status == STATUS_OK; ++i) {
     const Op op = ops[i];
+    const int64_t t0 = now_ticks();
+    E.bucket = -1;
     E.sc.base = st->scratch;
     E.sc.cap = st->scratch_cap;
     E.sc.top = 0;
@@ -1047,12 +1106,26 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
       default: break;
     }
     if (E.sc.fail && E.status == STATUS_OK) E.status = STATUS_SCRATCH_FULL;
+    if (E.sc.top > shwm) shwm = E.sc.top;
     if (E.status != STATUS_OK && g.tid == 0) st->fail_op = i;
     g.sync();
+    if (g.tid == 0) {
+      const int64_t dt = now_ticks() - t0;
+      const int c = op.code & 15;
+      st->op_ticks[c] += dt;
+      st->op_count[c] += 1;
+      if (E.bucket >= 0) {
+        st->svd_ticks[E.bucket] += dt;
+        st->svd_count[E.bucket] += 1;
+      }
+    }
   }
   if (g.tid == 0) {
     st->arena_top = E.top;
     st->status = E.status;
+    st->alg_flops += E.flops;
+    st->alg_bytes += E.bytes;
+    if (shwm > st->scratch_hwm) st->scratch_hwm = shwm;
   }
   g.sync();
   return E.status;

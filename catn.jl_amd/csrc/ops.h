@@ -90,6 +90,17 @@ struct InstState {
   double* dresults;    // per-op floating results (tsvd discarded)
   int32_t nsites, nresults;
   double lntail;       // network_lognorm sum, added to lnZ once at the end
+  // device-side profile: wall-clock ticks and counts per opcode, and per
+  // size bucket (min(m, n) of the op's main SVD: <=4,8,16,32,64,128,256,>256)
+  int64_t op_ticks[16];
+  int32_t op_count[16];
+  int64_t svd_ticks[8];
+  int32_t svd_count[8];
+  // algorithmic work of the executed ops (textbook counts: GEMM 2mnk,
+  // Householder QR 2mn^2-2n^3/3, thin SVD 6mn^2+20n^3; bytes = site tensors
+  // read once and written once), and the scratch high-water mark (elements)
+  double alg_flops, alg_bytes;
+  int64_t scratch_hwm;
 };
 
 enum Status : int32_t {

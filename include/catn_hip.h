@@ -74,6 +74,9 @@ int catn_net_add_node(catn_net* net, int32_t id, int32_t nsites, const int32_t* 
 /* MPSNode(tensor, neighbor; chi, cutoff) via raw2mps (src/mps_node.jl:23-50) */
 int catn_net_add_tensor_node(catn_net* net, int32_t id, int32_t ndim, const int64_t* dims,
                              const double* data, const int32_t* neighbor);
+/* Allocate the network's device state and upload its site tensors now
+ * (otherwise done lazily by the first call that runs on the device). */
+int catn_net_upload(catn_net* net);
 int catn_net_free(catn_net* net);
 
 /* contraction!(tn) -> (lnZ, error, psi) (src/tensor_network.jl:556-717).
@@ -89,6 +92,18 @@ int catn_net_info(catn_net* net, double* sign, int32_t* maxdim_intermediate,
                   int32_t* num_isolated, int32_t* n, double* beta);
 /* The selected edge sequence (pairs, 2*count ints); *count in: capacity. */
 int catn_net_edge_log(catn_net* net, int32_t* pairs, int32_t* count);
+/* Device-side profile of the ops executed for this network: wall-clock ticks
+ * and counts per opcode (16 entries) and per SVD size bucket (8 entries:
+ * min(m,n) <= 4, 8, 16, 32, 64, 128, 256, larger); tick_hz converts ticks. */
+int catn_net_profile(catn_net* net, int64_t* op_ticks, int32_t* op_count, int64_t* svd_ticks,
+                     int32_t* svd_count, double* tick_hz);
+/* Algorithmic work of the ops executed so far (flops by textbook counts,
+ * bytes = site tensors read and written once per op) and the device scratch
+ * high-water mark in elements. */
+int catn_net_work(catn_net* net, double* flops, double* bytes, int64_t* scratch_hwm);
+/* Number of interpreter-kernel launches on this context and their summed
+ * duration in seconds (HIP events on the context's stream); reset != 0 zeroes. */
+int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t reset);
 /* Device memory statistics: arena high-water mark and capacity (elements). */
 int catn_net_mem(catn_net* net, int64_t* arena_used, int64_t* arena_cap);
 
