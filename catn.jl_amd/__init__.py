@@ -76,6 +76,7 @@ _SIGS = {
     "catn_net_mem": (C.c_int, [_P, _I64P, _I64P]),
     "catn_net_profile": (C.c_int, [_P, _I64P, _I32P, _I64P, _I32P, _DP]),
     "catn_net_work": (C.c_int, [_P, _DP, _DP, _I64P]),
+    "catn_net_svd_profile": (C.c_int, [_P, _I64P]),
     "catn_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
     "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "catn_node_cano_to": (C.c_int, [_P, C.c_int32, C.c_int32]),
@@ -252,6 +253,10 @@ class TensorNetwork:
         ops = {self.OPNAMES[k]: (int(c[k]), float(t[k]) / hz.value) for k in range(16) if c[k]}
         buckets = {f"<={4 << k}" if k < 7 else ">256": (int(sc[k]), float(st[k]) / hz.value)
                    for k in range(8) if sc[k]}
+        sub = np.zeros(4, np.int64)
+        self.ctx._check(self.ctx.lib.catn_net_svd_profile(self.h, sub.ctypes.data_as(_I64P)))
+        buckets["_large_svd"] = {"qr_s": sub[0] / hz.value, "jacobi_s": sub[1] / hz.value,
+                                 "finish_s": sub[2] / hz.value, "sweeps": int(sub[3])}
         return ops, buckets
 
     def work(self):

@@ -6,8 +6,12 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <cmath>
+#include <exception>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <memory>
 #include <string>
 #include <vector>
@@ -207,6 +211,36 @@ inline void run_nets(Executor* ex, const std::vector<catn_net*>& nets) {
 
 inline void run_one(catn_net* n) { run_nets(n->ctx->ex.get(), {n}); }
 
+// f(i) for i in [0, n) on up to 16 host threads (planners of different
+// networks share no state); the first exception is rethrown.
+template <class F>
+void parallel_for(int n, F f) {
+  int T = (int)std::thread::hardware_concurrency();
+  T = std::max(1, std::min(T, 16));
+  if (n < 16 || T == 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  T = std::min(T, n);
+  std::atomic<int> next(0);
+  std::exception_ptr err;
+  std::mutex mu;
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&] {
+      for (int i; (i = next++) < n;) {
+        try {
+          f(i);
+        } catch (...) {
+          std::lock_guard<std::mutex> lk(mu);
+          if (!err) err = std::current_exception();
+        }
+      }
+    });
+  for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
+
 // ising_network site construction (src/ising.jl:20-227)
 inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, const double* w,
                         const double* h, double beta, int pos1, int val1, int pos2, int val2) {
@@ -322,11 +356,10 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
   std::vector<int> need(nets.size());
   while (!active.empty()) {
     std::vector<catn_net*> still;
-    std::vector<int> cut;
-    for (catn_net* n : active) {
-      const Planner::StepResult r = n->pl.step();
-      cut.push_back(r == Planner::NEED_CUT ? 1 : 0);
-    }
+    std::vector<int> cut(active.size());
+    parallel_for((int)active.size(), [&](int k) {
+      cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
+    });
     run_nets(ex, active);
     for (size_t k = 0; k < active.size(); ++k) {
       catn_net* n = active[k];
@@ -574,6 +607,11 @@ int catn_net_profile(catn_net* net, int64_t* op_ticks, int32_t* op_count, int64_
     if (svd_count) svd_count[k] = net->hs.svd_count[k];
   }
   if (tick_hz) *tick_hz = net->ctx->ex->tick_hz();
+  return 0;
+}
+
+int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks) {
+  for (int k = 0; k < 4; ++k) sub_ticks[k] = net->hs.sub_ticks[k];
   return 0;
 }
 

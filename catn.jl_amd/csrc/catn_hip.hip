@@ -19,7 +19,8 @@ namespace catn {
 constexpr int kThreads = 256;
 
 __global__ __launch_bounds__(kThreads) void exec_programs(InstState* const* states,
-                                                          const Op* ops, const int* ranges) {
+                                                          const Op* ops, const int* ranges,
+                                                          int lds_cap) {
   __shared__ double red[16];
   __shared__ int flag;
   WG g;
@@ -27,7 +28,9 @@ __global__ __launch_bounds__(kThreads) void exec_programs(InstState* const* stat
   g.nthr = blockDim.x;
   g.red = red;
   g.flag = &flag;
+  g.lds_cap = lds_cap;
   const int b = blockIdx.x;
+  g.sub = states[b]->sub_ticks;
   run_program(g, states[b], ops, ranges[2 * b], ranges[2 * b + 1]);
 }
 
@@ -52,6 +55,10 @@ class HipExecutor : public Executor {
     hipcheck(hipStreamCreate(&stream_), "hipStreamCreate");
     hipcheck(hipEventCreate(&ev0_), "hipEventCreate");
     hipcheck(hipEventCreate(&ev1_), "hipEventCreate");
+    hipcheck(hipFuncSetAttribute((const void*)exec_programs,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * kLdsMaxDoubles)),
+             "hipFuncSetAttribute");
   }
   ~HipExecutor() override {
     if (d_ops_) (void)hipFree(d_ops_);
@@ -96,7 +103,9 @@ class HipExecutor : public Executor {
     hipcheck(hipMemcpyAsync(d_ranges_, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice, stream_), "ranges");
     hipcheck(hipMemcpyAsync(d_states_, states.data(), sizeof(InstState*) * nb, hipMemcpyHostToDevice, stream_), "states");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
-    hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(kThreads), 0, stream_, d_states_, d_ops_, d_ranges_);
+    const int lds_cap = lds_doubles_for_batch(nb);
+    hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(kThreads), sizeof(double) * lds_cap, stream_,
+                       d_states_, d_ops_, d_ranges_, lds_cap);
     hipcheck(hipGetLastError(), "launch exec_programs");
     hipcheck(hipEventRecord(ev1_, stream_), "event record");
     hipcheck(hipStreamSynchronize(stream_), "exec_programs");

@@ -36,6 +36,8 @@ struct WG {
   int tid, nthr;
   double* red;     // LDS, >= 16 doubles (one per wavefront)
   int* flag;       // LDS int
+  int lds_cap;     // doubles available in the dynamic LDS region
+  int64_t* sub;    // sub-step tick accumulators (instance state), or null
   CATN_D void sync() const {
 #if defined(__HIP_DEVICE_COMPILE__)
     __syncthreads();
@@ -95,6 +97,10 @@ CATN_D int64_t now_ticks() {
 #else
   return 0;
 #endif
+}
+
+CATN_D void sub_tick(const WG& g, int k, int64_t t0) {
+  if (g.tid == 0 && g.sub) g.sub[k] += now_ticks() - t0;
 }
 
 CATN_D int size_bucket(int r) {
@@ -235,12 +241,40 @@ CATN_DN void wg_apply_q(const WG& g, const double* A, int m, int n, int64_t lda,
 }
 
 // ---------------------------------------------------------------------------
+// dynamic LDS region of the interpreter workgroup. On the device the base is
+// the kernel's dynamic shared array (so pointers derived from it compile to
+// ds_* instructions after inlining); the CPU emulator uses a static buffer.
+// ---------------------------------------------------------------------------
+constexpr int kLdsMaxDoubles = 10240;   // 80 KB upper bound (launch chooses <=)
+
+// LDS budget per workgroup (doubles). MI355X has 160 KB of LDS per CU: up to
+// 512 workgroups get ~80 KB each (2 per CU on 256 CUs); larger batches get
+// ~40 KB so that 4 workgroups co-reside per CU. The static part (reduction
+// slots, flags) is kept out of the budget.
+inline int lds_doubles_for_batch(int nb) { return nb <= 512 ? 10176 : 5056; }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+CATN_D double* lds_base() {
+  extern __shared__ double catn_dyn_lds[];
+  return catn_dyn_lds;
+}
+#elif defined(__HIPCC__)
+CATN_D double* lds_base() { return nullptr; }   // host pass of the HIP build: unused
+#else
+inline double* lds_base() {
+  static thread_local double buf[kLdsMaxDoubles];
+  return buf;
+}
+#endif
+
+// ---------------------------------------------------------------------------
 // one-sided Jacobi SVD: A (m x n, m >= n) -> A V with mutually orthogonal
 // columns; V (n x n) accumulates the rotations. Returns sweeps used, or -1
-// when not converged within maxsweeps.
+// when not converged within maxsweeps. Round-robin ordering: n/2 disjoint
+// column pairs per step, one aligned group of L lanes per pair.
 // ---------------------------------------------------------------------------
-CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double* V,
-                      int64_t ldv, int maxsweeps) {
+CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double* V,
+                       int64_t ldv, int maxsweeps, double thr_in = -1.0) {
   for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     V[i + (int64_t)j * ldv] = (i == j) ? 1.0 : 0.0;
@@ -264,7 +298,7 @@ CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double*
     const double v = A[(idx % m) + (idx / m) * lda];
     fpart += v * v;
   }
-  const double thr = 1e-36 * wg_sum(g, fpart);
+  const double thr = thr_in >= 0.0 ? thr_in : 1e-36 * wg_sum(g, fpart);
   for (int sweep = 1; sweep <= maxsweeps; ++sweep) {
     if (g.tid == 0) *g.flag = 0;
     g.sync();
@@ -314,10 +348,15 @@ CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double*
   return -1;
 }
 
+CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double* V,
+                      int64_t ldv, int maxsweeps) {
+  return jacobi_core(g, A, m, n, lda, V, ldv, maxsweeps);
+}
+
 // ---------------------------------------------------------------------------
 // full thin SVD: A (m x n, lda) is NOT modified. Outputs, sorted descending:
 // S[r], U (m x r, ld m), V (n x r, ld n), r = min(m, n). Returns r, or -1 on
-// scratch exhaustion / non-convergence (status set in *st).
+// scratch exhaustion / non-convergence.
 // ---------------------------------------------------------------------------
 struct SvdResult {
   int r;
@@ -326,27 +365,114 @@ struct SvdResult {
   double* V;  // n x r
 };
 
-CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
-                        bool transpose_in, SvdResult& out, int* sweeps_used) {
-  // Works on B = A (m x n) or B = A^T when transpose_in (then the caller
-  // passes m, n of B). Requires m >= n. Produces U (m x n), S, V (n x n) of B.
-  double* W = sc.alloc((int64_t)m * n);
-  double* V = sc.alloc((int64_t)n * n);
-  double* S = sc.alloc(n);
+// Sorted outputs from the converged Jacobi state: W (rows x n, ld ldw) holds
+// A V (orthogonal columns), V (n x n, ld ldv). Columns go to global U (top
+// `rows` rows of an ldu-leading matrix), Vs, Ss in descending order of the
+// column norms (stable in the column index). `key` is an n-double LDS/global
+// scratch, `rk` an n-int scratch.
+CATN_D void jacobi_finish(const WG& g, const double* W, int rows, int n, int64_t ldw,
+                          const double* V, int64_t ldv, double* key, int* rk, double* U,
+                          int64_t ldu, double* Ss, double* Vs) {
+  for (int j = g.tid; j < n; j += g.nthr) {
+    const double* c = W + (int64_t)j * ldw;
+    double s = 0.0;
+    for (int i = 0; i < rows; ++i) s += c[i] * c[i];
+    key[j] = sqrt(s);
+  }
+  g.sync();
+  for (int j = g.tid; j < n; j += g.nthr) {
+    const double sj = key[j];
+    int r = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = key[i];
+      r += (si > sj) || (si == sj && i < j);
+    }
+    rk[j] = r;
+    Ss[r] = sj;
+  }
+  g.sync();
+  for (int64_t idx = g.tid; idx < (int64_t)rows * n; idx += g.nthr) {
+    const int i = (int)(idx % rows), j = (int)(idx / rows);
+    const double sj = key[j];
+    const int r = rk[j];
+    // a zero column keeps the unit vector e_r (LAPACK returns some
+    // orthonormal completion; only the all-zero matrix keeps such a column)
+    U[i + (int64_t)r * ldu] = sj > 0.0 ? W[i + (int64_t)j * ldw] / sj : (i == r ? 1.0 : 0.0);
+  }
+  for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    Vs[i + (int64_t)rk[j] * n] = V[i + (int64_t)j * ldv];
+  }
+  g.sync();
+}
+
+// LDS-resident path: the whole m x n operand, V and the norms live in LDS.
+CATN_DN int wg_svd_lds(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
+                       bool transpose_in, SvdResult& out, int* sweeps_used) {
+  double* L = lds_base();
+  double* W = L;                              // m x n
+  double* V = W + (int64_t)m * n;             // n x n
+  double* key = V + (int64_t)n * n;           // n
+  int* rk = (int*)(key + n);                  // n ints
   double* U = sc.alloc((int64_t)m * n);
-  int* rank_tmp = nullptr;
-  (void)rank_tmp;
+  double* Ss = sc.alloc(n);
+  double* Vs = sc.alloc((int64_t)n * n);
   if (sc.fail) return -1;
-  // load B
   for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
     const int i = (int)(idx % m), j = (int)(idx / m);
     W[idx] = transpose_in ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
   }
   g.sync();
-  const bool use_qr = m >= 2 * n && n > 0;
-  double* R = W;
-  int64_t ldr = m;
+  const int sw = jacobi_core(g, W, m, n, m, V, n, 60);
+  if (sweeps_used) *sweeps_used = sw;
+  jacobi_finish(g, W, m, n, m, V, n, key, rk, U, m, Ss, Vs);
+  out.r = n;
+  out.U = U;
+  out.S = Ss;
+  out.V = Vs;
+  return sw < 0 ? -1 : n;
+}
+
+// LDS-resident Jacobi on the R factor of a global Householder QR.
+CATN_DN int wg_jacobi_r_lds(const WG& g, const double* R, int n, double* U, int64_t ldu,
+                            double* Ss, double* Vs, int* sweeps_used) {
+  double* L = lds_base();
+  double* W = L;
+  double* V = W + (int64_t)n * n;
+  double* key = V + (int64_t)n * n;
+  int* rk = (int*)(key + n);
+  for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) W[idx] = R[idx];
+  g.sync();
+  const int sw = jacobi_core(g, W, n, n, n, V, n, 60);
+  if (sweeps_used) *sweeps_used = sw;
+  jacobi_finish(g, W, n, n, n, V, n, key, rk, U, ldu, Ss, Vs);
+  return sw;
+}
+
+CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
+                        bool transpose_in, SvdResult& out, int* sweeps_used) {
+  // SVD of B = A (m x n) or B = A^T when transpose_in (then m, n are B's).
+  // Requires m >= n. Produces U (m x n), S, V (n x n) of B, sorted.
+  // Operands that fit run entirely in LDS; otherwise Householder QR (when
+  // m > n) reduces B to its n x n R factor, whose Jacobi runs in LDS when R
+  // and V fit, else block-pair by block-pair.
+  const int64_t lds_direct = (int64_t)m * n + (int64_t)n * n + 2 * (int64_t)n;
+  if (lds_direct <= g.lds_cap) return wg_svd_lds(g, sc, A, m, n, lda, transpose_in, out, sweeps_used);
+  double* W = sc.alloc((int64_t)m * n);
+  double* U = sc.alloc((int64_t)m * n);
+  double* Ss = sc.alloc(n);
+  double* Vs = sc.alloc((int64_t)n * n);
+  if (sc.fail) return -1;
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    W[idx] = transpose_in ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
+  }
+  g.sync();
+  const bool use_qr = m > n;
+  int sw = 0;
   double* tau = nullptr;
+  double* R = W;
+  int64_t t0 = now_ticks();
   if (use_qr) {
     tau = sc.alloc(n);
     R = sc.alloc((int64_t)n * n);
@@ -356,58 +482,28 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
       const int i = (int)(idx % n), j = (int)(idx / n);
       R[idx] = (i <= j) ? W[i + (int64_t)j * m] : 0.0;
     }
-    ldr = n;
     g.sync();
   }
-  const int rows = use_qr ? n : m;
-  const int sw = wg_jacobi(g, R, rows, n, ldr, V, n, 60);
-  if (sweeps_used) *sweeps_used = sw;
-  // singular values = column norms; stable descending order by rank sort
-  for (int j = g.tid; j < n; j += g.nthr) {
-    const double* c = R + (int64_t)j * ldr;
-    double s = 0.0;
-    for (int i = 0; i < rows; ++i) s += c[i] * c[i];
-    S[j] = sqrt(s);
+  sub_tick(g, 0, t0);
+  t0 = now_ticks();
+  if (2 * (int64_t)n * n + 2 * (int64_t)n <= g.lds_cap) {
+    sw = wg_jacobi_r_lds(g, R, n, U, m, Ss, Vs, sweeps_used);
+    sub_tick(g, 1, t0);
+    t0 = now_ticks();
+  } else {
+    double* V = sc.alloc((int64_t)n * n);
+    double* key = sc.alloc(n);
+    int* rk = (int*)sc.alloc(n);
+    if (sc.fail) return -1;
+    sw = wg_jacobi(g, R, n, n, n, V, n, 60);
+    if (sweeps_used) *sweeps_used = sw;
+    sub_tick(g, 1, t0);
+    t0 = now_ticks();
+    jacobi_finish(g, R, n, n, n, V, n, key, rk, U, m, Ss, Vs);
   }
-  g.sync();
-  double* Ss = sc.alloc(n);
-  double* Vs = sc.alloc((int64_t)n * n);
-  if (sc.fail) return -1;
-  // the sorted U (rows x n) is assembled into U's top block, V into Vs
-  for (int j = g.tid; j < n; j += g.nthr) {
-    const double sj = S[j];
-    int rk = 0;
-    for (int i = 0; i < n; ++i) {
-      const double si = S[i];
-      rk += (si > sj) || (si == sj && i < j);
-    }
-    Ss[rk] = sj;
-    const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
-    const double* c = R + (int64_t)j * ldr;
-    double* u = U + (int64_t)rk * m;
-    if (sj > 0.0) {
-      for (int i = 0; i < rows; ++i) u[i] = c[i] * inv;
-    } else {
-      for (int i = 0; i < rows; ++i) u[i] = 0.0;
-    }
-    const double* v = V + (int64_t)j * n;
-    double* vs = Vs + (int64_t)rk * n;
-    for (int i = 0; i < n; ++i) vs[i] = v[i];
-  }
-  g.sync();
-  // columns with zero singular value: give them a unit vector orthogonal to
-  // nothing in particular (LAPACK returns some orthonormal completion; only
-  // the all-zero matrix keeps such a column)
-  for (int j = g.tid; j < n; j += g.nthr) {
-    if (Ss[j] == 0.0) {
-      double* u = U + (int64_t)j * m;
-      if (j < rows) u[j] = 1.0;
-    }
-  }
-  g.sync();
-  if (use_qr) {
-    wg_apply_q(g, W, m, n, m, tau, U, n, m);
-  }
+  if (use_qr) wg_apply_q(g, W, m, n, m, tau, U, n, m);
+  sub_tick(g, 2, t0);
+  if (g.tid == 0 && g.sub && sw > 0) g.sub[3] += sw;
   out.r = n;
   out.U = U;
   out.S = Ss;
@@ -428,6 +524,7 @@ CATN_DN int wg_svd(const WG& g, Scratch& sc, const double* A, int m, int n, int6
   out.V = t.U;   // n x r (ld n)
   return r;
 }
+
 
 // ---------------------------------------------------------------------------
 // interpreter

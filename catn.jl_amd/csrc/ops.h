@@ -101,6 +101,8 @@ struct InstState {
   // read once and written once), and the scratch high-water mark (elements)
   double alg_flops, alg_bytes;
   int64_t scratch_hwm;
+  // sub-step ticks inside the SVD: 0 QR, 1 Jacobi, 2 finish + Q apply, 3 sweeps
+  int64_t sub_ticks[4];
 };
 
 enum Status : int32_t {

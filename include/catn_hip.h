@@ -104,6 +104,9 @@ int catn_net_work(catn_net* net, double* flops, double* bytes, int64_t* scratch_
 /* Number of interpreter-kernel launches on this context and their summed
  * duration in seconds (HIP events on the context's stream); reset != 0 zeroes. */
 int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t reset);
+/* Ticks spent inside the large-SVD path: [QR, Jacobi, finish + Q apply,
+ * Jacobi sweep count]. */
+int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks);
 /* Device memory statistics: arena high-water mark and capacity (elements). */
 int catn_net_mem(catn_net* net, int64_t* arena_used, int64_t* arena_cap);
 

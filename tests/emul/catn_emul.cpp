@@ -31,8 +31,10 @@ class CpuExecutor : public Executor {
     g.nthr = 1;
     g.red = red;
     g.flag = &flag;
+    g.lds_cap = lds_doubles_for_batch((int)states.size());
     for (size_t b = 0; b < states.size(); ++b) {
       const std::vector<Op>& p = *programs[b];
+      g.sub = states[b]->sub_ticks;
       run_program(g, states[b], p.data(), 0, (int)p.size());
     }
   }
