@@ -94,7 +94,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-kernel-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc) and exit")
     args = ap.parse_args()
+    if args.copy_kernel_mib:
+        import torch
+        nbytes = args.copy_kernel_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel_bytes_per_launch": nbytes}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(a solve's 801 contractions shard trivially across GPUs, "

@@ -141,7 +141,14 @@ struct catn_net {
       }
       if (top > arena_cap) throw catn::Error(-4, "arena capacity exceeded while uploading sites");
       if (!buf.empty()) ex->h2d(hs.arena + base, buf.data(), sizeof(double) * buf.size());
-      for (auto& pr : descs) ex->h2d(hs.sites + pr.first, &pr.second, sizeof(catn::SiteDesc));
+      // staged slots are always the contiguous id range [first, nslots)
+      const int first = descs.front().first;
+      std::vector<catn::SiteDesc> block(descs.size());
+      for (size_t k = 0; k < descs.size(); ++k) {
+        if (descs[k].first != first + (int)k) throw catn::Error(-3, "staged slots not contiguous");
+        block[k] = descs[k].second;
+      }
+      ex->h2d(hs.sites + first, block.data(), sizeof(catn::SiteDesc) * block.size());
       hs.arena_top = top;
       staged.clear();
     }

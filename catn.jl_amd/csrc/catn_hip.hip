@@ -18,7 +18,7 @@ namespace catn {
 
 constexpr int kThreads = 256;
 
-__global__ __launch_bounds__(kThreads) void exec_programs(InstState* const* states,
+__global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* const* states,
                                                           const Op* ops, const int* ranges,
                                                           int lds_cap) {
   __shared__ double red[16];

@@ -527,6 +527,150 @@ CATN_DN int wg_svd(const WG& g, Scratch& sc, const double* A, int m, int n, int6
 
 
 // ---------------------------------------------------------------------------
+// Certified full-rank factorisation (no truncation possible).
+//
+// When every singular value of A (m x n) provably exceeds the truncation
+// threshold and min(m, n) <= maxdim, the reference's truncated SVD keeps all
+// min(m, n) singular triplets and discards nothing; its factors U and S V'
+// then differ from any other factorisation A = L * Rt with the same
+// isometric side only by a unitary gauge on the new bond, which every later
+// contraction and SVD is invariant to (same bond dimension, same canonical
+// form, same represented tensor). Such steps are factorised by Householder QR
+// instead of the Jacobi SVD. The certificate is a lower bound on the smallest
+// singular value from the QR's triangular factor,
+//     sigma_min(A) = sigma_min(R) >= 1 / ||R^{-1}||_F,
+// accepted only above max(100 * cutoff, 1e-12 * ||A||_F), where the inverse
+// is computed to better than 1e-3 relative accuracy; any step that is not
+// certified takes the SVD path.
+//
+// iso_left: L (m x r) has orthonormal columns (U side), Rt is the centre.
+// otherwise Rt (r x n) has orthonormal rows (V' side), L is the centre.
+// Outputs live in scratch: L (ld m), Rt (ld r).
+// ---------------------------------------------------------------------------
+struct Factor {
+  int r;
+  double* L;
+  double* Rt;
+};
+
+// 1 / ||R^{-1}||_F of an upper-triangular n x n R (ld ldr); 0 if singular.
+CATN_DN double wg_inv_fro_bound(const WG& g, const double* R, int n, int64_t ldr, double* xbuf) {
+  double part = 0.0;
+  bool sing = false;
+  for (int j = g.tid; j < n; j += g.nthr) {
+    double* x = xbuf + (int64_t)j * n;     // column j of R^{-1}, rows 0..j
+    const double rjj = R[j + (int64_t)j * ldr];
+    if (rjj == 0.0) { sing = true; continue; }
+    x[j] = 1.0 / rjj;
+    part += x[j] * x[j];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += R[i + (int64_t)k * ldr] * x[k];
+      const double rii = R[i + (int64_t)i * ldr];
+      if (rii == 0.0) { sing = true; break; }
+      x[i] = -s / rii;
+      part += x[i] * x[i];
+    }
+  }
+  const double tot = wg_sum(g, sing ? INFINITY : part);
+  if (!(tot < INFINITY) || tot <= 0.0) return 0.0;
+  return 1.0 / sqrt(tot);
+}
+
+CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m, int n,
+                                int64_t lda, double cutoff, int maxdim, bool iso_left,
+                                Factor& F) {
+  const int r = m < n ? m : n;
+  if (r <= 0 || r > maxdim) return false;
+  const int64_t save = sc.top;
+  // tall orientation T (tm x r): A itself when m >= n, else A^T
+  const bool trans = m < n;
+  const int tm = trans ? n : m;
+  double* W = sc.alloc((int64_t)tm * r);
+  double* tau = sc.alloc(r);
+  double* R = sc.alloc((int64_t)r * r);
+  double* X = sc.alloc((int64_t)r * r);
+  if (sc.fail) return false;
+  double fpart = 0.0;
+  for (int64_t idx = g.tid; idx < (int64_t)tm * r; idx += g.nthr) {
+    const int i = (int)(idx % tm), j = (int)(idx / tm);
+    const double v = trans ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
+    W[idx] = v;
+    fpart += v * v;
+  }
+  const double fro = sqrt(wg_sum(g, fpart));
+  g.sync();
+  wg_geqr2(g, W, tm, r, tm, tau);
+  for (int64_t idx = g.tid; idx < (int64_t)r * r; idx += g.nthr) {
+    const int i = (int)(idx % r), j = (int)(idx / r);
+    R[idx] = (i <= j) ? W[i + (int64_t)j * tm] : 0.0;
+  }
+  g.sync();
+  const double lb = wg_inv_fro_bound(g, R, r, r, X);
+  const double thr = fmax(100.0 * cutoff, 1e-12 * fro);
+  if (!(lb > thr)) {
+    g.sync();
+    sc.top = save;
+    return false;
+  }
+  // the isometric factor is Q of the tall orientation when that side is the
+  // one QR'd (iso_left & !trans, or !iso_left & trans); otherwise identity
+  F.r = r;
+  const bool need_q = iso_left != trans;
+  double* Q = nullptr;
+  if (need_q) {
+    Q = sc.alloc((int64_t)tm * r);
+    if (sc.fail) return false;
+    for (int64_t idx = g.tid; idx < (int64_t)r * r; idx += g.nthr) {
+      const int i = (int)(idx % r), j = (int)(idx / r);
+      Q[i + (int64_t)j * tm] = (i == j) ? 1.0 : 0.0;
+    }
+    g.sync();
+    wg_apply_q(g, W, tm, r, tm, tau, Q, r, tm);
+  }
+  if (iso_left && !trans) {          // A = Q R: L = Q (m x n), Rt = R (n x n)
+    F.L = Q;
+    F.Rt = R;
+  } else if (iso_left && trans) {    // rank m: L = I_m, Rt = A (m x n)
+    double* L = sc.alloc((int64_t)m * m);
+    double* Rt = sc.alloc((int64_t)m * n);
+    if (sc.fail) return false;
+    for (int64_t idx = g.tid; idx < (int64_t)m * m; idx += g.nthr)
+      L[idx] = (idx % m == idx / m) ? 1.0 : 0.0;
+    for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr)
+      Rt[idx] = A[(idx % m) + (idx / m) * lda];
+    F.L = L;
+    F.Rt = Rt;
+  } else if (!iso_left && !trans) {  // rank n: L = A (m x n), Rt = I_n
+    double* L = sc.alloc((int64_t)m * n);
+    double* Rt = sc.alloc((int64_t)n * n);
+    if (sc.fail) return false;
+    for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr)
+      L[idx] = A[(idx % m) + (idx / m) * lda];
+    for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr)
+      Rt[idx] = (idx % n == idx / n) ? 1.0 : 0.0;
+    F.L = L;
+    F.Rt = Rt;
+  } else {                           // A^T = Q' R': L = R'^T (m x m), Rt = Q'^T (m x n)
+    double* L = sc.alloc((int64_t)m * m);
+    double* Rt = sc.alloc((int64_t)m * n);
+    if (sc.fail) return false;
+    for (int64_t idx = g.tid; idx < (int64_t)m * m; idx += g.nthr) {
+      const int i = (int)(idx % m), j = (int)(idx / m);
+      L[idx] = R[j + (int64_t)i * r];
+    }
+    for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+      const int i = (int)(idx % m), j = (int)(idx / m);
+      Rt[idx] = Q[j + (int64_t)i * tm];
+    }
+    F.L = L;
+    F.Rt = Rt;
+  }
+  g.sync();
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // interpreter
 // ---------------------------------------------------------------------------
 struct Exec {
@@ -598,37 +742,77 @@ CATN_D double tail_sum(const double* S, int from, int r) {
   return s;
 }
 
+// Split A (m x n) into A ~= L * R with the reference's truncation rule:
+// rule 0 = tsvd(cutoff, maxdim) (src/linalg_utils.jl:24-25), rule 1 = the
+// cano_to! rule (src/mps_node.jl:79-82). iso_left: L = U (isometry), R = S V';
+// else L = U S, R = V' (isometry rows). Certified full-rank steps are
+// factorised by QR (wg_factor_fullrank); the others by the Jacobi SVD.
+// L is m x k (ld m), R is k x n (ld k), both in scratch.
+struct Split {
+  int k;
+  double* L;
+  double* R;
+  double disc;
+};
+
+CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool iso_left, int rule,
+                      double cutoff, int maxdim, Split& sp) {
+  const WG& g = E.g;
+  Factor F;
+  const int maxd = rule == 1 ? (1 << 30) : maxdim;
+  const int r0 = m < n ? m : n;
+  E.bucket = size_bucket(r0);
+  if (wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, F)) {
+    sp.k = F.r; sp.L = F.L; sp.R = F.Rt; sp.disc = 0.0;
+    const double tm = m > n ? m : n;
+    E.work(qr_flops(tm, r0) + r0 * (double)r0 * r0 / 3.0 + (iso_left != (m < n) ? qr_flops(tm, r0) : 0.0),
+           0.0);
+    if (g.tid == 0 && g.sub) g.sub[3] += 0;   // (QR path: no Jacobi sweeps)
+    return true;
+  }
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return false; }
+  SvdResult R;
+  int sw = 0;
+  if (wg_svd(g, E.sc, A, m, n, lda, R, &sw) < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV;
+    return false;
+  }
+  E.work(svd_flops(m, n), 0.0);
+  const int k = rule == 1 ? keep_cano(R.S, R.r, cutoff) : keep_tsvd(R.S, R.r, cutoff, maxdim);
+  double* L = E.sc.alloc((int64_t)m * k);
+  double* Rm = E.sc.alloc((int64_t)k * n);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return false; }
+  for (int64_t idx = g.tid; idx < (int64_t)m * k; idx += g.nthr) {
+    const int p = (int)(idx / m);
+    L[idx] = iso_left ? R.U[idx] : R.U[idx] * R.S[p];
+  }
+  for (int64_t idx = g.tid; idx < (int64_t)k * n; idx += g.nthr) {
+    const int p = (int)(idx % k);
+    const int64_t c = idx / k;
+    const double v = R.V[c + (int64_t)n * p];
+    Rm[idx] = iso_left ? R.S[p] * v : v;
+  }
+  g.sync();
+  sp.k = k; sp.L = L; sp.R = Rm; sp.disc = tail_sum(R.S, k, R.r);
+  return true;
+}
+
 CATN_DN void op_cano_r(Exec& E, const Op& op) {
   const WG& g = E.g;
   SiteDesc a = E.site(op.a), b = E.site(op.b);
   const int m = a.dl * a.d, n = a.dr;
   if (b.dl != n) { E.status = STATUS_SHAPE; return; }
-  SvdResult R;
-  int sw = 0;
-  if (wg_svd(g, E.sc, E.ptr(a), m, n, m, R, &sw) < 0) {
-    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
-  }
-  E.bucket = size_bucket(R.r);
-  const int k = keep_cano(R.S, R.r, op.x);
-  const int64_t offa = E.alloc((int64_t)m * k);
+  Split sp;
+  if (!wg_split(E, E.ptr(a), m, n, m, true, 1, op.x, 0, sp)) return;
+  const int k = sp.k;
   const int64_t nb = (int64_t)b.d * b.dr;
+  const int64_t offa = E.alloc((int64_t)m * k);
   const int64_t offb = E.alloc((int64_t)k * nb);
   if (E.status) return;
-  double* na = E.st->arena + offa;
-  double* nbp = E.st->arena + offb;
-  wg_copy(g, na, R.U, (int64_t)m * k);
-  // new b[p, x] = sum_q S[p] V[q,p] b[q, x]
-  const double* bp = E.ptr(b);
-  for (int64_t idx = g.tid; idx < (int64_t)k * nb; idx += g.nthr) {
-    const int p = (int)(idx % k);
-    const int64_t x = idx / k;
-    double s = 0.0;
-    const double* v = R.V + (int64_t)p * n;
-    for (int q = 0; q < n; ++q) s += v[q] * bp[q + (int64_t)n * x];
-    nbp[idx] = R.S[p] * s;
-  }
-  E.work(svd_flops(m, n) + 2.0 * k * n * (double)nb,
-         (double)m * n + (double)n * nb + (double)m * k + (double)k * nb);
+  wg_copy(g, E.st->arena + offa, sp.L, (int64_t)m * k);
+  // new b = R * b   (k x n) * (n x nb)
+  wg_gemm_nn(g, k, (int)nb, n, sp.R, k, E.ptr(b), n, E.st->arena + offb, k);
+  E.work(2.0 * k * n * (double)nb, (double)m * n + (double)n * nb + (double)m * k + (double)k * nb);
   E.set_site(op.a, offa, a.dl, a.d, k);
   E.set_site(op.b, offb, k, b.d, b.dr);
   g.sync();
@@ -643,7 +827,7 @@ CATN_DN void op_cano_l(Exec& E, const Op& op) {
   double* Mt = E.sc.alloc((int64_t)m * n);
   if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
   const double* ap = E.ptr(a);
-  // Mt[(y + d*z), x] = a[x, y, z]
+  // Mt[(y + d*z), x] = a[x, y, z]   (permutedims(a, (2,3,1)) reshaped)
   for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
     const int64_t row = idx % m;
     const int x = (int)(idx / m);
@@ -651,37 +835,31 @@ CATN_DN void op_cano_l(Exec& E, const Op& op) {
     Mt[idx] = ap[x + (int64_t)dl * (y + (int64_t)d * z)];
   }
   g.sync();
-  SvdResult R;
-  int sw = 0;
-  if (wg_svd(g, E.sc, Mt, m, n, m, R, &sw) < 0) {
-    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
-  }
-  E.bucket = size_bucket(R.r);
-  const int k = keep_cano(R.S, R.r, op.x);
-  const int64_t offa = E.alloc((int64_t)k * d * dr);
-  const int64_t offb = E.alloc((int64_t)b.dl * b.d * k);
+  Split sp;
+  if (!wg_split(E, Mt, m, n, m, true, 1, op.x, 0, sp)) return;
+  const int k = sp.k;
+  const int64_t rb = (int64_t)b.dl * b.d;
+  const int64_t offa = E.alloc((int64_t)k * m);
+  const int64_t offb = E.alloc(rb * k);
   if (E.status) return;
   double* na = E.st->arena + offa;
   double* nbp = E.st->arena + offb;
-  // new a[p, y, z] = U[y + d*z, p]
+  // new a[p, y, z] = L[y + d*z, p]
   for (int64_t idx = g.tid; idx < (int64_t)k * m; idx += g.nthr) {
     const int p = (int)(idx % k);
     const int64_t row = idx / k;
-    na[idx] = R.U[row + (int64_t)m * p];
+    na[idx] = sp.L[row + (int64_t)m * p];
   }
-  // new b[u, p] = sum_c b[u, c] * S[p] * V[c, p]   (b viewed (b.dl*b.d) x dl)
+  // new b[u, p] = sum_c b[u, c] R[p, c]
   const double* bp = E.ptr(b);
-  const int64_t rb = (int64_t)b.dl * b.d;
   for (int64_t idx = g.tid; idx < rb * k; idx += g.nthr) {
     const int64_t u = idx % rb;
     const int p = (int)(idx / rb);
     double s = 0.0;
-    const double* v = R.V + (int64_t)p * n;
-    for (int c = 0; c < n; ++c) s += bp[u + rb * c] * v[c];
-    nbp[idx] = s * R.S[p];
+    for (int c = 0; c < n; ++c) s += bp[u + rb * c] * sp.R[p + (int64_t)k * c];
+    nbp[idx] = s;
   }
-  E.work(svd_flops(m, n) + 2.0 * (double)rb * n * k,
-         (double)m * n + (double)rb * n + (double)k * m + (double)rb * k);
+  E.work(2.0 * (double)rb * n * k, (double)m * n + (double)rb * n + (double)k * m + (double)rb * k);
   E.set_site(op.a, offa, k, d, dr);
   E.set_site(op.b, offb, b.dl, b.d, k);
   g.sync();
@@ -713,32 +891,16 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
     M[idx] = s;
   }
   g.sync();
-  SvdResult R;
-  int sw = 0;
-  if (wg_svd(g, E.sc, M, m, n, m, R, &sw) < 0) {
-    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
-  }
-  E.bucket = size_bucket(R.r);
-  const int k = keep_tsvd(R.S, R.r, op.x, op.d);
-  if (op.e) E.acc(&E.st->err, tail_sum(R.S, k, R.r));
+  Split sp;
+  if (!wg_split(E, M, m, n, m, op.c != 0, 0, op.x, op.d, sp)) return;
+  const int k = sp.k;
+  if (op.e) E.acc(&E.st->err, sp.disc);
   const int64_t offl = E.alloc((int64_t)m * k);
   const int64_t offr = E.alloc((int64_t)k * n);
   if (E.status) return;
-  double* nl = E.st->arena + offl;
-  double* nr = E.st->arena + offr;
-  const bool right = op.c != 0;
-  for (int64_t idx = g.tid; idx < (int64_t)m * k; idx += g.nthr) {
-    const int p = (int)(idx / m);
-    nl[idx] = right ? R.U[idx] : R.U[idx] * R.S[p];
-  }
-  // right site [p, c] = (S[p]) V[c, p]
-  for (int64_t idx = g.tid; idx < (int64_t)k * n; idx += g.nthr) {
-    const int p = (int)(idx % k);
-    const int64_t c = idx / k;
-    const double v = R.V[c + (int64_t)n * p];
-    nr[idx] = right ? R.S[p] * v : v;
-  }
-  E.work(2.0 * m * n * (double)dd + svd_flops(m, n),
+  wg_copy(g, E.st->arena + offl, sp.L, (int64_t)m * k);
+  wg_copy(g, E.st->arena + offr, sp.R, (int64_t)k * n);
+  E.work(2.0 * m * n * (double)dd,
          (double)d0 * d2 * dd + (double)dd * d1 * d3 + (double)m * k + (double)k * n);
   E.set_site(op.a, offl, d0, d1, k);
   E.set_site(op.b, offr, k, d2, d3);
@@ -1010,42 +1172,38 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
   g.sync();
 }
 
-// compress! / compress_opt! two-site step
+// compress! / compress_opt! two-site step (src/mps_node.jl:363-372, :407-443)
 CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
   const WG& g = E.g;
   SiteDesc A = E.site(op.a), B = E.site(op.b);
   const int d0 = A.dl, d1 = A.d, dd = A.dr, d2 = B.d, d3 = B.dr;
   if (B.dl != dd) { E.status = STATUS_SHAPE; return; }
   const int m = d0 * d1, n = d2 * d3;
-  SvdResult R;
-  int sw = 0;
   double *Ql = nullptr, *Qr = nullptr;
   bool fl = false, fr = false;
   int rl = m, rr = n;
+  double* M = nullptr;
   if (!opt) {
-    double* M = E.sc.alloc((int64_t)m * n);
+    M = E.sc.alloc((int64_t)m * n);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
     wg_gemm_nn(g, m, n, dd, E.ptr(A), m, E.ptr(B), dd, M, m);
     g.sync();
-    if (wg_svd(g, E.sc, M, m, n, m, R, &sw) < 0) {
-      E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
-    }
   } else {
-    // matl (m x dd), matr (dd x n)
+    // matl (m x dd), matr^T (n x dd)
     double* Ml = E.sc.alloc((int64_t)m * dd);
-    double* Mr = E.sc.alloc((int64_t)n * dd);   // matr^T (n x dd)
+    double* Mr = E.sc.alloc((int64_t)n * dd);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
     wg_copy(g, Ml, E.ptr(A), (int64_t)m * dd);
     wg_transpose(g, Mr, E.ptr(B), dd, n);
     g.sync();
     double* Rl = Ml;
-    double* Rr = Mr;   // Rr is n x dd (or dd x dd after QR)
+    double* Rr = Mr;
     fl = m > dd;
     fr = dd < n;
     if (fl) { if (!wg_thin_qr(g, E.sc, Ml, m, dd, &Ql, &Rl)) { E.status = STATUS_SCRATCH_FULL; return; } rl = dd; }
     if (fr) { if (!wg_thin_qr(g, E.sc, Mr, n, dd, &Qr, &Rr)) { E.status = STATUS_SCRATCH_FULL; return; } rr = dd; }
     // mat = Rl * Rr^T   (rl x rr)
-    double* M = E.sc.alloc((int64_t)rl * rr);
+    M = E.sc.alloc((int64_t)rl * rr);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
     for (int64_t idx = g.tid; idx < (int64_t)rl * rr; idx += g.nthr) {
       const int i = (int)(idx % rl), j = (int)(idx / rl);
@@ -1054,42 +1212,39 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
       M[idx] = s;
     }
     g.sync();
-    if (wg_svd(g, E.sc, M, rl, rr, rl, R, &sw) < 0) {
-      E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
-    }
+    E.work((fl ? 2.0 * qr_flops(m, dd) : 0.0) + (fr ? 2.0 * qr_flops(n, dd) : 0.0) +
+               2.0 * rl * rr * (double)dd, 0.0);
   }
-  const int k = keep_tsvd(R.S, R.r, op.x, op.d);
-  // left factor U*S (rl x k) -> Ql*(U S); right factor V (rr x k) -> Qr*V
-  double* Fl = E.sc.alloc((int64_t)rl * k);
-  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-  for (int64_t idx = g.tid; idx < (int64_t)rl * k; idx += g.nthr) {
-    const int p = (int)(idx / rl);
-    Fl[idx] = R.U[idx] * R.S[p];
-  }
-  g.sync();
-  double* Gl = Fl;
-  const double* Gr = R.V;
+  Split sp;
+  if (!wg_split(E, M, rl, rr, rl, false, 0, op.x, op.d, sp)) return;
+  const int k = sp.k;
+  // left = Ql * (U S)  (m x k); right = (V') Qr^T  (k x n)
+  double* Gl = sp.L;
+  double* Gr = sp.R;
   if (fl) {
     Gl = E.sc.alloc((int64_t)m * k);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    wg_gemm_nn(g, m, k, dd, Ql, m, Fl, dd, Gl, m);
+    wg_gemm_nn(g, m, k, dd, Ql, m, sp.L, dd, Gl, m);
   }
   if (fr) {
-    double* t = E.sc.alloc((int64_t)n * k);
+    Gr = E.sc.alloc((int64_t)k * n);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    wg_gemm_nn(g, n, k, dd, Qr, n, R.V, dd, t, n);
-    Gr = t;
+    for (int64_t idx = g.tid; idx < (int64_t)k * n; idx += g.nthr) {
+      const int p = (int)(idx % k);
+      const int c = (int)(idx / k);
+      double s = 0.0;
+      for (int q = 0; q < dd; ++q) s += sp.R[p + (int64_t)k * q] * Qr[c + (int64_t)n * q];
+      Gr[idx] = s;
+    }
   }
   g.sync();
   const int64_t offa = E.alloc((int64_t)m * k);
   const int64_t offb = E.alloc((int64_t)k * n);
   if (E.status) return;
   wg_copy(g, E.st->arena + offa, Gl, (int64_t)m * k);
-  wg_transpose(g, E.st->arena + offb, Gr, n, k);   // (k x n) = V^T
-  E.work(opt ? ((fl ? 2.0 * qr_flops(m, dd) + 2.0 * m * dd * (double)k : 0.0) +
-                 (fr ? 2.0 * qr_flops(n, dd) + 2.0 * n * dd * (double)k : 0.0) +
-                 2.0 * rl * rr * (double)dd + svd_flops(rl, rr))
-              : 2.0 * m * n * (double)dd + svd_flops(m, n),
+  wg_copy(g, E.st->arena + offb, Gr, (int64_t)k * n);
+  E.work(opt ? (fl ? 2.0 * m * dd * (double)k : 0.0) + (fr ? 2.0 * n * dd * (double)k : 0.0)
+             : 2.0 * m * n * (double)dd,
          (double)m * dd + (double)dd * n + (double)m * k + (double)k * n);
   E.set_site(op.a, offa, d0, d1, k);
   E.set_site(op.b, offb, k, d2, d3);
@@ -1102,24 +1257,15 @@ CATN_DN void op_raw_split(Exec& E, const Op& op) {
   const int dleft = s.dl, rest = s.d, di = op.c;
   if (rest % di != 0) { E.status = STATUS_SHAPE; return; }
   const int m = dleft * di, n = rest / di;
-  SvdResult R;
-  int sw = 0;
-  if (wg_svd(g, E.sc, E.ptr(s), m, n, m, R, &sw) < 0) {
-    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
-  }
-  E.bucket = size_bucket(R.r);
-  const int k = keep_tsvd(R.S, R.r, op.x, op.d);
+  Split sp;
+  if (!wg_split(E, E.ptr(s), m, n, m, true, 0, op.x, op.d, sp)) return;
+  const int k = sp.k;
   const int64_t offu = E.alloc((int64_t)m * k);
   const int64_t offr = E.alloc((int64_t)k * n);
   if (E.status) return;
-  wg_copy(g, E.st->arena + offu, R.U, (int64_t)m * k);
-  double* nr = E.st->arena + offr;
-  for (int64_t idx = g.tid; idx < (int64_t)k * n; idx += g.nthr) {
-    const int p = (int)(idx % k);
-    const int64_t c = idx / k;
-    nr[idx] = R.S[p] * R.V[c + (int64_t)n * p];
-  }
-  E.work(svd_flops(m, n), (double)m * n + (double)m * k + (double)k * n);
+  wg_copy(g, E.st->arena + offu, sp.L, (int64_t)m * k);
+  wg_copy(g, E.st->arena + offr, sp.R, (int64_t)k * n);
+  E.work(0.0, (double)m * n + (double)m * k + (double)k * n);
   E.set_site(op.b, offu, dleft, di, k);
   E.set_site(op.a, offr, k, n, 1);
   g.sync();
