@@ -268,6 +268,209 @@ inline double* lds_base() {
 #endif
 
 // ---------------------------------------------------------------------------
+// LDS-tiled FP64 GEMM for one workgroup:
+//   C[out(r, c)] = alpha * sum_k opA(r, k) * opB(k, c) + beta * C[out(r, c)]
+// opA = A (M x K, lda) or A^T (A stored K x M); opB = B (K x N) or B^T.
+// 64 x 64 output tiles, K chunks of 16 staged in LDS (16 KB at the top of the
+// dynamic LDS region), 4 x 4 register micro-tile per lane (256 "virtual
+// lanes" per tile; with fewer real lanes each one loops over several).
+// ---------------------------------------------------------------------------
+constexpr int kTM = 64, kTN = 64, kTK = 16;
+
+struct IdxPlain {
+  int64_t ldc;
+  CATN_D int64_t operator()(int r, int c) const { return r + ldc * c; }
+};
+
+template <class Out>
+CATN_D void wg_gemm_t(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
+                      bool tA, const double* B, int64_t ldb, bool tB, double beta, double* C,
+                      Out out) {
+  double* As = lds_base() + (g.lds_cap - 2 * kTK * kTM);   // [kTK][kTM]
+  double* Bs = As + kTK * kTM;                              // [kTK][kTN]
+  for (int r0 = 0; r0 < M; r0 += kTM) {
+    for (int c0 = 0; c0 < N; c0 += kTN) {
+      for (int vbase = 0; vbase < 256; vbase += g.nthr) {
+        const int vt = vbase + g.tid;
+        const int tx = vt & 15, ty = vt >> 4;
+        double acc[4][4];
+        for (int u = 0; u < 4; ++u)
+          for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+        for (int k0 = 0; k0 < K; k0 += kTK) {
+          g.sync();
+          for (int e = g.tid; e < kTK * kTM; e += g.nthr) {
+            const int kk = e / kTM, rr = e % kTM;
+            const int r = r0 + rr, k = k0 + kk;
+            double va = 0.0;
+            if (r < M && k < K) va = tA ? A[k + lda * (int64_t)r] : A[r + lda * (int64_t)k];
+            As[e] = va;
+          }
+          for (int e = g.tid; e < kTK * kTN; e += g.nthr) {
+            const int kk = e / kTN, cc = e % kTN;
+            const int c = c0 + cc, k = k0 + kk;
+            double vb = 0.0;
+            if (c < N && k < K) vb = tB ? B[c + ldb * (int64_t)k] : B[k + ldb * (int64_t)c];
+            Bs[e] = vb;
+          }
+          g.sync();
+          if (vt < 256) {
+            const int kmax = (K - k0) < kTK ? (K - k0) : kTK;
+            for (int kk = 0; kk < kmax; ++kk) {
+              double a[4], b[4];
+              for (int u = 0; u < 4; ++u) a[u] = As[kk * kTM + ty + 16 * u];
+              for (int v = 0; v < 4; ++v) b[v] = Bs[kk * kTN + tx + 16 * v];
+              for (int u = 0; u < 4; ++u)
+                for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+            }
+          }
+        }
+        if (vt < 256) {
+          for (int u = 0; u < 4; ++u) {
+            const int r = r0 + ty + 16 * u;
+            if (r >= M) continue;
+            for (int v = 0; v < 4; ++v) {
+              const int c = c0 + tx + 16 * v;
+              if (c >= N) continue;
+              const int64_t o = out(r, c);
+              C[o] = beta == 0.0 ? alpha * acc[u][v] : alpha * acc[u][v] + beta * C[o];
+            }
+          }
+        }
+      }
+    }
+  }
+  g.sync();
+}
+
+CATN_DN void wg_gemm(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
+                     bool tA, const double* B, int64_t ldb, bool tB, double beta, double* C,
+                     int64_t ldc) {
+  IdxPlain o;
+  o.ldc = ldc;
+  wg_gemm_t(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, o);
+}
+
+// C = A * B (no transposes) through the tiled GEMM when the product is large
+// enough to amortise its barriers, else the direct per-element loop.
+CATN_D void wg_mm(const WG& g, int M, int N, int K, const double* A, int64_t lda,
+                  const double* B, int64_t ldb, double* C, int64_t ldc) {
+  if ((int64_t)M * N >= 2048 && K >= 8 && g.lds_cap >= 2 * kTK * kTM + 64)
+    wg_gemm(g, M, N, K, 1.0, A, lda, false, B, ldb, false, 0.0, C, ldc);
+  else
+    wg_gemm_nn(g, M, N, K, A, lda, B, ldb, C, ldc);
+}
+
+// output index of the swap theta: X[(i + d0 j), (a + d1 b)] -> M[(i + d0 a), (j + d2 b)]
+struct IdxTheta {
+  int d0, d1, d2;
+  int64_t m;
+  CATN_D int64_t operator()(int r, int c) const {
+    const int i = r % d0, j = r / d0, a = c % d1, b = c / d1;
+    return (int64_t)(i + d0 * a) + m * (int64_t)(j + d2 * b);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Blocked Householder QR (LAPACK dgeqrf structure): panels of kQB columns are
+// factorised by wg_geqr2; the trailing matrix is updated with the compact WY
+// form Q_p^T C = C - V (T^T (V^T C)) through two tiled GEMMs, so the trailing
+// matrix is streamed twice per panel instead of twice per column. The panel
+// T factors are kept (T_all, kQB x n) for wg_apply_q_blocked.
+// ---------------------------------------------------------------------------
+constexpr int kQB = 16;
+
+// T (jb x jb, upper) of H_0..H_{jb-1} = I - V T V^T from the unit-lower V
+// (rows x jb, ld rows) and tau (LAPACK dlarft, forward columnwise).
+CATN_D void wg_larft(const WG& g, const double* V, int rows, int jb, const double* tau, double* T,
+                     double* Y) {
+  // Y = V^T V (jb x jb)
+  wg_gemm(g, jb, jb, rows, 1.0, V, rows, true, V, rows, false, 0.0, Y, jb);
+  if (g.tid == 0) {
+    for (int i = 0; i < jb; ++i) {
+      for (int j = 0; j < i; ++j) {
+        double s = 0.0;
+        for (int l = j; l < i; ++l) s += T[j + kQB * l] * Y[l + jb * i];
+        T[j + kQB * i] = -tau[i] * s;
+      }
+      T[i + kQB * i] = tau[i];
+      for (int j = i + 1; j < jb; ++j) T[j + kQB * i] = 0.0;
+    }
+  }
+  g.sync();
+}
+
+// unit-lower copy of the panel's Householder vectors
+CATN_D void wg_panel_v(const WG& g, const double* A, int rows, int jb, int64_t lda, double* V) {
+  for (int64_t idx = g.tid; idx < (int64_t)rows * jb; idx += g.nthr) {
+    const int i = (int)(idx % rows), j = (int)(idx / rows);
+    V[idx] = i > j ? A[i + lda * j] : (i == j ? 1.0 : 0.0);
+  }
+  g.sync();
+}
+
+CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t lda, double* tau,
+                      double* Tall) {
+  if (n <= kQB || g.lds_cap < 2 * kTK * kTM + 64) {
+    wg_geqr2(g, A, m, n, lda, tau);
+    return false;   // unblocked: no T factors
+  }
+  double* V = sc.alloc((int64_t)m * kQB);
+  double* W = sc.alloc((int64_t)kQB * n);
+  double* W2 = sc.alloc((int64_t)kQB * n);
+  double* Y = sc.alloc(kQB * kQB);
+  if (sc.fail) return false;
+  for (int p = 0; p < n; p += kQB) {
+    const int jb = (n - p) < kQB ? (n - p) : kQB;
+    const int rows = m - p;
+    double* Ap = A + p + lda * (int64_t)p;
+    wg_geqr2(g, Ap, rows, jb, lda, tau + p);
+    wg_panel_v(g, Ap, rows, jb, lda, V);
+    double* T = Tall + (int64_t)kQB * p;   // kQB x jb block, ld kQB
+    wg_larft(g, V, rows, jb, tau + p, T, Y);
+    const int nc = n - p - jb;
+    if (nc > 0) {
+      double* Cp = Ap + lda * (int64_t)jb;
+      // W = V^T C   (jb x nc)
+      wg_gemm(g, jb, nc, rows, 1.0, V, rows, true, Cp, lda, false, 0.0, W, jb);
+      // W2 = T^T W
+      wg_gemm(g, jb, nc, jb, 1.0, T, kQB, true, W, jb, false, 0.0, W2, jb);
+      // C -= V W2
+      wg_gemm(g, rows, nc, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Cp, lda);
+    }
+  }
+  return true;
+}
+
+// X (m x k, ld ldx) := Q X with Q from wg_geqrf (blocked: Tall given) or
+// wg_geqr2 (Tall null); X's rows n..m-1 must be set by the caller.
+CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m, int n,
+                                int64_t lda, const double* tau, const double* Tall, double* X,
+                                int k, int64_t ldx) {
+  if (!Tall) {
+    wg_apply_q(g, A, m, n, lda, tau, X, k, ldx);
+    return;
+  }
+  const int64_t save = sc.top;
+  double* V = sc.alloc((int64_t)m * kQB);
+  double* W = sc.alloc((int64_t)kQB * k);
+  double* W2 = sc.alloc((int64_t)kQB * k);
+  if (sc.fail) return;
+  const int last = ((n - 1) / kQB) * kQB;
+  for (int p = last; p >= 0; p -= kQB) {
+    const int jb = (n - p) < kQB ? (n - p) : kQB;
+    const int rows = m - p;
+    wg_panel_v(g, A + p + lda * (int64_t)p, rows, jb, lda, V);
+    const double* T = Tall + (int64_t)kQB * p;
+    double* Xp = X + p;
+    // Q_p X = X - V (T (V^T X))
+    wg_gemm(g, jb, k, rows, 1.0, V, rows, true, Xp, ldx, false, 0.0, W, jb);
+    wg_gemm(g, jb, k, jb, 1.0, T, kQB, false, W, jb, false, 0.0, W2, jb);
+    wg_gemm(g, rows, k, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Xp, ldx);
+  }
+  sc.top = save;
+}
+
+// ---------------------------------------------------------------------------
 // one-sided Jacobi SVD: A (m x n, m >= n) -> A V with mutually orthogonal
 // columns; V (n x n) accumulates the rotations. Returns sweeps used, or -1
 // when not converged within maxsweeps. Round-robin ordering: n/2 disjoint
@@ -473,11 +676,15 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
   double* tau = nullptr;
   double* R = W;
   int64_t t0 = now_ticks();
+  double* Tall = nullptr;
+  bool blocked = false;
   if (use_qr) {
     tau = sc.alloc(n);
     R = sc.alloc((int64_t)n * n);
+    Tall = sc.alloc((int64_t)kQB * n);
     if (sc.fail) return -1;
-    wg_geqr2(g, W, m, n, m, tau);
+    blocked = wg_geqrf(g, sc, W, m, n, m, tau, Tall);
+    if (sc.fail) return -1;
     for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
       const int i = (int)(idx % n), j = (int)(idx / n);
       R[idx] = (i <= j) ? W[i + (int64_t)j * m] : 0.0;
@@ -501,7 +708,14 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
     t0 = now_ticks();
     jacobi_finish(g, R, n, n, n, V, n, key, rk, U, m, Ss, Vs);
   }
-  if (use_qr) wg_apply_q(g, W, m, n, m, tau, U, n, m);
+  if (use_qr) {
+    for (int64_t idx = g.tid; idx < (int64_t)(m - n) * n; idx += g.nthr) {
+      const int i = n + (int)(idx % (m - n)), c = (int)(idx / (m - n));
+      U[i + (int64_t)c * m] = 0.0;
+    }
+    g.sync();
+    wg_apply_q_blocked(g, sc, W, m, n, m, tau, blocked ? Tall : nullptr, U, n, m);
+  }
   sub_tick(g, 2, t0);
   if (g.tid == 0 && g.sub && sw > 0) g.sub[3] += sw;
   out.r = n;
@@ -554,25 +768,29 @@ struct Factor {
 };
 
 // 1 / ||R^{-1}||_F of an upper-triangular n x n R (ld ldr); 0 if singular.
-CATN_DN double wg_inv_fro_bound(const WG& g, const double* R, int n, int64_t ldr, double* xbuf) {
+// X = R^{-1} (n x n, ld n) is built row by row from the bottom: row i needs
+// only rows > i, so each step is one parallel pass over the columns j > i.
+CATN_DN double wg_inv_fro_bound(const WG& g, const double* R, int n, int64_t ldr, double* X) {
   double part = 0.0;
-  bool sing = false;
-  for (int j = g.tid; j < n; j += g.nthr) {
-    double* x = xbuf + (int64_t)j * n;     // column j of R^{-1}, rows 0..j
-    const double rjj = R[j + (int64_t)j * ldr];
-    if (rjj == 0.0) { sing = true; continue; }
-    x[j] = 1.0 / rjj;
-    part += x[j] * x[j];
-    for (int i = j - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int k = i + 1; k <= j; ++k) s += R[i + (int64_t)k * ldr] * x[k];
-      const double rii = R[i + (int64_t)i * ldr];
-      if (rii == 0.0) { sing = true; break; }
-      x[i] = -s / rii;
-      part += x[i] * x[i];
+  for (int i = n - 1; i >= 0; --i) {
+    const double rii = R[i + (int64_t)i * ldr];
+    if (rii == 0.0) return 0.0;      // uniform: every lane reads the same value
+    const double inv = 1.0 / rii;
+    for (int j = i + g.tid; j < n; j += g.nthr) {
+      double x;
+      if (j == i) {
+        x = inv;
+      } else {
+        double sacc = 0.0;
+        for (int k = i + 1; k <= j; ++k) sacc += R[i + (int64_t)k * ldr] * X[k + (int64_t)j * n];
+        x = -sacc * inv;
+      }
+      X[i + (int64_t)j * n] = x;
+      part += x * x;
     }
+    g.sync();
   }
-  const double tot = wg_sum(g, sing ? INFINITY : part);
+  const double tot = wg_sum(g, part);
   if (!(tot < INFINITY) || tot <= 0.0) return 0.0;
   return 1.0 / sqrt(tot);
 }
@@ -590,6 +808,7 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   double* tau = sc.alloc(r);
   double* R = sc.alloc((int64_t)r * r);
   double* X = sc.alloc((int64_t)r * r);
+  double* Tall = sc.alloc((int64_t)kQB * r);
   if (sc.fail) return false;
   double fpart = 0.0;
   for (int64_t idx = g.tid; idx < (int64_t)tm * r; idx += g.nthr) {
@@ -600,7 +819,8 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   }
   const double fro = sqrt(wg_sum(g, fpart));
   g.sync();
-  wg_geqr2(g, W, tm, r, tm, tau);
+  const bool blocked = wg_geqrf(g, sc, W, tm, r, tm, tau, Tall);
+  if (sc.fail) return false;
   for (int64_t idx = g.tid; idx < (int64_t)r * r; idx += g.nthr) {
     const int i = (int)(idx % r), j = (int)(idx / r);
     R[idx] = (i <= j) ? W[i + (int64_t)j * tm] : 0.0;
@@ -621,12 +841,12 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   if (need_q) {
     Q = sc.alloc((int64_t)tm * r);
     if (sc.fail) return false;
-    for (int64_t idx = g.tid; idx < (int64_t)r * r; idx += g.nthr) {
-      const int i = (int)(idx % r), j = (int)(idx / r);
-      Q[i + (int64_t)j * tm] = (i == j) ? 1.0 : 0.0;
+    for (int64_t idx = g.tid; idx < (int64_t)tm * r; idx += g.nthr) {
+      const int i = (int)(idx % tm), j = (int)(idx / tm);
+      Q[idx] = (i == j) ? 1.0 : 0.0;
     }
     g.sync();
-    wg_apply_q(g, W, tm, r, tm, tau, Q, r, tm);
+    wg_apply_q_blocked(g, sc, W, tm, r, tm, tau, blocked ? Tall : nullptr, Q, r, tm);
   }
   if (iso_left && !trans) {          // A = Q R: L = Q (m x n), Rt = R (n x n)
     F.L = Q;
@@ -811,7 +1031,7 @@ CATN_DN void op_cano_r(Exec& E, const Op& op) {
   if (E.status) return;
   wg_copy(g, E.st->arena + offa, sp.L, (int64_t)m * k);
   // new b = R * b   (k x n) * (n x nb)
-  wg_gemm_nn(g, k, (int)nb, n, sp.R, k, E.ptr(b), n, E.st->arena + offb, k);
+  wg_mm(g, k, (int)nb, n, sp.R, k, E.ptr(b), n, E.st->arena + offb, k);
   E.work(2.0 * k * n * (double)nb, (double)m * n + (double)n * nb + (double)m * k + (double)k * nb);
   E.set_site(op.a, offa, a.dl, a.d, k);
   E.set_site(op.b, offb, k, b.d, b.dr);
@@ -879,7 +1099,13 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
   const double* tl = E.ptr(L);
   const double* tr = E.ptr(Rr);
-  // M[(i + d0*a), (j + d2*b)] = sum_k tl[i, j, k] tr[k, a, b]
+  // M[(i + d0*a), (j + d2*b)] = sum_k tl[i, j, k] tr[k, a, b]: the GEMM
+  // (tl as (d0 d2) x dd) * (tr as dd x (d1 d3)) written through the index map
+  if ((int64_t)m * n >= 2048 && dd >= 8 && g.lds_cap >= 2 * kTK * kTM + 64) {
+    IdxTheta o;
+    o.d0 = d0; o.d1 = d1; o.d2 = d2; o.m = m;
+    wg_gemm_t(g, d0 * d2, d1 * d3, dd, 1.0, tl, (int64_t)d0 * d2, false, tr, dd, false, 0.0, M, o);
+  } else
   for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
     const int64_t row = idx % m, col = idx / m;
     const int i = (int)(row % d0), a = (int)(row / d0);
@@ -1014,12 +1240,12 @@ CATN_DN void op_eat_c(Exec& E, const Op& op) {
   if (t.dr != 1 || h.dl != 1 || h.d != d || p.dr != Dl) { E.status = STATUS_SHAPE; return; }
   double* mat = E.sc.alloc((int64_t)Dl * Dr);
   if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-  wg_gemm_nn(g, Dl, Dr, d, E.ptr(t), Dl, E.ptr(h), d, mat, Dl);
+  wg_mm(g, Dl, Dr, d, E.ptr(t), Dl, E.ptr(h), d, mat, Dl);
   g.sync();
   const int64_t rows = (int64_t)p.dl * p.d;
   const int64_t off = E.alloc(rows * Dr);
   if (E.status) return;
-  wg_gemm_nn(g, (int)rows, Dr, Dl, E.ptr(p), rows, mat, Dl, E.st->arena + off, rows);
+  wg_mm(g, (int)rows, Dr, Dl, E.ptr(p), rows, mat, Dl, E.st->arena + off, rows);
   E.work(2.0 * Dl * Dr * d + 2.0 * rows * Dl * Dr,
          (double)Dl * d + (double)d * Dr + rows * Dl + rows * Dr);
   E.set_site(op.a, off, p.dl, p.d, Dr);
@@ -1039,7 +1265,7 @@ CATN_DN void op_merge_fuse(Exec& E, const Op& op) {
   const int m = a.dl * a.d, n = b.d * b.dr, k = a.dr;
   const int64_t off = E.alloc((int64_t)m * n);
   if (E.status) return;
-  wg_gemm_nn(g, m, n, k, E.ptr(a), m, E.ptr(b), k, E.st->arena + off, m);
+  wg_mm(g, m, n, k, E.ptr(a), m, E.ptr(b), k, E.st->arena + off, m);
   E.work(2.0 * m * n * (double)k, (double)m * k + (double)k * n + (double)m * n);
   E.set_site(op.a, off, a.dl, a.d * b.d, b.dr);
   if (g.tid == 0) E.st->sites[op.b].live = 0;
@@ -1052,15 +1278,20 @@ CATN_DN bool wg_thin_qr(const WG& g, Scratch& sc, double* Wk, int m, int n, doub
   double* tau = sc.alloc(n);
   double* R = sc.alloc((int64_t)n * n);
   double* Q = sc.alloc((int64_t)m * n);
+  double* Tall = sc.alloc((int64_t)kQB * n);
   if (sc.fail) return false;
-  wg_geqr2(g, Wk, m, n, m, tau);
+  const bool blocked = wg_geqrf(g, sc, Wk, m, n, m, tau, Tall);
+  if (sc.fail) return false;
   for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     R[idx] = (i <= j) ? Wk[i + (int64_t)j * m] : 0.0;
-    Q[i + (int64_t)j * m] = (i == j) ? 1.0 : 0.0;
+  }
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+    const int i = (int)(idx % m), j = (int)(idx / m);
+    Q[idx] = (i == j) ? 1.0 : 0.0;
   }
   g.sync();
-  wg_apply_q(g, Wk, m, n, m, tau, Q, n, m);
+  wg_apply_q_blocked(g, sc, Wk, m, n, m, tau, blocked ? Tall : nullptr, Q, n, m);
   *Qo = Q;
   *Ro = R;
   return true;
@@ -1136,12 +1367,12 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
   if (fl) {
     Fi = E.sc.alloc((int64_t)mi * myd);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    wg_gemm_nn(g, mi, myd, d, Qi, mi, Ui, d, Fi, mi);
+    wg_mm(g, mi, myd, d, Qi, mi, Ui, d, Fi, mi);
   }
   if (fr) {
     Fj = E.sc.alloc((int64_t)mj * myd);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    wg_gemm_nn(g, mj, myd, d, Qj, mj, Uj, d, Fj, mj);
+    wg_mm(g, mj, myd, d, Qj, mj, Uj, d, Fj, mj);
   }
   g.sync();
   const int64_t offa = E.alloc((int64_t)dal * myd * dar);
@@ -1186,7 +1417,7 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
   if (!opt) {
     M = E.sc.alloc((int64_t)m * n);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    wg_gemm_nn(g, m, n, dd, E.ptr(A), m, E.ptr(B), dd, M, m);
+    wg_mm(g, m, n, dd, E.ptr(A), m, E.ptr(B), dd, M, m);
     g.sync();
   } else {
     // matl (m x dd), matr^T (n x dd)
@@ -1224,7 +1455,7 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
   if (fl) {
     Gl = E.sc.alloc((int64_t)m * k);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    wg_gemm_nn(g, m, k, dd, Ql, m, sp.L, dd, Gl, m);
+    wg_mm(g, m, k, dd, Ql, m, sp.L, dd, Gl, m);
   }
   if (fr) {
     Gr = E.sc.alloc((int64_t)k * n);
