@@ -78,6 +78,8 @@ _SIGS = {
     "catn_net_work": (C.c_int, [_P, _DP, _DP, _I64P]),
     "catn_net_svd_profile": (C.c_int, [_P, _I64P]),
     "catn_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
+    "catn_host_stats": (C.c_int, [_P, _DP, _DP]),
+    "catn_set_threads": (C.c_int, [_P, C.c_int32]),
     "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "catn_node_cano_to": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "catn_node_left_canonical": (C.c_int, [_P, C.c_int32]),
@@ -142,6 +144,16 @@ class Context:
         n, sec = C.c_int64(), C.c_double()
         self._check(self.lib.catn_kernel_stats(self.h, C.byref(n), C.byref(sec), int(bool(reset))))
         return n.value, sec.value
+
+    def host_stats(self):
+        """(planning seconds, device round-trip seconds) since the last reset."""
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.catn_host_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_threads(self, threads: int):
+        """Workgroup size of the interpreter kernel (64, 128 or 256)."""
+        self._check(self.lib.catn_set_threads(self.h, int(threads)))
 
     def close(self):
         if getattr(self, "h", None):

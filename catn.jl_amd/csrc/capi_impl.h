@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <exception>
 #include <map>
@@ -30,8 +31,9 @@ class Executor {
   virtual void dfree(void* p) = 0;
   virtual void h2d(void* dst, const void* src, size_t bytes) = 0;
   virtual void d2h(void* dst, const void* src, size_t bytes) = 0;
-  // Execute programs[k] on the instance whose device state is states[k]; one
-  // workgroup per instance; returns after completion.
+  // Execute programs[k] on instance k whose state (host mirror) is states[k]:
+  // the executor moves the states to the device, runs one workgroup per
+  // instance and writes the updated states back; returns after completion.
   virtual void run(const std::vector<InstState*>& states,
                    const std::vector<const std::vector<Op>*>& programs) = 0;
   // frequency of the device wall clock read by the op timers (Hz)
@@ -40,6 +42,10 @@ class Executor {
   // measured with events on the stream the kernel runs on)
   int64_t launches = 0;
   double kernel_seconds = 0.0;
+  // host-side time: planning, host<->device copies + launch overhead
+  double plan_seconds = 0.0, run_seconds = 0.0;
+  // workgroup size of the interpreter (64, 128 or 256 lanes)
+  int threads = 256;
 };
 
 Executor* make_executor(int device);   // provided by the executor TU
@@ -67,8 +73,9 @@ struct catn_net {
   std::vector<Staged> staged;
   int nslots = 0;
   // device state
-  catn::InstState hs{};          // host mirror
-  catn::InstState* ds = nullptr; // device copy
+  catn::InstState hs{};          // authoritative between launches (the executor
+                                 // stages it to the device for each launch)
+  bool allocated = false;
   int slot_cap = 0, res_cap = 0;
   int64_t arena_cap = 0, scratch_cap = 0;
   int64_t arena_hwm = 0;
@@ -76,11 +83,11 @@ struct catn_net {
   ~catn_net() { release(); }
 
   void release() {
-    if (!ctx || !ds) return;
+    if (!ctx || !allocated) return;
     catn::Executor* ex = ctx->ex.get();
     ex->dfree(hs.arena); ex->dfree(hs.scratch); ex->dfree(hs.sites);
-    ex->dfree(hs.results); ex->dfree(hs.dresults); ex->dfree(ds);
-    ds = nullptr;
+    ex->dfree(hs.results); ex->dfree(hs.dresults);
+    allocated = false;
   }
 
   int new_slot(int dl, int d, int dr, const double* data) {
@@ -96,7 +103,7 @@ struct catn_net {
 
   void ensure_device() {
     catn::Executor* ex = ctx->ex.get();
-    if (!ds) {
+    if (!allocated) {
       int64_t staged_elems = 0;
       for (auto& s : staged) staged_elems += ((int64_t)s.data.size() + 31) / 32 * 32;
       if (arena_cap <= 0) arena_cap = std::max<int64_t>(int64_t(1) << 24, 4 * staged_elems);
@@ -115,7 +122,7 @@ struct catn_net {
       hs.dresults = (double*)ex->dmalloc(sizeof(double) * res_cap);
       hs.nsites = slot_cap;
       hs.nresults = res_cap;
-      ds = (catn::InstState*)ex->dmalloc(sizeof(catn::InstState));
+      allocated = true;
       std::vector<catn::SiteDesc> zero(slot_cap);
       for (auto& z : zero) { z.off = 0; z.dl = z.d = z.dr = 0; z.live = 0; }
       ex->h2d(hs.sites, zero.data(), sizeof(catn::SiteDesc) * slot_cap);
@@ -152,13 +159,9 @@ struct catn_net {
       hs.arena_top = top;
       staged.clear();
     }
-    ex->h2d(ds, &hs, sizeof(catn::InstState));
   }
 
-  void pull_state() {
-    ctx->ex->d2h(&hs, ds, sizeof(catn::InstState));
-    arena_hwm = std::max<int64_t>(arena_hwm, hs.arena_top);
-  }
+  void pull_state() { arena_hwm = std::max<int64_t>(arena_hwm, hs.arena_top); }
 
   void check_status() {
     if (hs.status != catn::STATUS_OK) {
@@ -174,7 +177,6 @@ struct catn_net {
 
   void reset_acc() {
     hs.lnZ = 0.0; hs.err = 0.0; hs.psi = 1.0; hs.sign = 1.0; hs.lntail = 0.0;
-    if (ds) ctx->ex->h2d(ds, &hs, sizeof(catn::InstState));
   }
 
   int read_result(int idx) {
@@ -200,12 +202,17 @@ inline int fail(int code, const std::string& msg) {
 }
 
 // run the pending programs of several networks in one launch
+inline double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 inline void run_nets(Executor* ex, const std::vector<catn_net*>& nets) {
+  const double t0 = wall();
   std::vector<InstState*> st;
   std::vector<const std::vector<Op>*> pr;
   for (catn_net* n : nets) {
     n->ensure_device();
-    st.push_back(n->ds);
+    st.push_back(&n->hs);
     pr.push_back(&n->pl.prog);
   }
   ex->run(st, pr);
@@ -214,6 +221,7 @@ inline void run_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     n->pl.prog.clear();
     n->check_status();
   }
+  ex->run_seconds += wall() - t0;
 }
 
 inline void run_one(catn_net* n) { run_nets(n->ctx->ex.get(), {n}); }
@@ -364,15 +372,16 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
   while (!active.empty()) {
     std::vector<catn_net*> still;
     std::vector<int> cut(active.size());
+    const double t0 = wall();
     parallel_for((int)active.size(), [&](int k) {
       cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
     });
+    ex->plan_seconds += wall() - t0;
     run_nets(ex, active);
     for (size_t k = 0; k < active.size(); ++k) {
       catn_net* n = active[k];
       if (cut[k]) {
-        const int myd = n->read_result(n->pl.pending_cut_result());
-        n->pl.apply_cut(myd);
+        n->pl.apply_cut(n->hs.last_cut);
         still.push_back(n);
       } else {
         n->consumed = true;
@@ -635,7 +644,22 @@ int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t
   if (reset) {
     ctx->ex->launches = 0;
     ctx->ex->kernel_seconds = 0.0;
+    ctx->ex->plan_seconds = 0.0;
+    ctx->ex->run_seconds = 0.0;
   }
+  return 0;
+}
+
+int catn_host_stats(catn_ctx* ctx, double* plan_seconds, double* run_seconds) {
+  if (plan_seconds) *plan_seconds = ctx->ex->plan_seconds;
+  if (run_seconds) *run_seconds = ctx->ex->run_seconds;
+  return 0;
+}
+
+int catn_set_threads(catn_ctx* ctx, int32_t threads) {
+  if (threads != 64 && threads != 128 && threads != 256)
+    return catn::fail(-2, "catn_set_threads: threads must be 64, 128 or 256");
+  ctx->ex->threads = threads;
   return 0;
 }
 

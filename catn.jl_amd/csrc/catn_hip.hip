@@ -18,9 +18,8 @@ namespace catn {
 
 constexpr int kThreads = 256;
 
-__global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* const* states,
-                                                          const Op* ops, const int* ranges,
-                                                          int lds_cap) {
+__global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, const Op* ops,
+                                                             const int* ranges, int lds_cap) {
   __shared__ double red[16];
   __shared__ int flag;
   WG g;
@@ -30,8 +29,9 @@ __global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* const* s
   g.flag = &flag;
   g.lds_cap = lds_cap;
   const int b = blockIdx.x;
-  g.sub = states[b]->sub_ticks;
-  run_program(g, states[b], ops, ranges[2 * b], ranges[2 * b + 1]);
+  InstState* st = states + b;
+  g.sub = st->sub_ticks;
+  run_program(g, st, ops, ranges[2 * b], ranges[2 * b + 1]);
 }
 
 static void hipcheck(hipError_t e, const char* what) {
@@ -99,12 +99,15 @@ class HipExecutor : public Executor {
     }
     if (all.empty()) return;
     reserve(all.size(), nb);
+    // stage every instance state into one contiguous array: one copy each way
+    std::vector<InstState> packed(nb);
+    for (int b = 0; b < nb; ++b) packed[b] = *states[b];
     hipcheck(hipMemcpyAsync(d_ops_, all.data(), sizeof(Op) * all.size(), hipMemcpyHostToDevice, stream_), "ops");
     hipcheck(hipMemcpyAsync(d_ranges_, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice, stream_), "ranges");
-    hipcheck(hipMemcpyAsync(d_states_, states.data(), sizeof(InstState*) * nb, hipMemcpyHostToDevice, stream_), "states");
+    hipcheck(hipMemcpyAsync(d_states_, packed.data(), sizeof(InstState) * nb, hipMemcpyHostToDevice, stream_), "states");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
     const int lds_cap = lds_doubles_for_batch(nb);
-    hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(kThreads), sizeof(double) * lds_cap, stream_,
+    hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
                        d_states_, d_ops_, d_ranges_, lds_cap);
     hipcheck(hipGetLastError(), "launch exec_programs");
     hipcheck(hipEventRecord(ev1_, stream_), "event record");
@@ -113,6 +116,8 @@ class HipExecutor : public Executor {
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
     launches += 1;
     kernel_seconds += 1e-3 * ms;
+    hipcheck(hipMemcpy(packed.data(), d_states_, sizeof(InstState) * nb, hipMemcpyDeviceToHost), "states back");
+    for (int b = 0; b < nb; ++b) *states[b] = packed[b];
   }
 
  private:
@@ -127,7 +132,7 @@ class HipExecutor : public Executor {
       if (d_states_) (void)hipFree(d_states_);
       st_cap_ = (size_t)nb * 2;
       hipcheck(hipMalloc(&d_ranges_, sizeof(int) * 2 * st_cap_), "hipMalloc ranges");
-      hipcheck(hipMalloc(&d_states_, sizeof(InstState*) * st_cap_), "hipMalloc states");
+      hipcheck(hipMalloc(&d_states_, sizeof(InstState) * st_cap_), "hipMalloc states");
     }
   }
   int device_;
@@ -136,7 +141,7 @@ class HipExecutor : public Executor {
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   Op* d_ops_ = nullptr;
   int* d_ranges_ = nullptr;
-  InstState** d_states_ = nullptr;
+  InstState* d_states_ = nullptr;
   size_t ops_cap_ = 0, st_cap_ = 0;
   double tick_hz_ = 1e8;
 };

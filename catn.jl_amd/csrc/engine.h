@@ -1399,7 +1399,7 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
          (double)mi * d + (double)mj * d + (double)mi * myd + (double)mj * myd);
   E.set_site(op.a, offa, dal, myd, dar);
   E.set_site(op.b, offb, dbl, myd, dbr);
-  if (g.tid == 0) E.st->results[op.e] = myd;
+  if (g.tid == 0) { E.st->results[op.e] = myd; E.st->last_cut = myd; }
   g.sync();
 }
 

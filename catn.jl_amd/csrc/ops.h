@@ -103,6 +103,8 @@ struct InstState {
   int64_t scratch_hwm;
   // sub-step ticks inside the SVD: 0 QR, 1 Jacobi, 2 finish + Q apply, 3 sweeps
   int64_t sub_ticks[4];
+  int32_t last_cut;    // kept count of the last OP_CUT (read by the planner)
+  int32_t pad3;
 };
 
 enum Status : int32_t {

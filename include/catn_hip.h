@@ -107,6 +107,11 @@ int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t
 /* Ticks spent inside the large-SVD path: [QR, Jacobi, finish + Q apply,
  * Jacobi sweep count]. */
 int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks);
+/* Host time spent planning and in device round trips (launch, copies,
+ * waits) since the last catn_kernel_stats reset, in seconds. */
+int catn_host_stats(catn_ctx* ctx, double* plan_seconds, double* run_seconds);
+/* Workgroup size of the interpreter kernel: 64, 128 or 256 lanes (default 256). */
+int catn_set_threads(catn_ctx* ctx, int32_t threads);
 /* Device memory statistics: arena high-water mark and capacity (elements). */
 int catn_net_mem(catn_net* net, int64_t* arena_used, int64_t* arena_cap);
 
