@@ -169,11 +169,20 @@ CATN_D void wg_transpose(const WG& g, double* dst, const double* src, int m, int
 // A (m x n, m >= n) overwritten by R (upper) and v (below the diagonal).
 // ---------------------------------------------------------------------------
 CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double* tau) {
+  // Three barriers per column: the reflector v = [1; scal * x(1:)] is applied
+  // with the scaling folded into the update, and the stored sub-diagonal of
+  // column j is scaled during the norm pass of column j+1 (disjoint memory).
+  int pend_col = -1;
+  double pend_scal = 1.0;
   for (int j = 0; j < n; ++j) {
     double* x = A + j + (int64_t)j * lda;
     double part = 0.0;
     for (int i = 1 + g.tid; i < m - j; i += g.nthr) part += x[i] * x[i];
-    const double s = wg_sum(g, part);
+    if (pend_col >= 0) {
+      double* xp = A + pend_col + (int64_t)pend_col * lda;
+      for (int i = 1 + g.tid; i < m - pend_col; i += g.nthr) xp[i] *= pend_scal;
+    }
+    const double s = wg_sum(g, part);   // barriers: pending scaling now visible
     const double alpha = x[0];
     double t = 0.0, beta = alpha, scal = 0.0;
     if (s > 0.0) {
@@ -181,17 +190,6 @@ CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double*
       beta = alpha >= 0.0 ? -nrm : nrm;
       t = (beta - alpha) / beta;
       scal = 1.0 / (alpha - beta);
-    }
-    g.sync();
-    if (s > 0.0) {
-      for (int i = 1 + g.tid; i < m - j; i += g.nthr) x[i] *= scal;
-    }
-    if (g.tid == 0) { tau[j] = t; }
-    g.sync();
-    if (s > 0.0) {
-      if (g.tid == 0) x[0] = 1.0;   // temporarily, for the update
-      g.sync();
-      // apply H to the trailing columns: one group of lanes per column
       const int ncol = n - j - 1;
       if (ncol > 0) {
         int L = pow2floor(g.nthr / ncol < 1 ? 1 : g.nthr / ncol);
@@ -200,16 +198,28 @@ CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double*
         for (int c = grp; c < ncol; c += ng) {
           double* y = A + j + (int64_t)(j + 1 + c) * lda;
           double w = 0.0;
-          for (int i = lane; i < m - j; i += L) w += x[i] * y[i];
-          w = grp_sum(w, L) * t;
-          for (int i = lane; i < m - j; i += L) y[i] -= w * x[i];
+          for (int i = 1 + lane; i < m - j; i += L) w += x[i] * y[i];
+          w = grp_sum(w, L);
+          w = t * (y[0] + scal * w);
+          if (lane == 0) y[0] -= w;
+          const double ws = w * scal;
+          for (int i = 1 + lane; i < m - j; i += L) y[i] -= ws * x[i];
         }
       }
-      g.sync();
-      if (g.tid == 0) x[0] = beta;
+    }
+    if (g.tid == 0) {
+      tau[j] = t;
+      x[0] = beta;
     }
     g.sync();
+    pend_col = s > 0.0 ? j : -1;
+    pend_scal = scal;
   }
+  if (pend_col >= 0) {
+    double* xp = A + pend_col + (int64_t)pend_col * lda;
+    for (int i = 1 + g.tid; i < m - pend_col; i += g.nthr) xp[i] *= pend_scal;
+  }
+  g.sync();
 }
 
 // U (m x k) := Q * U where Q = H_0 ... H_{n-1} from wg_geqr2 (stored in A);
@@ -298,19 +308,25 @@ CATN_D void wg_gemm_t(const WG& g, int M, int N, int K, double alpha, const doub
           for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
         for (int k0 = 0; k0 < K; k0 += kTK) {
           g.sync();
+          // tile loads walk the operand's contiguous (leading) index across
+          // consecutive lanes so that global reads coalesce
           for (int e = g.tid; e < kTK * kTM; e += g.nthr) {
-            const int kk = e / kTM, rr = e % kTM;
+            int kk, rr;
+            if (tA) { kk = e % kTK; rr = e / kTK; }   // A[k + lda r]: k contiguous
+            else { kk = e / kTM; rr = e % kTM; }      // A[r + lda k]: r contiguous
             const int r = r0 + rr, k = k0 + kk;
             double va = 0.0;
             if (r < M && k < K) va = tA ? A[k + lda * (int64_t)r] : A[r + lda * (int64_t)k];
-            As[e] = va;
+            As[kk * kTM + rr] = va;
           }
           for (int e = g.tid; e < kTK * kTN; e += g.nthr) {
-            const int kk = e / kTN, cc = e % kTN;
+            int kk, cc;
+            if (tB) { kk = e / kTN; cc = e % kTN; }   // B[c + ldb k]: c contiguous
+            else { kk = e % kTK; cc = e / kTK; }      // B[k + ldb c]: k contiguous
             const int c = c0 + cc, k = k0 + kk;
             double vb = 0.0;
             if (c < N && k < K) vb = tB ? B[c + ldb * (int64_t)k] : B[k + ldb * (int64_t)c];
-            Bs[e] = vb;
+            Bs[kk * kTN + cc] = vb;
           }
           g.sync();
           if (vt < 256) {
@@ -804,14 +820,30 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   // tall orientation T (tm x r): A itself when m >= n, else A^T
   const bool trans = m < n;
   const int tm = trans ? n : m;
-  double* W = sc.alloc((int64_t)tm * r);
-  double* tau = sc.alloc(r);
-  double* R = sc.alloc((int64_t)r * r);
-  double* X = sc.alloc((int64_t)r * r);
-  double* Tall = sc.alloc((int64_t)kQB * r);
-  if (sc.fail) return false;
+  // the isometric factor is Q of the tall orientation when that side is the
+  // one QR'd (iso_left & !trans, or !iso_left & trans); otherwise identity
+  const bool need_q = iso_left != trans;
+  const int64_t tmr = (int64_t)tm * r, rr = (int64_t)r * r;
+  // small operands: W, R, R^{-1} (and Q) in LDS, unblocked Householder
+  const bool in_lds = tmr * (need_q ? 2 : 1) + 2 * rr + r + 64 <= g.lds_cap;
+  double *W, *R, *X, *Q = nullptr, *tau, *Tall = nullptr;
+  if (in_lds) {
+    W = lds_base();
+    R = W + tmr;
+    X = R + rr;
+    tau = X + rr;
+    if (need_q) Q = tau + r;
+  } else {
+    W = sc.alloc(tmr);
+    tau = sc.alloc(r);
+    R = sc.alloc(rr);
+    X = sc.alloc(rr);
+    Tall = sc.alloc((int64_t)kQB * r);
+    if (need_q) Q = sc.alloc(tmr);
+    if (sc.fail) return false;
+  }
   double fpart = 0.0;
-  for (int64_t idx = g.tid; idx < (int64_t)tm * r; idx += g.nthr) {
+  for (int64_t idx = g.tid; idx < tmr; idx += g.nthr) {
     const int i = (int)(idx % tm), j = (int)(idx / tm);
     const double v = trans ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
     W[idx] = v;
@@ -819,62 +851,56 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   }
   const double fro = sqrt(wg_sum(g, fpart));
   g.sync();
-  const bool blocked = wg_geqrf(g, sc, W, tm, r, tm, tau, Tall);
+  int64_t t0 = now_ticks();
+  bool blocked = false;
+  if (in_lds) wg_geqr2(g, W, tm, r, tm, tau);
+  else blocked = wg_geqrf(g, sc, W, tm, r, tm, tau, Tall);
   if (sc.fail) return false;
-  for (int64_t idx = g.tid; idx < (int64_t)r * r; idx += g.nthr) {
+  sub_tick(g, 4, t0);
+  t0 = now_ticks();
+  for (int64_t idx = g.tid; idx < rr; idx += g.nthr) {
     const int i = (int)(idx % r), j = (int)(idx / r);
     R[idx] = (i <= j) ? W[i + (int64_t)j * tm] : 0.0;
   }
   g.sync();
   const double lb = wg_inv_fro_bound(g, R, r, r, X);
+  sub_tick(g, 5, t0);
+  t0 = now_ticks();
   const double thr = fmax(100.0 * cutoff, 1e-12 * fro);
   if (!(lb > thr)) {
     g.sync();
     sc.top = save;
     return false;
   }
-  // the isometric factor is Q of the tall orientation when that side is the
-  // one QR'd (iso_left & !trans, or !iso_left & trans); otherwise identity
   F.r = r;
-  const bool need_q = iso_left != trans;
-  double* Q = nullptr;
   if (need_q) {
-    Q = sc.alloc((int64_t)tm * r);
-    if (sc.fail) return false;
-    for (int64_t idx = g.tid; idx < (int64_t)tm * r; idx += g.nthr) {
+    for (int64_t idx = g.tid; idx < tmr; idx += g.nthr) {
       const int i = (int)(idx % tm), j = (int)(idx / tm);
       Q[idx] = (i == j) ? 1.0 : 0.0;
     }
     g.sync();
-    wg_apply_q_blocked(g, sc, W, tm, r, tm, tau, blocked ? Tall : nullptr, Q, r, tm);
+    if (in_lds) wg_apply_q(g, W, tm, r, tm, tau, Q, r, tm);
+    else wg_apply_q_blocked(g, sc, W, tm, r, tm, tau, blocked ? Tall : nullptr, Q, r, tm);
+    sub_tick(g, 6, t0);
   }
+  // outputs in scratch (LDS is reused by the caller's GEMMs)
+  double* L = sc.alloc((int64_t)m * r);
+  double* Rt = sc.alloc((int64_t)r * n);
+  if (sc.fail) return false;
   if (iso_left && !trans) {          // A = Q R: L = Q (m x n), Rt = R (n x n)
-    F.L = Q;
-    F.Rt = R;
+    for (int64_t idx = g.tid; idx < tmr; idx += g.nthr) L[idx] = Q[idx];
+    for (int64_t idx = g.tid; idx < rr; idx += g.nthr) Rt[idx] = R[idx];
   } else if (iso_left && trans) {    // rank m: L = I_m, Rt = A (m x n)
-    double* L = sc.alloc((int64_t)m * m);
-    double* Rt = sc.alloc((int64_t)m * n);
-    if (sc.fail) return false;
     for (int64_t idx = g.tid; idx < (int64_t)m * m; idx += g.nthr)
       L[idx] = (idx % m == idx / m) ? 1.0 : 0.0;
     for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr)
       Rt[idx] = A[(idx % m) + (idx / m) * lda];
-    F.L = L;
-    F.Rt = Rt;
   } else if (!iso_left && !trans) {  // rank n: L = A (m x n), Rt = I_n
-    double* L = sc.alloc((int64_t)m * n);
-    double* Rt = sc.alloc((int64_t)n * n);
-    if (sc.fail) return false;
     for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr)
       L[idx] = A[(idx % m) + (idx / m) * lda];
     for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr)
       Rt[idx] = (idx % n == idx / n) ? 1.0 : 0.0;
-    F.L = L;
-    F.Rt = Rt;
   } else {                           // A^T = Q' R': L = R'^T (m x m), Rt = Q'^T (m x n)
-    double* L = sc.alloc((int64_t)m * m);
-    double* Rt = sc.alloc((int64_t)m * n);
-    if (sc.fail) return false;
     for (int64_t idx = g.tid; idx < (int64_t)m * m; idx += g.nthr) {
       const int i = (int)(idx % m), j = (int)(idx / m);
       L[idx] = R[j + (int64_t)i * r];
@@ -883,9 +909,9 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
       const int i = (int)(idx % m), j = (int)(idx / m);
       Rt[idx] = Q[j + (int64_t)i * tm];
     }
-    F.L = L;
-    F.Rt = Rt;
   }
+  F.L = L;
+  F.Rt = Rt;
   g.sync();
   return true;
 }
@@ -1101,6 +1127,7 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   const double* tr = E.ptr(Rr);
   // M[(i + d0*a), (j + d2*b)] = sum_k tl[i, j, k] tr[k, a, b]: the GEMM
   // (tl as (d0 d2) x dd) * (tr as dd x (d1 d3)) written through the index map
+  const int64_t tth = now_ticks();
   if ((int64_t)m * n >= 2048 && dd >= 8 && g.lds_cap >= 2 * kTK * kTM + 64) {
     IdxTheta o;
     o.d0 = d0; o.d1 = d1; o.d2 = d2; o.m = m;
@@ -1117,6 +1144,7 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
     M[idx] = s;
   }
   g.sync();
+  sub_tick(g, 7, tth);
   Split sp;
   if (!wg_split(E, M, m, n, m, op.c != 0, 0, op.x, op.d, sp)) return;
   const int k = sp.k;

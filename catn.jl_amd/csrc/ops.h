@@ -101,8 +101,9 @@ struct InstState {
   // read once and written once), and the scratch high-water mark (elements)
   double alg_flops, alg_bytes;
   int64_t scratch_hwm;
-  // sub-step ticks inside the SVD: 0 QR, 1 Jacobi, 2 finish + Q apply, 3 sweeps
-  int64_t sub_ticks[4];
+  // sub-step ticks: SVD path 0 QR, 1 Jacobi, 2 finish + Q apply, 3 sweeps;
+  // certified-QR path 4 QR, 5 inverse bound, 6 Q formation; 7 swap theta
+  int64_t sub_ticks[8];
   int32_t last_cut;    // kept count of the last OP_CUT (read by the planner)
   int32_t pad3;
 };
