@@ -395,12 +395,50 @@ struct IdxTheta {
 // ---------------------------------------------------------------------------
 constexpr int kQB = 16;
 
+// Tall-skinny products of the blocked Householder updates (jb <= kQB rows of
+// V^T). C (jb x N, ldc) = V^T B with V (rows x jb, ldv), B (rows x N, ldb):
+// one wavefront per output column, its lanes split the long dimension and
+// each lane keeps jb partial sums; a shuffle tree reduces them. No workgroup
+// barrier inside the column loop.
+CATN_D void wg_vtb(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
+                   const double* B, int64_t ldb, double* C, int64_t ldc) {
+  const int W = g.nthr >= 64 ? 64 : g.nthr;
+  const int nw = g.nthr / W, wv = g.tid / W, lane = g.tid % W;
+  for (int c = wv; c < N; c += nw) {
+    double acc[kQB];
+    for (int r = 0; r < kQB; ++r) acc[r] = 0.0;
+    const double* b = B + ldb * (int64_t)c;
+    for (int i = lane; i < rows; i += W) {
+      const double bi = b[i];
+      for (int r = 0; r < jb; ++r) acc[r] += V[i + ldv * (int64_t)r] * bi;
+    }
+    for (int r = 0; r < jb; ++r) {
+      const double t = grp_sum(acc[r], W);
+      if (lane == 0) C[r + ldc * (int64_t)c] = t;
+    }
+  }
+  g.sync();
+}
+
+// C (rows x N, ldc) -= V (rows x jb, ldv) * Wm (jb x N, ldw), element-parallel
+// with coalesced access down the columns.
+CATN_D void wg_rank_update(const WG& g, int rows, int N, int jb, const double* V, int64_t ldv,
+                           const double* Wm, int64_t ldw, double* C, int64_t ldc) {
+  for (int64_t idx = g.tid; idx < (int64_t)rows * N; idx += g.nthr) {
+    const int i = (int)(idx % rows), c = (int)(idx / rows);
+    double s = 0.0;
+    for (int r = 0; r < jb; ++r) s += V[i + ldv * (int64_t)r] * Wm[r + ldw * (int64_t)c];
+    C[i + ldc * (int64_t)c] -= s;
+  }
+  g.sync();
+}
+
 // T (jb x jb, upper) of H_0..H_{jb-1} = I - V T V^T from the unit-lower V
 // (rows x jb, ld rows) and tau (LAPACK dlarft, forward columnwise).
 CATN_D void wg_larft(const WG& g, const double* V, int rows, int jb, const double* tau, double* T,
                      double* Y) {
   // Y = V^T V (jb x jb)
-  wg_gemm(g, jb, jb, rows, 1.0, V, rows, true, V, rows, false, 0.0, Y, jb);
+  wg_vtb(g, jb, jb, rows, V, rows, V, rows, Y, jb);
   if (g.tid == 0) {
     for (int i = 0; i < jb; ++i) {
       for (int j = 0; j < i; ++j) {
@@ -411,6 +449,23 @@ CATN_D void wg_larft(const WG& g, const double* V, int rows, int jb, const doubl
       T[i + kQB * i] = tau[i];
       for (int j = i + 1; j < jb; ++j) T[j + kQB * i] = 0.0;
     }
+  }
+  g.sync();
+}
+
+// W2 (jb x N, ld jb) = op(T) W (jb x N, ld jb); T upper triangular, ld kQB;
+// op(T) = T^T when trans.
+CATN_D void wg_gemm_nn_t(const WG& g, int jb, int N, const double* T, const double* W, double* W2,
+                         bool trans) {
+  for (int64_t idx = g.tid; idx < (int64_t)jb * N; idx += g.nthr) {
+    const int r = (int)(idx % jb), c = (int)(idx / jb);
+    double s = 0.0;
+    if (trans) {
+      for (int q = 0; q <= r; ++q) s += T[q + kQB * r] * W[q + (int64_t)jb * c];
+    } else {
+      for (int q = r; q < jb; ++q) s += T[r + kQB * q] * W[q + (int64_t)jb * c];
+    }
+    W2[idx] = s;
   }
   g.sync();
 }
@@ -426,7 +481,7 @@ CATN_D void wg_panel_v(const WG& g, const double* A, int rows, int jb, int64_t l
 
 CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t lda, double* tau,
                       double* Tall) {
-  if (n <= kQB || g.lds_cap < 2 * kTK * kTM + 64) {
+  if (n <= kQB) {
     wg_geqr2(g, A, m, n, lda, tau);
     return false;   // unblocked: no T factors
   }
@@ -447,11 +502,11 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     if (nc > 0) {
       double* Cp = Ap + lda * (int64_t)jb;
       // W = V^T C   (jb x nc)
-      wg_gemm(g, jb, nc, rows, 1.0, V, rows, true, Cp, lda, false, 0.0, W, jb);
+      wg_vtb(g, jb, nc, rows, V, rows, Cp, lda, W, jb);
       // W2 = T^T W
-      wg_gemm(g, jb, nc, jb, 1.0, T, kQB, true, W, jb, false, 0.0, W2, jb);
+      wg_gemm_nn_t(g, jb, nc, T, W, W2, true);
       // C -= V W2
-      wg_gemm(g, rows, nc, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Cp, lda);
+      wg_rank_update(g, rows, nc, jb, V, rows, W2, jb, Cp, lda);
     }
   }
   return true;
@@ -479,9 +534,9 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
     const double* T = Tall + (int64_t)kQB * p;
     double* Xp = X + p;
     // Q_p X = X - V (T (V^T X))
-    wg_gemm(g, jb, k, rows, 1.0, V, rows, true, Xp, ldx, false, 0.0, W, jb);
-    wg_gemm(g, jb, k, jb, 1.0, T, kQB, false, W, jb, false, 0.0, W2, jb);
-    wg_gemm(g, rows, k, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Xp, ldx);
+    wg_vtb(g, jb, k, rows, V, rows, Xp, ldx, W, jb);
+    wg_gemm_nn_t(g, jb, k, T, W, W2, false);
+    wg_rank_update(g, rows, k, jb, V, rows, W2, jb, Xp, ldx);
   }
   sc.top = save;
 }
