@@ -292,59 +292,64 @@ struct IdxPlain {
   CATN_D int64_t operator()(int r, int c) const { return r + ldc * c; }
 };
 
-template <class Out>
-CATN_D void wg_gemm_t(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
-                      bool tA, const double* B, int64_t ldb, bool tB, double beta, double* C,
-                      Out out) {
-  double* As = lds_base() + (g.lds_cap - 2 * kTK * kTM);   // [kTK][kTM]
-  double* Bs = As + kTK * kTM;                              // [kTK][kTN]
-  for (int r0 = 0; r0 < M; r0 += kTM) {
-    for (int c0 = 0; c0 < N; c0 += kTN) {
+// Tile shape TM x TN, K chunk TK, micro-tile (TM/16) x (TN/16) per lane over a
+// 16 x 16 grid of 256 virtual lanes. Square 64x64x16 for general products;
+// 16x128x32 for the tall-skinny V^T C of the Householder updates (4x fewer
+// barriers for long K).
+template <int TM, int TN, int TK, class Out>
+CATN_D void wg_gemm_tile(const WG& g, int M, int N, int K, double alpha, const double* A,
+                         int64_t lda, bool tA, const double* B, int64_t ldb, bool tB, double beta,
+                         double* C, Out out) {
+  constexpr int MU = TM / 16, MV = TN / 16;
+  double* As = lds_base() + (g.lds_cap - TK * (TM + TN));   // [TK][TM]
+  double* Bs = As + TK * TM;                                 // [TK][TN]
+  for (int r0 = 0; r0 < M; r0 += TM) {
+    for (int c0 = 0; c0 < N; c0 += TN) {
       for (int vbase = 0; vbase < 256; vbase += g.nthr) {
         const int vt = vbase + g.tid;
         const int tx = vt & 15, ty = vt >> 4;
-        double acc[4][4];
-        for (int u = 0; u < 4; ++u)
-          for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-        for (int k0 = 0; k0 < K; k0 += kTK) {
+        double acc[MU][MV];
+        for (int u = 0; u < MU; ++u)
+          for (int v = 0; v < MV; ++v) acc[u][v] = 0.0;
+        for (int k0 = 0; k0 < K; k0 += TK) {
           g.sync();
           // tile loads walk the operand's contiguous (leading) index across
           // consecutive lanes so that global reads coalesce
-          for (int e = g.tid; e < kTK * kTM; e += g.nthr) {
+          for (int e = g.tid; e < TK * TM; e += g.nthr) {
             int kk, rr;
-            if (tA) { kk = e % kTK; rr = e / kTK; }   // A[k + lda r]: k contiguous
-            else { kk = e / kTM; rr = e % kTM; }      // A[r + lda k]: r contiguous
+            if (tA) { kk = e % TK; rr = e / TK; }   // A[k + lda r]: k contiguous
+            else { kk = e / TM; rr = e % TM; }      // A[r + lda k]: r contiguous
             const int r = r0 + rr, k = k0 + kk;
             double va = 0.0;
             if (r < M && k < K) va = tA ? A[k + lda * (int64_t)r] : A[r + lda * (int64_t)k];
-            As[kk * kTM + rr] = va;
+            As[kk * TM + rr] = va;
           }
-          for (int e = g.tid; e < kTK * kTN; e += g.nthr) {
+          for (int e = g.tid; e < TK * TN; e += g.nthr) {
             int kk, cc;
-            if (tB) { kk = e / kTN; cc = e % kTN; }   // B[c + ldb k]: c contiguous
-            else { kk = e % kTK; cc = e / kTK; }      // B[k + ldb c]: k contiguous
+            if (tB) { kk = e / TN; cc = e % TN; }   // B[c + ldb k]: c contiguous
+            else { kk = e % TK; cc = e / TK; }      // B[k + ldb c]: k contiguous
             const int c = c0 + cc, k = k0 + kk;
             double vb = 0.0;
             if (c < N && k < K) vb = tB ? B[c + ldb * (int64_t)k] : B[k + ldb * (int64_t)c];
-            Bs[kk * kTN + cc] = vb;
+            Bs[kk * TN + cc] = vb;
           }
           g.sync();
           if (vt < 256) {
-            const int kmax = (K - k0) < kTK ? (K - k0) : kTK;
+            const int kmax = (K - k0) < TK ? (K - k0) : TK;
             for (int kk = 0; kk < kmax; ++kk) {
-              double a[4], b[4];
-              for (int u = 0; u < 4; ++u) a[u] = As[kk * kTM + ty + 16 * u];
-              for (int v = 0; v < 4; ++v) b[v] = Bs[kk * kTN + tx + 16 * v];
-              for (int u = 0; u < 4; ++u)
-                for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+              double a[MU], b[MV];
+              for (int u = 0; u < MU; ++u) a[u] = As[kk * TM + ty + 16 * u];
+              for (int v = 0; v < MV; ++v) b[v] = Bs[kk * TN + tx + 16 * v];
+              for (int u = 0; u < MU; ++u)
+                for (int v = 0; v < MV; ++v) acc[u][v] += a[u] * b[v];
             }
           }
         }
         if (vt < 256) {
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < MU; ++u) {
             const int r = r0 + ty + 16 * u;
             if (r >= M) continue;
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < MV; ++v) {
               const int c = c0 + tx + 16 * v;
               if (c >= N) continue;
               const int64_t o = out(r, c);
@@ -356,6 +361,13 @@ CATN_D void wg_gemm_t(const WG& g, int M, int N, int K, double alpha, const doub
     }
   }
   g.sync();
+}
+
+template <class Out>
+CATN_D void wg_gemm_t(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
+                      bool tA, const double* B, int64_t ldb, bool tB, double beta, double* C,
+                      Out out) {
+  wg_gemm_tile<kTM, kTN, kTK>(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, out);
 }
 
 CATN_DN void wg_gemm(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
@@ -395,42 +407,15 @@ struct IdxTheta {
 // ---------------------------------------------------------------------------
 constexpr int kQB = 16;
 
-// Tall-skinny products of the blocked Householder updates (jb <= kQB rows of
-// V^T). C (jb x N, ldc) = V^T B with V (rows x jb, ldv), B (rows x N, ldb):
-// one wavefront per output column, its lanes split the long dimension and
-// each lane keeps jb partial sums; a shuffle tree reduces them. No workgroup
-// barrier inside the column loop.
-CATN_D void wg_vtb(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
-                   const double* B, int64_t ldb, double* C, int64_t ldc) {
-  const int W = g.nthr >= 64 ? 64 : g.nthr;
-  const int nw = g.nthr / W, wv = g.tid / W, lane = g.tid % W;
-  for (int c = wv; c < N; c += nw) {
-    double acc[kQB];
-    for (int r = 0; r < kQB; ++r) acc[r] = 0.0;
-    const double* b = B + ldb * (int64_t)c;
-    for (int i = lane; i < rows; i += W) {
-      const double bi = b[i];
-      for (int r = 0; r < jb; ++r) acc[r] += V[i + ldv * (int64_t)r] * bi;
-    }
-    for (int r = 0; r < jb; ++r) {
-      const double t = grp_sum(acc[r], W);
-      if (lane == 0) C[r + ldc * (int64_t)c] = t;
-    }
-  }
-  g.sync();
-}
-
-// C (rows x N, ldc) -= V (rows x jb, ldv) * Wm (jb x N, ldw), element-parallel
-// with coalesced access down the columns.
-CATN_D void wg_rank_update(const WG& g, int rows, int N, int jb, const double* V, int64_t ldv,
-                           const double* Wm, int64_t ldw, double* C, int64_t ldc) {
-  for (int64_t idx = g.tid; idx < (int64_t)rows * N; idx += g.nthr) {
-    const int i = (int)(idx % rows), c = (int)(idx / rows);
-    double s = 0.0;
-    for (int r = 0; r < jb; ++r) s += V[i + ldv * (int64_t)r] * Wm[r + ldw * (int64_t)c];
-    C[i + ldc * (int64_t)c] -= s;
-  }
-  g.sync();
+// C (jb x N) = V^T B for jb <= 16 with the skinny tile (LDS budget permitting)
+CATN_DN void wg_gemm_skinny_tn(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
+                               const double* B, int64_t ldb, double* C, int64_t ldc) {
+  IdxPlain o;
+  o.ldc = ldc;
+  if (g.lds_cap >= 32 * (16 + 128) + 64)
+    wg_gemm_tile<16, 128, 32>(g, jb, N, rows, 1.0, V, ldv, true, B, ldb, false, 0.0, C, o);
+  else
+    wg_gemm_tile<kTM, kTN, kTK>(g, jb, N, rows, 1.0, V, ldv, true, B, ldb, false, 0.0, C, o);
 }
 
 // T (jb x jb, upper) of H_0..H_{jb-1} = I - V T V^T from the unit-lower V
@@ -438,7 +423,7 @@ CATN_D void wg_rank_update(const WG& g, int rows, int N, int jb, const double* V
 CATN_D void wg_larft(const WG& g, const double* V, int rows, int jb, const double* tau, double* T,
                      double* Y) {
   // Y = V^T V (jb x jb)
-  wg_vtb(g, jb, jb, rows, V, rows, V, rows, Y, jb);
+  wg_gemm_skinny_tn(g, jb, jb, rows, V, rows, V, rows, Y, jb);
   if (g.tid == 0) {
     for (int i = 0; i < jb; ++i) {
       for (int j = 0; j < i; ++j) {
@@ -502,11 +487,11 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     if (nc > 0) {
       double* Cp = Ap + lda * (int64_t)jb;
       // W = V^T C   (jb x nc)
-      wg_vtb(g, jb, nc, rows, V, rows, Cp, lda, W, jb);
+      wg_gemm_skinny_tn(g, jb, nc, rows, V, rows, Cp, lda, W, jb);
       // W2 = T^T W
       wg_gemm_nn_t(g, jb, nc, T, W, W2, true);
       // C -= V W2
-      wg_rank_update(g, rows, nc, jb, V, rows, W2, jb, Cp, lda);
+      wg_gemm(g, rows, nc, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Cp, lda);
     }
   }
   return true;
@@ -534,9 +519,9 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
     const double* T = Tall + (int64_t)kQB * p;
     double* Xp = X + p;
     // Q_p X = X - V (T (V^T X))
-    wg_vtb(g, jb, k, rows, V, rows, Xp, ldx, W, jb);
+    wg_gemm_skinny_tn(g, jb, k, rows, V, rows, Xp, ldx, W, jb);
     wg_gemm_nn_t(g, jb, k, T, W, W2, false);
-    wg_rank_update(g, rows, k, jb, V, rows, W2, jb, Xp, ldx);
+    wg_gemm(g, rows, k, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Xp, ldx);
   }
   sc.top = save;
 }
