@@ -968,6 +968,36 @@ struct Exec {
   int bucket;      // size bucket of the op's main SVD (-1: none)
   double flops, bytes;   // algorithmic work of this launch segment
   CATN_D void work(double f, double elems) { flops += f; bytes += 8.0 * elems; }
+  // tape record of the current op: lane 0 writes it straight into the
+  // instance's tape when the instance records (rp is null elsewhere)
+  TapeRec* rp;
+  int tape_len;
+  CATN_D void rec_begin(int code) {
+    rp = nullptr;
+    if (!st->record) return;
+    if (tape_len >= st->tape_cap) { status = STATUS_TAPE_FULL; return; }
+    if (g.tid != 0) return;
+    rp = st->tape + tape_len;
+    rp->code = code; rp->nin = 0; rp->nout = 0; rp->k = 0; rp->path = 0; rp->flag = 0;
+    rp->aux0 = 0; rp->aux1 = 0; rp->val = 0.0; rp->val2 = 0.0;
+  }
+  CATN_D void rec_in(const SiteDesc& d) {
+    if (!rp) return;
+    TapeSite& t = rp->in[rp->nin++];
+    t.off = d.off; t.dl = d.dl; t.d = d.d; t.dr = d.dr;
+  }
+  CATN_D void rec_out(int64_t off, int dl, int d, int dr) {
+    if (!rp) return;
+    TapeSite& t = rp->out[rp->nout++];
+    t.off = off; t.dl = dl; t.d = d; t.dr = dr;
+  }
+  CATN_D void rec_set(int k, int path, int flag, int aux0, double val) {
+    if (!rp) return;
+    rp->k = k; rp->path = path; rp->flag = flag; rp->aux0 = aux0; rp->val = val;
+  }
+  CATN_D void rec_commit() {
+    if (st->record && status == STATUS_OK) ++tape_len;
+  }
 
   CATN_D SiteDesc site(int s) const {
     volatile SiteDesc* p = st->sites + s;
@@ -1039,6 +1069,7 @@ struct Split {
   double* L;
   double* R;
   double disc;
+  int qr;   // 1 when the certified QR factorisation was used
 };
 
 CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool iso_left, int rule,
@@ -1049,7 +1080,7 @@ CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool 
   const int r0 = m < n ? m : n;
   E.bucket = size_bucket(r0);
   if (wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, F)) {
-    sp.k = F.r; sp.L = F.L; sp.R = F.Rt; sp.disc = 0.0;
+    sp.k = F.r; sp.L = F.L; sp.R = F.Rt; sp.disc = 0.0; sp.qr = 1;
     const double tm = m > n ? m : n;
     E.work(qr_flops(tm, r0) + r0 * (double)r0 * r0 / 3.0 + (iso_left != (m < n) ? qr_flops(tm, r0) : 0.0),
            0.0);
@@ -1079,7 +1110,7 @@ CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool 
     Rm[idx] = iso_left ? R.S[p] * v : v;
   }
   g.sync();
-  sp.k = k; sp.L = L; sp.R = Rm; sp.disc = tail_sum(R.S, k, R.r);
+  sp.k = k; sp.L = L; sp.R = Rm; sp.disc = tail_sum(R.S, k, R.r); sp.qr = 0;
   return true;
 }
 
@@ -1099,6 +1130,10 @@ CATN_DN void op_cano_r(Exec& E, const Op& op) {
   // new b = R * b   (k x n) * (n x nb)
   wg_mm(g, k, (int)nb, n, sp.R, k, E.ptr(b), n, E.st->arena + offb, k);
   E.work(2.0 * k * n * (double)nb, (double)m * n + (double)n * nb + (double)m * k + (double)k * nb);
+  E.rec_begin(OP_CANO_R); E.rec_in(a); E.rec_in(b);
+  E.rec_out(offa, a.dl, a.d, k); E.rec_out(offb, k, b.d, b.dr);
+  E.rec_set(k, sp.qr, 0, 0, op.x);
+  E.rec_commit();
   E.set_site(op.a, offa, a.dl, a.d, k);
   E.set_site(op.b, offb, k, b.d, b.dr);
   g.sync();
@@ -1146,6 +1181,10 @@ CATN_DN void op_cano_l(Exec& E, const Op& op) {
     nbp[idx] = s;
   }
   E.work(2.0 * (double)rb * n * k, (double)m * n + (double)rb * n + (double)k * m + (double)rb * k);
+  E.rec_begin(OP_CANO_L); E.rec_in(a); E.rec_in(b);
+  E.rec_out(offa, k, d, dr); E.rec_out(offb, b.dl, b.d, k);
+  E.rec_set(k, sp.qr, 0, 0, op.x);
+  E.rec_commit();
   E.set_site(op.a, offa, k, d, dr);
   E.set_site(op.b, offb, b.dl, b.d, k);
   g.sync();
@@ -1196,6 +1235,10 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   wg_copy(g, E.st->arena + offr, sp.R, (int64_t)k * n);
   E.work(2.0 * m * n * (double)dd,
          (double)d0 * d2 * dd + (double)dd * d1 * d3 + (double)m * k + (double)k * n);
+  E.rec_begin(OP_SWAP); E.rec_in(L); E.rec_in(Rr);
+  E.rec_out(offl, d0, d1, k); E.rec_out(offr, k, d2, d3);
+  E.rec_set(k, sp.qr, op.c, op.d, op.x);
+  E.rec_commit();
   E.set_site(op.a, offl, d0, d1, k);
   E.set_site(op.b, offr, k, d2, d3);
   g.sync();
@@ -1217,6 +1260,8 @@ CATN_DN void op_permute321(Exec& E, const Op& op) {
     dst[idx] = s[x + (int64_t)a.dl * (y + (int64_t)a.d * z)];
   }
   E.work(0.0, 2.0 * tot);
+  E.rec_begin(OP_PERMUTE321); E.rec_in(a); E.rec_out(off, a.dr, a.d, a.dl);
+  E.rec_commit();
   E.set_site(op.a, off, a.dr, a.d, a.dl);
   g.sync();
 }
@@ -1237,6 +1282,9 @@ CATN_DN void op_eat_a(Exec& E, const Op& op) {
     E.acc(&E.st->lnZ, log(ar));
     if (g.tid == 0) E.st->psi = E.st->psi * (r / ar);
   }
+  E.rec_begin(OP_EAT_A); E.rec_in(a); E.rec_in(b);
+  E.rec_set(0, 0, ar > op.x ? 1 : 0, 0, r);
+  E.rec_commit();
   if (g.tid == 0) E.st->sites[op.a].live = 0;
   g.sync();
 }
@@ -1254,17 +1302,26 @@ CATN_D double site_norm(const WG& g, const double* x, int64_t n, int method) {
   return 1.0;
 }
 
+// Normalise site `slot` per norm_method into a new tensor version (the
+// previous version stays for the reverse sweep); log(norm) is added to lnZ
+// unless norm <= cutoff (src/mps_node.jl:506-527, :558-574).
 CATN_D void scale_normalize(Exec& E, int slot, const SiteDesc& s, int method, double cutoff) {
   const WG& g = E.g;
-  double* x = E.ptr(s);
+  const double* x = E.ptr(s);
   const int64_t n = (int64_t)s.dl * s.d * s.dr;
   const double nrm = site_norm(g, x, n, method);
   E.work(3.0 * n, 2.0 * n);
-  if (nrm <= cutoff) { g.sync(); return; }   // src/mps_node.jl:522-524, :569-571
-  const double inv = 1.0 / nrm;
-  for (int64_t i = g.tid; i < n; i += g.nthr) x[i] = x[i] * inv;
-  E.acc(&E.st->lnZ, log(nrm));
-  (void)slot;
+  const int64_t off = E.alloc(n);
+  if (E.status) return;
+  double* y = E.st->arena + off;
+  const bool counted = nrm > cutoff;   // src/mps_node.jl:522-524, :569-571
+  const double inv = counted ? 1.0 / nrm : 1.0;
+  for (int64_t i = g.tid; i < n; i += g.nthr) y[i] = x[i] * inv;
+  if (counted) E.acc(&E.st->lnZ, log(nrm));
+  E.rec_begin(OP_NORMALIZE); E.rec_in(s); E.rec_out(off, s.dl, s.d, s.dr);
+  E.rec_set(0, 0, counted ? 1 : 0, method, nrm);
+  E.rec_commit();
+  E.set_site(slot, off, s.dl, s.d, s.dr);
   g.sync();
 }
 
@@ -1293,12 +1350,12 @@ CATN_DN void op_eat_b(Exec& E, const Op& op) {
     for (int q = 0; q < D; ++q) s += pp[r + rows * q] * mat[q];
     np_[r] = s;
   }
-  E.work(2.0 * D * d + 2.0 * rows * D + 3.0 * rows, (double)D * d + d + rows * D + rows);
+  E.work(2.0 * D * d + 2.0 * rows * D, (double)D * d + d + rows * D + rows);
+  E.rec_begin(OP_EAT_B); E.rec_in(p); E.rec_in(t); E.rec_in(j); E.rec_out(off, p.dl, p.d, 1);
+  E.rec_commit();
   E.set_site(op.a, off, p.dl, p.d, 1);
   if (g.tid == 0) { E.st->sites[op.b].live = 0; E.st->sites[op.c].live = 0; }
   g.sync();
-  SiteDesc nn = E.site(op.a);
-  scale_normalize(E, op.a, nn, op.d, op.x);
 }
 
 CATN_DN void op_eat_c(Exec& E, const Op& op) {
@@ -1316,6 +1373,8 @@ CATN_DN void op_eat_c(Exec& E, const Op& op) {
   wg_mm(g, (int)rows, Dr, Dl, E.ptr(p), rows, mat, Dl, E.st->arena + off, rows);
   E.work(2.0 * Dl * Dr * d + 2.0 * rows * Dl * Dr,
          (double)Dl * d + (double)d * Dr + rows * Dl + rows * Dr);
+  E.rec_begin(OP_EAT_C); E.rec_in(p); E.rec_in(t); E.rec_in(h); E.rec_out(off, p.dl, p.d, Dr);
+  E.rec_commit();
   E.set_site(op.a, off, p.dl, p.d, Dr);
   if (g.tid == 0) { E.st->sites[op.b].live = 0; E.st->sites[op.c].live = 0; }
   g.sync();
@@ -1335,6 +1394,8 @@ CATN_DN void op_merge_fuse(Exec& E, const Op& op) {
   if (E.status) return;
   wg_mm(g, m, n, k, E.ptr(a), m, E.ptr(b), k, E.st->arena + off, m);
   E.work(2.0 * m * n * (double)k, (double)m * k + (double)k * n + (double)m * n);
+  E.rec_begin(OP_MERGE_FUSE); E.rec_in(a); E.rec_in(b); E.rec_out(off, a.dl, a.d * b.d, b.dr);
+  E.rec_commit();
   E.set_site(op.a, off, a.dl, a.d * b.d, b.dr);
   if (g.tid == 0) E.st->sites[op.b].live = 0;
   g.sync();
@@ -1465,6 +1526,10 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
              (fr ? 2.0 * qr_flops(mj, d) + 2.0 * mj * d * (double)myd : 0.0) +
              2.0 * ri_rows * rj_rows * (double)d + svd_flops(ri_rows, rj_rows),
          (double)mi * d + (double)mj * d + (double)mi * myd + (double)mj * myd);
+  E.rec_begin(OP_CUT); E.rec_in(A); E.rec_in(B);
+  E.rec_out(offa, dal, myd, dar); E.rec_out(offb, dbl, myd, dbr);
+  E.rec_set(myd, 0, op.f, 0, op.x);
+  E.rec_commit();
   E.set_site(op.a, offa, dal, myd, dar);
   E.set_site(op.b, offb, dbl, myd, dbr);
   if (g.tid == 0) { E.st->results[op.e] = myd; E.st->last_cut = myd; }
@@ -1545,6 +1610,7 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
   E.work(opt ? (fl ? 2.0 * m * dd * (double)k : 0.0) + (fr ? 2.0 * n * dd * (double)k : 0.0)
              : 2.0 * m * n * (double)dd,
          (double)m * dd + (double)dd * n + (double)m * k + (double)k * n);
+  E.rec_begin(opt ? OP_COMPRESS_OPT : OP_COMPRESS); E.rec_commit();
   E.set_site(op.a, offa, d0, d1, k);
   E.set_site(op.b, offb, k, d2, d3);
   g.sync();
@@ -1565,6 +1631,7 @@ CATN_DN void op_raw_split(Exec& E, const Op& op) {
   wg_copy(g, E.st->arena + offu, sp.L, (int64_t)m * k);
   wg_copy(g, E.st->arena + offr, sp.R, (int64_t)k * n);
   E.work(0.0, (double)m * n + (double)m * k + (double)k * n);
+  E.rec_begin(OP_RAW_SPLIT); E.rec_commit();
   E.set_site(op.b, offu, dleft, di, k);
   E.set_site(op.a, offr, k, n, 1);
   g.sync();
@@ -1579,6 +1646,8 @@ CATN_DN void op_scalar_lognorm(Exec& E, const Op& op) {
   for (int64_t i = g.tid; i < n; i += g.nthr) part += x[i];
   const double z = wg_sum(g, part);
   E.acc(&E.st->lntail, log(fabs(z)));
+  E.rec_begin(OP_SCALAR_LOGNORM); E.rec_in(s); E.rec_set(0, 0, 0, 0, z);
+  E.rec_commit();
   if (g.tid == 0) E.st->sign = E.st->sign * (z > 0.0 ? 1.0 : (z < 0.0 ? -1.0 : 0.0));
   g.sync();
 }
@@ -1620,6 +1689,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
   E.status = st->status;
   E.flops = 0.0;
   E.bytes = 0.0;
+  E.tape_len = st->tape_len;
   int64_t shwm = 0;
   for (int i = begin; i < end && E.status == STATUS_OK; ++i) {
     const Op op = ops[i];
@@ -1665,6 +1735,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
   if (g.tid == 0) {
     st->arena_top = E.top;
     st->status = E.status;
+    st->tape_len = E.tape_len;
     st->alg_flops += E.flops;
     st->alg_bytes += E.bytes;
     if (shwm > st->scratch_hwm) st->scratch_hwm = shwm;

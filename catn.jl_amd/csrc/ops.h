@@ -56,6 +56,8 @@ enum OpCode : int32_t {
   // lognorm of a scalar node: z = sum(site a); lnZ += log|z|; sign *= sign(z)
   // (src/mps_node.jl:319-328, src/tensor_network.jl:489-501)
   OP_SCALAR_LOGNORM = 14,
+  // reverse-mode step of the tape record a (adjoint sweep)
+  OP_BACKWARD = 16,
   // standalone truncated SVD of the matrix held in slot a viewed as
   // (dl*d) x dr: U -> slot b (m x k), S*V' kept in slot c (k x n), S -> result
   // (src/linalg_utils.jl:15-28); d = maxdim, e = result index
@@ -72,6 +74,26 @@ struct Op {
 struct SiteDesc {
   int64_t off;   // element offset into the instance arena
   int32_t dl, d, dr, live;
+};
+
+// One tensor version referenced by a tape record (the arena is append-only
+// while recording, so every version stays resident for the reverse sweep).
+struct TapeSite {
+  int64_t off;
+  int32_t dl, d, dr;
+};
+
+// Tape record of one executed op: its input and output tensor versions and
+// the decisions the reverse sweep must replay (factorisation path, kept
+// count, norms, phases).
+struct TapeRec {
+  int32_t code, nin, nout, k;
+  int32_t path;    // split ops: 1 certified QR, 0 Jacobi SVD
+  int32_t flag;    // op specific (direction, counted log term, ...)
+  int32_t aux0, aux1;
+  TapeSite in[3];
+  TapeSite out[2];
+  double val, val2;
 };
 
 // Per-instance device state.
@@ -105,7 +127,10 @@ struct InstState {
   // certified-QR path 4 QR, 5 inverse bound, 6 Q formation; 7 swap theta
   int64_t sub_ticks[8];
   int32_t last_cut;    // kept count of the last OP_CUT (read by the planner)
-  int32_t pad3;
+  int32_t record;      // 1: write a tape record per executed op
+  TapeRec* tape;       // tape records (device), tape_cap entries
+  double* adj;         // adjoint arena, same indexing as `arena`
+  int32_t tape_len, tape_cap;
 };
 
 enum Status : int32_t {
@@ -115,6 +140,7 @@ enum Status : int32_t {
   STATUS_SHAPE = 3,
   STATUS_NOCONV = 4,
   STATUS_UNSUPPORTED = 5,
+  STATUS_TAPE_FULL = 6,
 };
 
 }  // namespace catn

@@ -184,7 +184,9 @@ class Planner {
     move2tail(ni, idx);
     if (nj.order() == 1) {                       // case (b)
       const int L = ni.order();
-      emit(OP_EAT_B, ni.slot[L - 2], ni.slot[L - 1], nj.slot[0], p.norm_method, 0, 0, p.cutoff);
+      emit(OP_EAT_B, ni.slot[L - 2], ni.slot[L - 1], nj.slot[0]);
+      // the folded tensor is stored, then normalised (src/mps_node.jl:506-527)
+      emit(OP_NORMALIZE, ni.slot[L - 2], 0, 0, p.norm_method, 0, 0, p.cutoff);
       ni.cano -= 1;
       ni.slot.pop_back(); ni.phys.pop_back();
       ni.nb.pop_back();
