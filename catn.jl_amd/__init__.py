@@ -9,7 +9,11 @@ over the C ABI of `libcatn_hip.so` (include/catn_hip.h):
   contraction_batch([tn...])                many networks per persistent launch
   free_energy(tn) -> (lnZ/n, F)             src/ising.jl:239-244
   magnetization(...), correlation(...)      src/ising.jl:263-339 (pinning; all
-                                            pinned networks run as one batch)
+                                            pinned networks run as one batch), or
+                                            method="adjoint": one reverse sweep
+  gradient_batch([tn...])                   lnZ + hand-written adjoint per network
+  ising_gradient(tn) -> (dw, dh, dbeta)     d lnZ / d(w, h, beta) after the sweep
+  lnz_and_grad(n, edges, w, h, beta; ...)   one ising solve: lnZ and its gradient
   tsvd(A; cutoff, maxdim)                   src/linalg_utils.jl:15-28
   MPSNet: hand-built MPSNode primitives      src/mps_node.jl (swap!, cano_to!, ...)
 
@@ -30,7 +34,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libcatn_hip.so")
 __all__ = [
     "Context", "TensorNetwork", "MPSNet", "ising_network", "tensor_network", "contraction",
     "contraction_batch", "free_energy", "magnetization", "correlation", "tsvd",
-    "default_context", "LIB_PATH", "CatnError",
+    "default_context", "LIB_PATH", "CatnError", "gradient_batch", "ising_gradient",
+    "lnz_and_grad", "lnz_and_grad_batch",
 ]
 
 
@@ -71,6 +76,12 @@ _SIGS = {
     "catn_net_free": (C.c_int, [_P]),
     "catn_contraction": (C.c_int, [_P, _DP, _DP, _DP]),
     "catn_contraction_batch": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _DP, _DP, _DP]),
+    "catn_gradient_batch": (C.c_int, [_P, C.c_int32, C.POINTER(_P), C.c_int32, _DP, _DP, _DP]),
+    "catn_ising_gradient": (C.c_int, [_P, _DP, _DP, _DP]),
+    "catn_net_site_adjoint": (C.c_int, [_P, C.c_int32, _DP, C.c_int64]),
+    "catn_net_bwd_profile": (C.c_int, [_P, _I64P, _DP, _I32P]),
+    "catn_net_tape": (C.c_int, [_P, _I32P, _DP, C.c_int32]),
+    "catn_bwd_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
     "catn_net_info": (C.c_int, [_P, _DP, _I32P, _I32P, _I32P, _DP]),
     "catn_net_edge_log": (C.c_int, [_P, _I32P, _I32P]),
     "catn_net_mem": (C.c_int, [_P, _I64P, _I64P]),
@@ -143,6 +154,13 @@ class Context:
         """(launches, seconds) of the interpreter kernel on this context."""
         n, sec = C.c_int64(), C.c_double()
         self._check(self.lib.catn_kernel_stats(self.h, C.byref(n), C.byref(sec), int(bool(reset))))
+        return n.value, sec.value
+
+    def bwd_kernel_stats(self, reset=False):
+        """(launches, seconds) of the reverse-sweep kernel on this context."""
+        n, sec = C.c_int64(), C.c_double()
+        self._check(self.lib.catn_bwd_kernel_stats(self.h, C.byref(n), C.byref(sec),
+                                                   int(bool(reset))))
         return n.value, sec.value
 
     def host_stats(self):
@@ -275,6 +293,32 @@ class TensorNetwork:
         buckets["_theta_s"] = sub[7] / hzv
         return ops, buckets
 
+    def bwd_profile(self):
+        """Reverse sweep: ({opname: seconds}, replay deviation, tape length)."""
+        t = np.zeros(16, np.int64)
+        dev = C.c_double()
+        tl = C.c_int32()
+        self.ctx._check(self.ctx.lib.catn_net_bwd_profile(self.h, t.ctypes.data_as(_I64P),
+                                                          C.byref(dev), C.byref(tl)))
+        hz = C.c_double()
+        self.ctx._check(self.ctx.lib.catn_net_profile(self.h, None, None, None, None, C.byref(hz)))
+        ops = {self.OPNAMES[k]: float(t[k]) / hz.value for k in range(16) if t[k]}
+        return ops, dev.value, tl.value
+
+    def tape(self):
+        """Tape records as an (n, 16) int array (see catn_net_tape) and their scalars."""
+        _, _, tl = self.bwd_profile()
+        rec = np.zeros((max(tl, 1), 16), np.int32)
+        val = np.zeros(max(tl, 1))
+        self.ctx._check(self.ctx.lib.catn_net_tape(self.h, rec.ctypes.data_as(_I32P), _dptr(val), tl))
+        return rec[:tl], val[:tl]
+
+    def site_adjoint(self, slot: int, size: int) -> np.ndarray:
+        """d lnZ / d (initial tensor of `slot`), flat column-major."""
+        out = np.zeros(size)
+        self.ctx._check(self.ctx.lib.catn_net_site_adjoint(self.h, int(slot), _dptr(out), size))
+        return out
+
     def work(self):
         """(algorithmic flops, algorithmic bytes, scratch high-water mark)."""
         f, b, h = C.c_double(), C.c_double(), C.c_int64()
@@ -356,6 +400,50 @@ def contraction_batch(tns: Sequence[TensorNetwork]):
     return lz, err, psi
 
 
+def gradient_batch(tns: Sequence[TensorNetwork], tape_cap: int = 0):
+    """contraction! of many networks with a tape, then the hand-written reverse
+    sweep of lnZ (one workgroup per network, one launch). Returns (lnZ, error,
+    psi) exactly as contraction_batch; the gradients stay with the networks."""
+    if not tns:
+        return np.zeros(0), np.zeros(0), np.zeros(0)
+    ctx = tns[0].ctx
+    arr = (_P * len(tns))(*[t.h for t in tns])
+    lz, err, psi = np.zeros(len(tns)), np.zeros(len(tns)), np.zeros(len(tns))
+    ctx._check(ctx.lib.catn_gradient_batch(ctx.h, len(tns), arr, int(tape_cap), _dptr(lz),
+                                           _dptr(err), _dptr(psi)))
+    for k, t in enumerate(tns):
+        t.lnZ, t.psi = float(lz[k]), float(psi[k])
+    return lz, err, psi
+
+
+def ising_gradient(tn: TensorNetwork, nedges: int, n: int):
+    """(d lnZ/d w, d lnZ/d h, d lnZ/d beta) of an ising_network after
+    gradient_batch. Exact contraction gives beta <s_i s_j> and beta <s_i>."""
+    dw, dh, db = np.zeros(nedges), np.zeros(n), C.c_double()
+    tn.ctx._check(tn.ctx.lib.catn_ising_gradient(tn.h, _dptr(dw), _dptr(dh), C.byref(db)))
+    return dw, dh, db.value
+
+
+def lnz_and_grad_batch(instances, *, ctx=None, tape_cap: int = 0, **kw):
+    """Many ising instances [(n, edges, w, h, beta), ...] (any sizes and betas)
+    in one forward + reverse sweep: [(lnZ, dlnZ/dw, dlnZ/dh), ...]."""
+    ctx = ctx or default_context()
+    nets = [ising_network(n, e, w, h, b, ctx=ctx, **kw) for (n, e, w, h, b) in instances]
+    lz, _, _ = gradient_batch(nets, tape_cap)
+    out = []
+    for t, (n, e, w, h, b), l in zip(nets, instances, lz):
+        dw, dh, _ = ising_gradient(t, len(e), n)
+        out.append((float(l), dw, dh))
+    return out
+
+
+def lnz_and_grad(n, edges, weights, fields, beta, *, ctx=None, **kw):
+    """lnZ and (d lnZ/d w, d lnZ/d h) of one ising_network by one forward
+    contraction and one reverse sweep."""
+    (l, dw, dh), = lnz_and_grad_batch([(n, edges, weights, fields, beta)], ctx=ctx, **kw)
+    return l, dw, dh
+
+
 def free_energy(tn: TensorNetwork):
     """free_energy (src/ising.jl:239-244)."""
     lnZ, _, _ = contraction(tn)
@@ -364,10 +452,16 @@ def free_energy(tn: TensorNetwork):
     return lz, -lz / beta
 
 
-def magnetization(n, edges, weights, fields, beta, *, ctx=None, **kw):
+def magnetization(n, edges, weights, fields, beta, *, ctx=None, method="pinning", **kw):
     """magnetization by pinning (src/ising.jl:263-290); the 1+2n networks are
-    contracted as one batch."""
+    contracted as one batch. method="adjoint": <s_i> = (d lnZ/d h_i) / beta
+    from one reverse sweep."""
     ctx = ctx or default_context()
+    if method == "adjoint":
+        _, _, dh = lnz_and_grad(n, edges, weights, fields, beta, ctx=ctx, **kw)
+        return dh / beta
+    if method != "pinning":
+        raise CatnError(f"unknown method {method!r}")
     nets = [ising_network(n, edges, weights, fields, beta, ctx=ctx, **kw)]
     for k in range(1, n + 1):
         for v in (1, 2):
@@ -381,9 +475,15 @@ def magnetization(n, edges, weights, fields, beta, *, ctx=None, **kw):
     return mag
 
 
-def correlation(n, edges, weights, fields, beta, *, ctx=None, **kw):
-    """correlation by pinning (src/ising.jl:309-339); 1+4|E| networks in one batch."""
+def correlation(n, edges, weights, fields, beta, *, ctx=None, method="pinning", **kw):
+    """correlation by pinning (src/ising.jl:309-339); 1+4|E| networks in one
+    batch. method="adjoint": <s_i s_j> = (d lnZ/d w_ij) / beta."""
     ctx = ctx or default_context()
+    if method == "adjoint":
+        _, dw, _ = lnz_and_grad(n, edges, weights, fields, beta, ctx=ctx, **kw)
+        return dw / beta
+    if method != "pinning":
+        raise CatnError(f"unknown method {method!r}")
     nets = [ising_network(n, edges, weights, fields, beta, ctx=ctx, **kw)]
     for (m, nn) in edges:
         for v1 in (1, 2):

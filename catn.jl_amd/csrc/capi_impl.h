@@ -6,7 +6,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <exception>
@@ -14,6 +16,7 @@
 #include <mutex>
 #include <thread>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -36,12 +39,18 @@ class Executor {
   // instance and writes the updated states back; returns after completion.
   virtual void run(const std::vector<InstState*>& states,
                    const std::vector<const std::vector<Op>*>& programs) = 0;
+  // Reverse sweep over the recorded tape of every instance (one workgroup
+  // each, one launch) with an LDS budget of lds_cap doubles.
+  virtual void run_backward(const std::vector<InstState*>& states, int lds_cap) = 0;
   // frequency of the device wall clock read by the op timers (Hz)
   virtual double tick_hz() const { return 1.0; }
   // launches of the interpreter kernel and their summed duration (seconds,
   // measured with events on the stream the kernel runs on)
   int64_t launches = 0;
   double kernel_seconds = 0.0;
+  // the same for the reverse-sweep kernel
+  int64_t bwd_launches = 0;
+  double bwd_seconds = 0.0;
   // host-side time: planning, host<->device copies + launch overhead
   double plan_seconds = 0.0, run_seconds = 0.0;
   // workgroup size of the interpreter (64, 128 or 256 lanes)
@@ -57,8 +66,13 @@ struct Error : std::runtime_error {
 
 }  // namespace catn
 
+struct catn_net;
+
 struct catn_ctx {
   std::unique_ptr<catn::Executor> ex;
+  // live networks of this context: destroying the context releases their
+  // device memory first (host languages may finalise the two in any order)
+  std::set<catn_net*> nets;
 };
 
 struct catn_net {
@@ -79,15 +93,52 @@ struct catn_net {
   int slot_cap = 0, res_cap = 0;
   int64_t arena_cap = 0, scratch_cap = 0;
   int64_t arena_hwm = 0;
+  // initial site tensors: arena offset of every slot uploaded with data
+  std::vector<int64_t> off0;
+  // ising_network provenance of the initial sites (chain rule of the adjoint)
+  struct IsingSite { int slot, edge, side, node, kind; };   // kind: 0 leaf, 1 first,
+                                                           // 2 last, 3 interior, 4 isolated
+  bool is_ising = false;
+  std::vector<IsingSite> isites;
+  std::vector<int32_t> iedges;
+  std::vector<double> iw, ih;
+  int ipos1 = 0, ival1 = 0, ipos2 = 0, ival2 = 0;
+  bool have_adjoint = false;
 
-  ~catn_net() { release(); }
+  ~catn_net() {
+    release();
+    if (ctx) ctx->nets.erase(this);
+  }
+
+  void attach(catn_ctx* c) {
+    ctx = c;
+    c->nets.insert(this);
+  }
 
   void release() {
     if (!ctx || !allocated) return;
     catn::Executor* ex = ctx->ex.get();
     ex->dfree(hs.arena); ex->dfree(hs.scratch); ex->dfree(hs.sites);
     ex->dfree(hs.results); ex->dfree(hs.dresults);
+    if (hs.tape) ex->dfree(hs.tape);
+    if (hs.adj) ex->dfree(hs.adj);
+    hs.tape = nullptr; hs.adj = nullptr;
     allocated = false;
+  }
+
+  // switch on the tape for the coming contraction (before any op has run)
+  void enable_record(int cap) {
+    ensure_device();
+    if (hs.tape_len != 0 || !pl.prog.empty())
+      throw catn::Error(-5, "gradient: the network has already executed device ops");
+    catn::Executor* ex = ctx->ex.get();
+    if (!hs.tape || hs.tape_cap < cap) {
+      if (hs.tape) ex->dfree(hs.tape);
+      hs.tape = (catn::TapeRec*)ex->dmalloc(sizeof(catn::TapeRec) * (size_t)cap);
+      hs.tape_cap = cap;
+    }
+    if (!hs.adj) hs.adj = (double*)ex->dmalloc(sizeof(double) * (size_t)arena_cap);
+    hs.record = 1;
   }
 
   int new_slot(int dl, int d, int dr, const double* data) {
@@ -102,6 +153,7 @@ struct catn_net {
   }
 
   void ensure_device() {
+    if (!ctx) throw catn::Error(-5, "the network's context has been destroyed");
     catn::Executor* ex = ctx->ex.get();
     if (!allocated) {
       int64_t staged_elems = 0;
@@ -140,6 +192,8 @@ struct catn_net {
         d.dl = s.dl; d.d = s.d; d.dr = s.dr; d.live = s.data.empty() ? 0 : 1;
         if (!s.data.empty()) {
           d.off = top;
+          if ((int)off0.size() <= s.slot) off0.resize(s.slot + 1, -1);
+          off0[s.slot] = top;
           buf.resize((size_t)(top - base), 0.0);
           buf.insert(buf.end(), s.data.begin(), s.data.end());
           top = (top + (int64_t)s.data.size() + 31) & ~int64_t(31);
@@ -166,10 +220,12 @@ struct catn_net {
   void check_status() {
     if (hs.status != catn::STATUS_OK) {
       static const char* names[] = {"ok", "arena full", "scratch full", "shape mismatch",
-                                    "SVD did not converge", "unsupported (randomized SVD branch)"};
+                                    "SVD did not converge", "unsupported (randomized SVD branch "
+                                    "or an op without a reverse step)", "tape full",
+                                    "reverse sweep could not replay a forward factorisation"};
       const int s = hs.status;
       std::string msg = "device op failed: ";
-      msg += (s >= 0 && s <= 5) ? names[s] : "unknown";
+      msg += (s >= 0 && s <= 7) ? names[s] : "unknown";
       msg += " at op " + std::to_string(hs.fail_op);
       throw catn::Error(-10 - s, msg);
     }
@@ -224,7 +280,12 @@ inline void run_nets(Executor* ex, const std::vector<catn_net*>& nets) {
   ex->run_seconds += wall() - t0;
 }
 
-inline void run_one(catn_net* n) { run_nets(n->ctx->ex.get(), {n}); }
+inline Executor* exec_of(const catn_net* n) {
+  if (!n->ctx) throw Error(-5, "the network's context has been destroyed");
+  return n->ctx->ex.get();
+}
+
+inline void run_one(catn_net* n) { run_nets(exec_of(n), {n}); }
 
 // f(i) for i in [0, n) on up to 16 host threads (planners of different
 // networks share no state); the first exception is rethrown.
@@ -287,27 +348,28 @@ inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, 
       const double hv = h[id - 1];
       const double f[2] = {std::exp(beta * hv * s[0]), std::exp(beta * hv * s[1])};
       double site[8];
-      int dl, d, dr;
+      int dl, d, dr, kind;
       if (degree[id] == 1) {            // leaf: column sums of field*Q
         site[0] = f[0] * mat[0][0] + f[1] * mat[1][0];
         site[1] = f[0] * mat[0][1] + f[1] * mat[1][1];
-        dl = 1; d = 2; dr = 1;
+        dl = 1; d = 2; dr = 1; kind = 0;
       } else if (nd.nb.empty()) {       // first: (field*Q)' reshaped (1,2,2)
         // site[1, a, b] = (fQ)[b, a]  -> linear a + 2b
         for (int a = 0; a < 2; ++a)
           for (int b = 0; b < 2; ++b) site[a + 2 * b] = f[b] * mat[b][a];
-        dl = 1; d = 2; dr = 2;
+        dl = 1; d = 2; dr = 2; kind = 1;
       } else if ((int)nd.nb.size() == degree[id] - 1) {   // last: Q reshaped (2,2,1)
         for (int a = 0; a < 2; ++a)
           for (int b = 0; b < 2; ++b) site[a + 2 * b] = mat[a][b];
-        dl = 2; d = 2; dr = 1;
+        dl = 2; d = 2; dr = 1; kind = 2;
       } else {                          // interior: t3[a, s, b] = delta_ab Q[a, s]
         for (int q = 0; q < 8; ++q) site[q] = 0.0;
         for (int a = 0; a < 2; ++a)
           for (int sv = 0; sv < 2; ++sv) site[a + 2 * sv + 4 * a] = mat[a][sv];
-        dl = 2; d = 2; dr = 2;
+        dl = 2; d = 2; dr = 2; kind = 3;
       }
       const int slot = net->new_slot(dl, d, dr, site);
+      net->isites.push_back({slot, e, side, id, kind});
       nd.slot.push_back(slot);
       nd.phys.push_back(d);
       nd.nb.push_back(other);
@@ -318,6 +380,7 @@ inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, 
       const double hv = h[k - 1];
       const double v = std::exp(beta * hv) + std::exp(-beta * hv);
       const int slot = net->new_slot(1, 1, 1, &v);
+      net->isites.push_back({slot, -1, 0, k, 4});
       net->pl.nodes[k].slot.push_back(slot);
       net->pl.nodes[k].phys.push_back(1);
     }
@@ -330,6 +393,90 @@ inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, 
   net->num_isolated = iso;
   net->n = n;
   net->beta = beta;
+  net->is_ising = true;
+  net->iedges.assign(edges, edges + 2 * nedges);
+  net->iw.assign(w, w + nedges);
+  net->ih.assign(h, h + n);
+  net->ipos1 = pos1; net->ival1 = val1; net->ipos2 = pos2; net->ival2 = val2;
+}
+
+// Chain rule of the reverse sweep through the ising_network site formulas
+// (src/ising.jl:20-227): site entries are products of B = exp(beta w s_i s_j')
+// (side i of an edge; side j carries the identity) and the field vector
+// f = exp(beta h s) (leaf and first sites only); adj holds d lnZ / d site.
+// dbeta = (sum_e w_e dw_e + sum_i h_i dh_i) / beta.
+inline void ising_chain_rule(const catn_net* net, const std::vector<double>& adj0, int64_t base,
+                             double* dw, double* dh, double* dbeta) {
+  const int n = net->n, ne = (int)net->iw.size();
+  const double beta = net->beta;
+  std::vector<double> gw(ne, 0.0), gh(n, 0.0);
+  auto spins = [&](int id, double s[2]) {
+    s[0] = 1.0; s[1] = -1.0;
+    if (id == net->ipos1) s[net->ival1 - 1] = 0.0;
+    else if (id == net->ipos2) s[net->ival2 - 1] = 0.0;
+  };
+  for (const auto& is : net->isites) {
+    const double* a = adj0.data() + (net->off0[is.slot] - base);
+    const int id = is.node;
+    const double hv = net->ih[id - 1];
+    if (is.kind == 4) {
+      gh[id - 1] += a[0] * beta * (std::exp(beta * hv) - std::exp(-beta * hv));
+      continue;
+    }
+    const int i = net->iedges[2 * is.edge], j = net->iedges[2 * is.edge + 1];
+    double si[2], sj[2];
+    spins(i, si);
+    spins(j, sj);
+    const double we = net->iw[is.edge];
+    double mat[2][2], dmat[2][2];   // mat and d mat / d w_e
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y) {
+        if (is.side == 0) {
+          mat[x][y] = std::exp(beta * we * (si[x] * sj[y]));
+          dmat[x][y] = beta * si[x] * sj[y] * mat[x][y];
+        } else {
+          mat[x][y] = x == y ? 1.0 : 0.0;
+          dmat[x][y] = 0.0;
+        }
+      }
+    double s[2];
+    spins(id, s);
+    const double f[2] = {std::exp(beta * hv * s[0]), std::exp(beta * hv * s[1])};
+    const double df[2] = {beta * s[0] * f[0], beta * s[1] * f[1]};
+    double ddw = 0.0, ddh = 0.0;
+    switch (is.kind) {
+      case 0:   // site[c] = sum_x f[x] mat[x][c]
+        for (int c = 0; c < 2; ++c)
+          for (int x = 0; x < 2; ++x) {
+            ddw += a[c] * f[x] * dmat[x][c];
+            ddh += a[c] * df[x] * mat[x][c];
+          }
+        break;
+      case 1:   // site[x + 2y] = f[y] mat[y][x]
+        for (int x = 0; x < 2; ++x)
+          for (int y = 0; y < 2; ++y) {
+            ddw += a[x + 2 * y] * f[y] * dmat[y][x];
+            ddh += a[x + 2 * y] * df[y] * mat[y][x];
+          }
+        break;
+      case 2:   // site[x + 2y] = mat[x][y]
+        for (int x = 0; x < 2; ++x)
+          for (int y = 0; y < 2; ++y) ddw += a[x + 2 * y] * dmat[x][y];
+        break;
+      default:  // site[x + 2 sv + 4 x] = mat[x][sv]
+        for (int x = 0; x < 2; ++x)
+          for (int sv = 0; sv < 2; ++sv) ddw += a[x + 2 * sv + 4 * x] * dmat[x][sv];
+        break;
+    }
+    gw[is.edge] += ddw;
+    gh[id - 1] += ddh;
+  }
+  double db = 0.0;
+  for (int e = 0; e < ne; ++e) db += net->iw[e] * gw[e];
+  for (int k = 0; k < n; ++k) db += net->ih[k] * gh[k];
+  if (dw) for (int e = 0; e < ne; ++e) dw[e] = gw[e];
+  if (dh) for (int k = 0; k < n; ++k) dh[k] = gh[k];
+  if (dbeta) *dbeta = beta != 0.0 ? db / beta : 0.0;
 }
 
 // raw2mps emitted as device split ops (src/mps_node.jl:31-50)
@@ -363,6 +510,7 @@ inline void add_tensor_node(catn_net* net, int id, int nd, const int64_t* dims, 
 inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
   for (catn_net* n : nets) {
     if (n->consumed) throw Error(-5, "contraction!: network already contracted");
+    if (exec_of(n) != ex) throw Error(-2, "contraction: networks of another context");
     n->ensure_device();
     n->reset_acc();
     n->pl.start();
@@ -388,6 +536,28 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
       }
     }
     active.swap(still);
+  }
+}
+
+// contraction! with the tape on, then one reverse-sweep launch for all
+inline void gradient_nets(Executor* ex, const std::vector<catn_net*>& nets, int tape_cap) {
+  for (catn_net* n : nets) {
+    if (n->consumed) throw Error(-5, "contraction!: network already contracted");
+    n->enable_record(tape_cap);
+  }
+  contract_nets(ex, nets);
+  int lds = 0;
+  std::vector<InstState*> st;
+  for (catn_net* n : nets) {
+    lds = std::max(lds, (int)n->hs.lds_max);
+    st.push_back(&n->hs);
+  }
+  const double t0 = wall();
+  ex->run_backward(st, lds);
+  ex->run_seconds += wall() - t0;
+  for (catn_net* n : nets) {
+    n->check_status();
+    n->have_adjoint = true;
   }
 }
 
@@ -423,6 +593,11 @@ int catn_create(catn_ctx** ctx, int device) {
 }
 
 int catn_destroy(catn_ctx* ctx) {
+  for (catn_net* n : ctx->nets) {
+    n->release();
+    n->ctx = nullptr;
+  }
+  ctx->nets.clear();
   delete ctx;
   return 0;
 }
@@ -448,7 +623,7 @@ int catn_ising_network(catn_ctx* ctx, int32_t n, int32_t nedges, const int32_t* 
                        catn_net** out) {
   CATN_TRY
   std::unique_ptr<catn_net> net(new catn_net);
-  net->ctx = ctx;
+  net->attach(ctx);
   apply_params(net.get(), p);
   catn::build_ising(net.get(), n, nedges, edges, w, h, beta, pos1, val1, pos2, val2);
   *out = net.release();
@@ -461,7 +636,7 @@ int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
                         const catn_params* p, catn_net** out) {
   CATN_TRY
   std::unique_ptr<catn_net> net(new catn_net);
-  net->ctx = ctx;
+  net->attach(ctx);
   apply_params(net.get(), p);
   // label -> occurrences (src/tensor_network.jl:59-102)
   std::map<int64_t, std::vector<std::pair<int, int>>> occ;
@@ -520,7 +695,7 @@ int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
 int catn_net_create(catn_ctx* ctx, const catn_params* p, catn_net** out) {
   CATN_TRY
   std::unique_ptr<catn_net> net(new catn_net);
-  net->ctx = ctx;
+  net->attach(ctx);
   apply_params(net.get(), p);
   *out = net.release();
   return 0;
@@ -569,7 +744,7 @@ int catn_net_free(catn_net* net) {
 
 int catn_contraction(catn_net* net, double* lnZ, double* error, double* psi) {
   CATN_TRY
-  catn::contract_nets(net->ctx->ex.get(), {net});
+  catn::contract_nets(catn::exec_of(net), {net});
   if (lnZ) *lnZ = net->hs.lnZ + net->hs.lntail;
   if (error) *error = net->hs.err;
   if (psi) *psi = net->hs.psi;
@@ -589,6 +764,76 @@ int catn_contraction_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, 
   }
   return 0;
   CATN_CATCH
+}
+
+int catn_gradient_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, int32_t tape_cap,
+                        double* lnZ, double* error, double* psi) {
+  CATN_TRY
+  std::vector<catn_net*> v(nets, nets + count);
+  catn::gradient_nets(ctx->ex.get(), v, tape_cap > 0 ? tape_cap : (1 << 16));
+  for (int k = 0; k < count; ++k) {
+    if (lnZ) lnZ[k] = v[k]->hs.lnZ + v[k]->hs.lntail;
+    if (error) error[k] = v[k]->hs.err;
+    if (psi) psi[k] = v[k]->hs.psi;
+  }
+  return 0;
+  CATN_CATCH
+}
+
+int catn_ising_gradient(catn_net* net, double* dw, double* dh, double* dbeta) {
+  CATN_TRY
+  if (!net->is_ising) throw catn::Error(-2, "ising gradient: network was not built by ising_network");
+  if (!net->have_adjoint) throw catn::Error(-5, "ising gradient: run catn_gradient_batch first");
+  int64_t lo = INT64_MAX, hi = 0;
+  for (const auto& is : net->isites) {
+    const int64_t o = net->off0[is.slot];
+    lo = std::min(lo, o);
+    hi = std::max(hi, o + 8);
+  }
+  hi = std::min<int64_t>(hi, net->arena_cap);
+  std::vector<double> adj0((size_t)(hi - lo), 0.0);
+  if (hi > lo) catn::exec_of(net)->d2h(adj0.data(), net->hs.adj + lo, sizeof(double) * (size_t)(hi - lo));
+  catn::ising_chain_rule(net, adj0, lo, dw, dh, dbeta);
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_site_adjoint(catn_net* net, int32_t slot, double* out, int64_t cap) {
+  CATN_TRY
+  if (!net->have_adjoint) throw catn::Error(-5, "site adjoint: run catn_gradient_batch first");
+  if (slot < 0 || slot >= (int)net->off0.size() || net->off0[slot] < 0)
+    throw catn::Error(-2, "site adjoint: slot was not an initial site");
+  if (cap > 0) catn::exec_of(net)->d2h(out, net->hs.adj + net->off0[slot], sizeof(double) * (size_t)cap);
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_tape(catn_net* net, int32_t* rec, double* val, int32_t cap) {
+  CATN_TRY
+  const int n = std::min<int>(cap, net->hs.tape_len);
+  if (n <= 0 || !net->hs.tape) return 0;
+  std::vector<catn::TapeRec> t((size_t)n);
+  catn::exec_of(net)->d2h(t.data(), net->hs.tape, sizeof(catn::TapeRec) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    int32_t* r = rec + 16 * i;
+    r[0] = t[i].code; r[1] = t[i].k; r[2] = t[i].path; r[3] = t[i].flag;
+    r[4] = t[i].aux0; r[5] = t[i].aux1; r[6] = t[i].nin; r[7] = t[i].nout;
+    for (int q = 0; q < 2; ++q) {
+      r[8 + 4 * q] = t[i].in[q].dl; r[9 + 4 * q] = t[i].in[q].d; r[10 + 4 * q] = t[i].in[q].dr;
+      r[11 + 4 * q] = (int32_t)t[i].in[q].off;
+    }
+    if (val) val[i] = t[i].val;
+  }
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_bwd_profile(catn_net* net, int64_t* ticks, double* replay_dev, int32_t* tape_len) {
+  for (int k = 0; k < 16; ++k)
+    if (ticks) ticks[k] = net->hs.bwd_ticks[k];
+  if (replay_dev) *replay_dev = net->hs.replay_dev;
+  if (tape_len) *tape_len = net->hs.tape_len;
+  return 0;
 }
 
 int catn_net_info(catn_net* net, double* sign, int32_t* maxdim, int32_t* num_isolated, int32_t* n,
@@ -622,7 +867,7 @@ int catn_net_profile(catn_net* net, int64_t* op_ticks, int32_t* op_count, int64_
     if (svd_ticks) svd_ticks[k] = net->hs.svd_ticks[k];
     if (svd_count) svd_count[k] = net->hs.svd_count[k];
   }
-  if (tick_hz) *tick_hz = net->ctx->ex->tick_hz();
+  if (tick_hz) *tick_hz = net->ctx ? net->ctx->ex->tick_hz() : 1.0;
   return 0;
 }
 
@@ -646,6 +891,16 @@ int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t
     ctx->ex->kernel_seconds = 0.0;
     ctx->ex->plan_seconds = 0.0;
     ctx->ex->run_seconds = 0.0;
+  }
+  return 0;
+}
+
+int catn_bwd_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t reset) {
+  if (launches) *launches = ctx->ex->bwd_launches;
+  if (seconds) *seconds = ctx->ex->bwd_seconds;
+  if (reset) {
+    ctx->ex->bwd_launches = 0;
+    ctx->ex->bwd_seconds = 0.0;
   }
   return 0;
 }
@@ -773,7 +1028,7 @@ int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t ca
   catn::SiteDesc d = net->read_desc(nd.slot[k - 1]);
   const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
   if (cnt > cap) throw catn::Error(-2, "output buffer too small");
-  net->ctx->ex->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
+  catn::exec_of(net)->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
   return 0;
   CATN_CATCH
 }
@@ -782,7 +1037,7 @@ int catn_tsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, double cutof
               int32_t* k, double* U, double* S, double* V, double* discarded) {
   CATN_TRY
   catn_net net;
-  net.ctx = ctx;
+  net.attach(ctx);
   catn_params p;
   catn_params_default(&p);
   apply_params(&net, &p);

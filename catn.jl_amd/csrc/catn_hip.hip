@@ -34,6 +34,21 @@ __global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, 
   run_program(g, st, ops, ranges[2 * b], ranges[2 * b + 1]);
 }
 
+// Reverse sweep: workgroup b walks the tape of instance b backwards.
+__global__ __launch_bounds__(kThreads, 4) void exec_backward(InstState* states, int lds_cap) {
+  __shared__ double red[16];
+  __shared__ int flag;
+  WG g;
+  g.tid = threadIdx.x;
+  g.nthr = blockDim.x;
+  g.red = red;
+  g.flag = &flag;
+  g.lds_cap = lds_cap;
+  InstState* st = states + blockIdx.x;
+  g.sub = st->sub_ticks;
+  run_backward(g, st);
+}
+
 static void hipcheck(hipError_t e, const char* what) {
   if (e != hipSuccess)
     throw Error(-100, std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
@@ -56,6 +71,10 @@ class HipExecutor : public Executor {
     hipcheck(hipEventCreate(&ev0_), "hipEventCreate");
     hipcheck(hipEventCreate(&ev1_), "hipEventCreate");
     hipcheck(hipFuncSetAttribute((const void*)exec_programs,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * kLdsMaxDoubles)),
+             "hipFuncSetAttribute");
+    hipcheck(hipFuncSetAttribute((const void*)exec_backward,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(sizeof(double) * kLdsMaxDoubles)),
              "hipFuncSetAttribute");
@@ -116,6 +135,28 @@ class HipExecutor : public Executor {
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
     launches += 1;
     kernel_seconds += 1e-3 * ms;
+    hipcheck(hipMemcpy(packed.data(), d_states_, sizeof(InstState) * nb, hipMemcpyDeviceToHost), "states back");
+    for (int b = 0; b < nb; ++b) *states[b] = packed[b];
+  }
+
+  void run_backward(const std::vector<InstState*>& states, int lds_cap) override {
+    const int nb = (int)states.size();
+    if (nb == 0) return;
+    if (lds_cap <= 0 || lds_cap > kLdsMaxDoubles) lds_cap = kLdsMaxDoubles;
+    reserve(0, nb);
+    std::vector<InstState> packed(nb);
+    for (int b = 0; b < nb; ++b) packed[b] = *states[b];
+    hipcheck(hipMemcpyAsync(d_states_, packed.data(), sizeof(InstState) * nb, hipMemcpyHostToDevice, stream_), "states");
+    hipcheck(hipEventRecord(ev0_, stream_), "event record");
+    hipLaunchKernelGGL(exec_backward, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
+                       d_states_, lds_cap);
+    hipcheck(hipGetLastError(), "launch exec_backward");
+    hipcheck(hipEventRecord(ev1_, stream_), "event record");
+    hipcheck(hipStreamSynchronize(stream_), "exec_backward");
+    float ms = 0.0f;
+    hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
+    bwd_launches += 1;
+    bwd_seconds += 1e-3 * ms;
     hipcheck(hipMemcpy(packed.data(), d_states_, sizeof(InstState) * nb, hipMemcpyDeviceToHost), "states back");
     for (int b = 0; b < nb; ++b) *states[b] = packed[b];
   }

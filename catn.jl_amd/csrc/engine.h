@@ -979,7 +979,7 @@ struct Exec {
     if (g.tid != 0) return;
     rp = st->tape + tape_len;
     rp->code = code; rp->nin = 0; rp->nout = 0; rp->k = 0; rp->path = 0; rp->flag = 0;
-    rp->aux0 = 0; rp->aux1 = 0; rp->val = 0.0; rp->val2 = 0.0;
+    rp->aux0 = 0; rp->aux1 = g.lds_cap; rp->val = 0.0; rp->val2 = 0.0;
   }
   CATN_D void rec_in(const SiteDesc& d) {
     if (!rp) return;
@@ -1690,6 +1690,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
   E.flops = 0.0;
   E.bytes = 0.0;
   E.tape_len = st->tape_len;
+  if (g.tid == 0 && g.lds_cap > st->lds_max) st->lds_max = g.lds_cap;
   int64_t shwm = 0;
   for (int i = begin; i < end && E.status == STATUS_OK; ++i) {
     const Op op = ops[i];
@@ -1745,3 +1746,5 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
 }
 
 }  // namespace catn
+
+#include "adjoint.h"

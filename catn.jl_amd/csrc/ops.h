@@ -56,8 +56,6 @@ enum OpCode : int32_t {
   // lognorm of a scalar node: z = sum(site a); lnZ += log|z|; sign *= sign(z)
   // (src/mps_node.jl:319-328, src/tensor_network.jl:489-501)
   OP_SCALAR_LOGNORM = 14,
-  // reverse-mode step of the tape record a (adjoint sweep)
-  OP_BACKWARD = 16,
   // standalone truncated SVD of the matrix held in slot a viewed as
   // (dl*d) x dr: U -> slot b (m x k), S*V' kept in slot c (k x n), S -> result
   // (src/linalg_utils.jl:15-28); d = maxdim, e = result index
@@ -90,7 +88,8 @@ struct TapeRec {
   int32_t code, nin, nout, k;
   int32_t path;    // split ops: 1 certified QR, 0 Jacobi SVD
   int32_t flag;    // op specific (direction, counted log term, ...)
-  int32_t aux0, aux1;
+  int32_t aux0;
+  int32_t aux1;    // LDS budget (doubles) of the launch that executed the op
   TapeSite in[3];
   TapeSite out[2];
   double val, val2;
@@ -131,6 +130,11 @@ struct InstState {
   TapeRec* tape;       // tape records (device), tape_cap entries
   double* adj;         // adjoint arena, same indexing as `arena`
   int32_t tape_len, tape_cap;
+  // reverse sweep: ticks per opcode of the replayed records, and the largest
+  // deviation of a replayed factorisation from the forward's stored output
+  int64_t bwd_ticks[16];
+  double replay_dev;
+  int32_t lds_max;     // largest LDS budget (doubles) of the launches that ran ops
 };
 
 enum Status : int32_t {
@@ -141,6 +145,7 @@ enum Status : int32_t {
   STATUS_NOCONV = 4,
   STATUS_UNSUPPORTED = 5,
   STATUS_TAPE_FULL = 6,
+  STATUS_REPLAY = 7,     // the reverse sweep could not replay a forward step
 };
 
 }  // namespace catn

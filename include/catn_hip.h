@@ -87,6 +87,35 @@ int catn_contraction(catn_net* net, double* lnZ, double* error, double* psi);
 int catn_contraction_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, double* lnZ,
                            double* error, double* psi);
 
+/* Chosen form, differentiation: contraction! of `count` networks exactly as
+ * catn_contraction_batch (same lnZ, error, psi, same schedule) with a tape of
+ * the executed device ops, then the hand-written reverse sweep of lnZ in one
+ * launch (one workgroup per network). Afterwards every network holds
+ * d lnZ / d (initial site tensors). Replaces the pinning re-contractions of
+ * magnetization / correlation (src/ising.jl:263-339) with one backward pass;
+ * the reference has no adjoint (no rrule is defined in src/). tape_cap:
+ * tape records per network (0: 65536). Networks built by catn_tensor_network
+ * (raw2mps at construction) and compress=true runs are rejected by the sweep. */
+int catn_gradient_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, int32_t tape_cap,
+                        double* lnZ, double* error, double* psi);
+/* d lnZ / d w (nedges), d lnZ / d h (n) and d lnZ / d beta of a network built
+ * by catn_ising_network, after catn_gradient_batch: the chain rule through the
+ * site formulas of ising_network (src/ising.jl:20-227). Exact contraction gives
+ * dw_ij = beta <s_i s_j>, dh_i = beta <s_i> (src/ising.jl:263-339). Any output
+ * pointer may be NULL. */
+int catn_ising_gradient(catn_net* net, double* dw, double* dh, double* dbeta);
+/* d lnZ / d (initial tensor of `slot`), `cap` doubles, column-major (dl,d,dr). */
+int catn_net_site_adjoint(catn_net* net, int32_t slot, double* out, int64_t cap);
+/* Reverse-sweep profile: wall-clock ticks per opcode (16 entries), the
+ * largest deviation of a replayed factorisation from the forward's stored
+ * output, and the tape length. */
+int catn_net_bwd_profile(catn_net* net, int64_t* ticks, double* replay_dev, int32_t* tape_len);
+/* The first `cap` tape records, 16 ints each: opcode, kept count, path
+ * (1 certified QR, 0 SVD), flag, aux0, LDS budget, #inputs, #outputs, then
+ * (dl, d, dr, arena offset) of inputs 1 and 2; `val` (may be NULL) gets each
+ * record's scalar (norm, cutoff, phase or final scalar). */
+int catn_net_tape(catn_net* net, int32_t* rec, double* val, int32_t cap);
+
 /* Per-network results after contraction. */
 int catn_net_info(catn_net* net, double* sign, int32_t* maxdim_intermediate,
                   int32_t* num_isolated, int32_t* n, double* beta);
@@ -110,6 +139,8 @@ int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks);
 /* Host time spent planning and in device round trips (launch, copies,
  * waits) since the last catn_kernel_stats reset, in seconds. */
 int catn_host_stats(catn_ctx* ctx, double* plan_seconds, double* run_seconds);
+/* Launches of the reverse-sweep kernel and their summed duration (HIP events). */
+int catn_bwd_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t reset);
 /* Workgroup size of the interpreter kernel: 64, 128 or 256 lanes (default 256). */
 int catn_set_threads(catn_ctx* ctx, int32_t threads);
 /* Device memory statistics: arena high-water mark and capacity (elements). */

@@ -38,6 +38,20 @@ class CpuExecutor : public Executor {
       run_program(g, states[b], p.data(), 0, (int)p.size());
     }
   }
+  void run_backward(const std::vector<InstState*>& states, int lds_cap) override {
+    double red[16];
+    int flag = 0;
+    WG g;
+    g.tid = 0;
+    g.nthr = 1;
+    g.red = red;
+    g.flag = &flag;
+    g.lds_cap = lds_cap > 0 ? lds_cap : kLdsMaxDoubles;
+    for (InstState* st : states) {
+      g.sub = st->sub_ticks;
+      catn::run_backward(g, st);
+    }
+  }
 };
 
 Executor* make_executor(int) { return new CpuExecutor(); }
