@@ -1,16 +1,22 @@
 """Benchmark: lnZ + grad lnZ solves/s on the n=100 random 3-regular spin glass
 (w = +-1, h ~ N(0, 0.1^2), beta = 1, Dmax = 32, chi = 256, FP64).
 
-One step = one solve = lnZ and the full gradient (d lnZ/d w_ij = beta <s_i s_j>
-for the 150 couplings, d lnZ/d h_i = beta <s_i> for the 100 fields) by the
-plain GPU form: the reference's pinning method (src/ising.jl:263-339), i.e.
-1 + 2n + 4|E| = 801 contraction! runs, executed as ONE batch (one workgroup per
-network per persistent launch). Networks are built and uploaded to HBM before
-the timed region of each step; the timed region is the batched contraction
-(host planner + device kernels) until the results are back on the host.
+One step = one pass of each form over one batch of synthetic instances:
 
-Prints one JSON line (see keys below). `--gpus N` with N > 1 exits: this build
-targets one GPU.
+  chosen form  lnZ and the full gradient (d lnZ/d w, 150 entries; d lnZ/d h,
+               100 entries) of B seeded instances by one recorded forward
+               contraction! and the hand-written reverse sweep, all B
+               instances in one batch (one workgroup per instance per launch).
+               value = B / step time.
+  plain form   the reference's own gradient route, pinning
+               (src/ising.jl:263-339): 1 + 2n + 4|E| = 801 contraction! runs
+               per solve, all 801 in one batch. Reported beside the chosen form.
+
+Networks are built and uploaded to HBM before the timed region; the timed
+region is the batched contraction (host planner + device kernels), the reverse
+sweep and the host chain rule to d lnZ/d(w, h), until results are on the host.
+
+Prints one JSON line. `--gpus N` with N > 1 exits: this build targets one GPU.
 """
 from __future__ import annotations
 
@@ -32,19 +38,23 @@ N_SPINS = 100
 DMAX, CHI, BETA = 32, 256, 1.0
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+NET_KW = dict(Dmax=DMAX, chi=CHI, arena_elems=3 << 20, scratch_elems=1 << 21)
 
 
-def build_batch(catn, ctx, n, edges, w, h):
-    kw = dict(Dmax=DMAX, chi=CHI, arena_elems=3 << 20, scratch_elems=1 << 20, ctx=ctx)
-    nets = [catn.ising_network(n, edges, w, h, BETA, **kw)]
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pinned_batch(catn, ctx, n, edges, w, h):
+    nets = [catn.ising_network(n, edges, w, h, BETA, ctx=ctx, **NET_KW)]
     for k in range(1, n + 1):
         for v in (1, 2):
-            nets.append(catn.ising_network(n, edges, w, h, BETA, pos1=k, val1=v, **kw))
+            nets.append(catn.ising_network(n, edges, w, h, BETA, pos1=k, val1=v, ctx=ctx, **NET_KW))
     for (a, b) in edges:
         for v1 in (1, 2):
             for v2 in (1, 2):
                 nets.append(catn.ising_network(n, edges, w, h, BETA, pos1=a, val1=v1, pos2=b,
-                                               val2=v2, **kw))
+                                               val2=v2, ctx=ctx, **NET_KW))
     return nets
 
 
@@ -61,8 +71,8 @@ def gradient_from_lnz(lz, n, edges):
 
 def cpu_baseline(n, edges, w, h, budget_s=15.0):
     """The FP64 oracle (NumPy restatement, 1 BLAS thread) on a bounded sample
-    of the same workload: as many of the 801 pinned contractions as fit in
-    `budget_s`, scaled to solves/s."""
+    of the plain form's work: as many of the 801 pinned contractions of one
+    solve as fit in `budget_s`, scaled to solves/s."""
     from oracle import catn_oracle as O
     jobs = [dict()]
     for k in range(1, n + 1):
@@ -84,7 +94,71 @@ def cpu_baseline(n, edges, w, h, budget_s=15.0):
     per = dt / done
     return {"value": 1.0 / (per * len(jobs)), "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": f"{done} of {len(jobs)} pinned contraction! runs of one solve "
-                      f"(FP64 NumPy oracle, OpenBLAS 1 thread), {dt:.1f} s, scaled to one solve"}
+                      f"(FP64 NumPy oracle, OpenBLAS 1 thread), {dt:.1f} s, scaled to one "
+                      f"solve (the reference's own gradient route)"}
+
+
+def run_chosen(catn, ctx, instances, steps, warmup):
+    """Forward with tape + reverse sweep of `instances` in one batch per step."""
+    times, res = [], None
+    fwd_flops = 0.0
+    fwd_bytes = 0.0
+    ctx.kernel_stats(reset=True)
+    ctx.bwd_kernel_stats(reset=True)
+    for step in range(warmup + steps):
+        nets = [catn.ising_network(n, e, w, h, BETA, ctx=ctx, **NET_KW) for (n, e, w, h) in instances]
+        _prefetch(nets)
+        timed = step >= warmup
+        if step == warmup:
+            ctx.kernel_stats(reset=True)
+            ctx.bwd_kernel_stats(reset=True)
+        t0 = time.perf_counter()
+        lz, _, _ = catn.gradient_batch(nets)
+        grads = [catn.ising_gradient(t, len(e), n)[:2] for t, (n, e, _, _) in zip(nets, instances)]
+        t1 = time.perf_counter()
+        if timed:
+            times.append(t1 - t0)
+            for t in nets:
+                f, b, _ = t.work()
+                fwd_flops += f
+                fwd_bytes += b
+            res = (lz.copy(), grads)
+        log(f"chosen step {step} {'timed' if timed else 'warmup'}: {t1 - t0:.3f} s "
+            f"({len(nets)} instances), lnZ/N[0]={lz[0] / instances[0][0]:.12f}")
+        del nets
+    fl, fs = ctx.kernel_stats()
+    bl, bs = ctx.bwd_kernel_stats()
+    return dict(times=times, lz=res[0], grads=res[1], fwd_launches=fl, fwd_seconds=fs,
+                bwd_launches=bl, bwd_seconds=bs, fwd_flops=fwd_flops, fwd_bytes=fwd_bytes)
+
+
+def run_plain(catn, ctx, inst, steps, warmup):
+    n, edges, w, h = inst
+    times, res = [], None
+    flops = bytes_ = 0.0
+    for step in range(warmup + steps):
+        nets = pinned_batch(catn, ctx, n, edges, w, h)
+        _prefetch(nets)
+        timed = step >= warmup
+        if step == warmup:
+            ctx.kernel_stats(reset=True)
+        t0 = time.perf_counter()
+        lz, _, _ = catn.contraction_batch(nets)
+        g = gradient_from_lnz(lz, n, edges)
+        t1 = time.perf_counter()
+        if timed:
+            times.append(t1 - t0)
+            for t in nets:
+                f, b, _ = t.work()
+                flops += f
+                bytes_ += b
+            res = (lz.copy(), g)
+        log(f"plain step {step} {'timed' if timed else 'warmup'}: {t1 - t0:.3f} s "
+            f"(801 pinned networks), lnZ/N={lz[0] / n:.12f}")
+        del nets
+    launches, ksec = ctx.kernel_stats()
+    return dict(times=times, lz=res[0], grad=res[1], launches=launches, seconds=ksec,
+                flops=flops, bytes=bytes_)
 
 
 def main():
@@ -93,6 +167,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=512, help="instances per chosen-form step")
+    ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both")
+    ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates the "
@@ -110,72 +187,94 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
-                                   "(a solve's 801 contractions shard trivially across GPUs, "
-                                   "see DESIGN.md)"}))
+                                   "(instances and pinned contractions are independent units "
+                                   "that would shard across GPUs with one final all-gather)"}))
         sys.exit(2)
 
     import __graft_entry__ as G
     from oracle import configs as Cf
+    from oracle import catn_oracle as O
     catn = G.load_package()
     ctx = catn.Context(0)
-    n, edges, w, h = Cf.spin_glass_3reg(N_SPINS, seed=args.seed)
+    instances = [Cf.spin_glass_3reg(N_SPINS, seed=args.seed + k) for k in range(args.batch)]
+    n, edges, w, h = instances[0]
 
-    # FP64 oracle value of the unpinned lnZ at the same (Dmax, chi)
-    from oracle import catn_oracle as O
-    ref_lnz = O.contraction(O.ising_network(n, edges, w, h, BETA, Dmax=DMAX, chi=CHI))[0]
+    out_cfg = {
+        "workload": f"spin glass n=100 random 3-regular (150 edges), w=+-1, h~N(0,0.1^2), "
+                    f"beta=1.0, Dmax=32, chi=256, FP64; one solve = lnZ + 250 gradient entries; "
+                    f"chosen form: {args.batch} seeded instances per batch, forward + reverse sweep",
+        "batch": args.batch,
+    }
+    chosen = plain = None
+    if args.form in ("both", "chosen"):
+        chosen = run_chosen(catn, ctx, instances, args.steps, args.warmup)
+    if args.form in ("both", "plain"):
+        plain = run_plain(catn, ctx, instances[0], args.plain_steps, min(args.warmup, 1))
 
-    times, flops, bytes_ = [], 0.0, 0.0
+    # FP64 oracle lnZ of a few instances at the same (Dmax, chi)
+    ncheck = 2
     dlnz = 0.0
-    grads = None
-    for step in range(args.warmup + args.steps):
-        nets = build_batch(catn, ctx, n, edges, w, h)
-        _prefetch(catn, nets)     # sites resident in HBM before the timed region
-        timed = step >= args.warmup
-        if timed and step == args.warmup:
-            ctx.kernel_stats(reset=True)
-        t0 = time.perf_counter()
-        lz, err, psi = catn.contraction_batch(nets)
-        t1 = time.perf_counter()
-        if timed:
-            times.append(t1 - t0)
-            for t in nets:
-                f, b, _ = t.work()
-                flops += f
-                bytes_ += b
-            dlnz = max(dlnz, abs(lz[0] - ref_lnz) / n)
-            grads = gradient_from_lnz(lz, n, edges)
-        print(f"step {step} {'timed' if timed else 'warmup'}: {t1 - t0:.3f} s, "
-              f"lnZ/N={lz[0] / n:.12f}", file=sys.stderr, flush=True)
-        del nets
-    launches, ksec = ctx.kernel_stats()
-    total = sum(times)
-    value = args.steps / total
-    achieved = flops / ksec / 1e12 if ksec > 0 else 0.0
+    ref = []
+    for k in range(ncheck):
+        nk, ek, wk, hk = instances[k]
+        ref.append(O.contraction(O.ising_network(nk, ek, wk, hk, BETA, Dmax=DMAX, chi=CHI))[0])
+    if chosen is not None:
+        dlnz = max(abs(chosen["lz"][k] - ref[k]) / N_SPINS for k in range(ncheck))
+    if plain is not None:
+        dlnz = max(dlnz, abs(plain["lz"][0] - ref[0]) / N_SPINS)
+
+    if chosen is not None:
+        total = sum(chosen["times"])
+        value = args.batch * len(chosen["times"]) / total
+        ms = 1e3 * total / len(chosen["times"])
+        form = "chosen"
+        # dominant kernel: the recorded forward interpreter
+        fl = max(chosen["fwd_launches"], 1)
+        avg = chosen["fwd_seconds"] / fl
+        flops_per_launch = chosen["fwd_flops"] / fl
+        bytes_per_launch = chosen["fwd_bytes"] / fl
+        out_cfg.update({
+            "chosen_solves_per_s": value,
+            "forward_kernel": {"launches": chosen["fwd_launches"], "seconds": chosen["fwd_seconds"]},
+            "reverse_kernel": {"launches": chosen["bwd_launches"], "seconds": chosen["bwd_seconds"]},
+        })
+        if plain is not None:
+            g_adj = chosen["grads"][0]
+            g_pin = plain["grad"]
+            out_cfg["adjoint_vs_pinning_max_abs_dw"] = float(np.max(np.abs(g_adj[0] - g_pin[0])))
+            out_cfg["adjoint_vs_pinning_max_abs_dh"] = float(np.max(np.abs(g_adj[1] - g_pin[1])))
+    else:
+        total = sum(plain["times"])
+        value = len(plain["times"]) / total
+        ms = 1e3 * total / len(plain["times"])
+        form = "plain"
+        fl = max(plain["launches"], 1)
+        avg = plain["seconds"] / fl
+        flops_per_launch = plain["flops"] / fl
+        bytes_per_launch = plain["bytes"] / fl
+    if plain is not None:
+        pv = len(plain["times"]) / sum(plain["times"])
+        out_cfg["plain_solves_per_s"] = pv
+        out_cfg["plain_kernel"] = {"launches": plain["launches"], "seconds": plain["seconds"]}
+        if chosen is not None:
+            out_cfg["chosen_over_plain"] = value / pv
+    out_cfg["max_abs_dlnZ_per_N_vs_fp64_oracle"] = dlnz
+    out_cfg["form"] = form
+    achieved = flops_per_launch / avg / 1e12 if avg > 0 else 0.0
     out = {
-        "metric": "lnZ+gradlnZ solves/s at |dlnZ/N|<=1e-6 (plain GPU form: pinning gradient)",
+        "metric": "lnZ+gradlnZ solves/s at |dlnZ/N|<=1e-6, chosen vs plain GPU form",
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random 3-regular spin glass)",
-        "config": {
-            "workload": "spin glass n=100 random 3-regular (150 edges), w=+-1, h~N(0,0.1^2), "
-                        "beta=1.0, Dmax=32, chi=256, FP64; one solve = lnZ + 250 gradient "
-                        "entries by 801 pinned contraction! runs in one batch",
-            "form": "plain",
-            "contractions_per_solve": 801,
-            "max_abs_dlnZ_per_N_vs_fp64_oracle": dlnz,
-            "grad_w_norm": float(np.linalg.norm(grads[0])) if grads is not None else None,
-            "kernel_launches": launches,
-            "kernel_seconds": ksec,
-            "swap_mfma_pct": 0.0,
-        },
+        "data": "synthetic (seeded random 3-regular spin glasses)",
+        "config": out_cfg,
         "roofline": {
             "bound": "mfma",
             "achieved": achieved,
@@ -183,9 +282,10 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "algorithmic_flops_per_launch": flops / max(launches, 1),
-            "algorithmic_bytes_per_launch": bytes_ / max(launches, 1),
-            "avg_launch_seconds": ksec / max(launches, 1),
+            "kernel": "exec_programs (forward interpreter)",
+            "algorithmic_flops_per_launch": flops_per_launch,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_seconds": avg,
         },
     }
     if not args.no_cpu:
@@ -193,7 +293,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def _prefetch(catn, nets):
+def _prefetch(nets):
     """Allocate and upload every network's device state (sites in HBM) before
     the timed region."""
     for t in nets:

@@ -29,6 +29,17 @@
 
 namespace catn {
 
+#if defined(CATN_EMU_MT)
+// Test-only multi-threaded CPU emulation (tests/emul/catn_emul_mt.cpp): every
+// lane of a workgroup is a host thread; these hooks stand in for the
+// workgroup barrier, the cross-lane reductions and the LDS base.
+void emu_sync(const char* file, int line);
+double emu_wg_sum(double v);
+double emu_wg_max(double v);
+double emu_grp_sum(double v, int L);
+double* emu_lds_base();
+#endif
+
 // ---------------------------------------------------------------------------
 // workgroup context
 // ---------------------------------------------------------------------------
@@ -38,11 +49,18 @@ struct WG {
   int* flag;       // LDS int
   int lds_cap;     // doubles available in the dynamic LDS region
   int64_t* sub;    // sub-step tick accumulators (instance state), or null
+#if defined(CATN_EMU_MT)
+  // the emulator checks that every lane arrives from the same call site
+  void sync(const char* file = __builtin_FILE(), int line = __builtin_LINE()) const {
+    emu_sync(file, line);
+  }
+#else
   CATN_D void sync() const {
 #if defined(__HIP_DEVICE_COMPILE__)
     __syncthreads();
 #endif
   }
+#endif
 };
 
 CATN_D double wg_sum(const WG& g, double v) {
@@ -56,6 +74,9 @@ CATN_D double wg_sum(const WG& g, double v) {
   const int nw = (g.nthr + 63) >> 6;
   for (int i = 0; i < nw; ++i) s += g.red[i];
   return s;
+#elif defined(CATN_EMU_MT)
+  (void)g;
+  return emu_wg_sum(v);
 #else
   (void)g;
   return v;
@@ -73,6 +94,9 @@ CATN_D double wg_max(const WG& g, double v) {
   const int nw = (g.nthr + 63) >> 6;
   for (int i = 1; i < nw; ++i) s = fmax(s, g.red[i]);
   return s;
+#elif defined(CATN_EMU_MT)
+  (void)g;
+  return emu_wg_max(v);
 #else
   (void)g;
   return v;
@@ -85,6 +109,8 @@ CATN_D double wg_max(const WG& g, double v) {
 CATN_D double grp_sum(double v, int L) {
 #if defined(__HIP_DEVICE_COMPILE__)
   for (int o = L >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#elif defined(CATN_EMU_MT)
+  v = emu_grp_sum(v, L);
 #else
   (void)L;
 #endif
@@ -172,8 +198,11 @@ CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double*
   // Three barriers per column: the reflector v = [1; scal * x(1:)] is applied
   // with the scaling folded into the update, and the stored sub-diagonal of
   // column j is scaled during the norm pass of column j+1 (disjoint memory).
+  // The diagonal entry beta of column j is stored in the same deferred pass:
+  // every lane reads alpha = A(j, j) after the norm barrier of column j, and
+  // an earlier store could overtake a lagging wavefront's read.
   int pend_col = -1;
-  double pend_scal = 1.0;
+  double pend_scal = 1.0, pend_beta = 0.0;
   for (int j = 0; j < n; ++j) {
     double* x = A + j + (int64_t)j * lda;
     double part = 0.0;
@@ -181,6 +210,7 @@ CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double*
     if (pend_col >= 0) {
       double* xp = A + pend_col + (int64_t)pend_col * lda;
       for (int i = 1 + g.tid; i < m - pend_col; i += g.nthr) xp[i] *= pend_scal;
+      if (g.tid == 0) xp[0] = pend_beta;
     }
     const double s = wg_sum(g, part);   // barriers: pending scaling now visible
     const double alpha = x[0];
@@ -197,27 +227,27 @@ CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double*
         const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
         for (int c = grp; c < ncol; c += ng) {
           double* y = A + j + (int64_t)(j + 1 + c) * lda;
+          const double y0 = y[0];   // read by the whole group before lane 0 updates it
           double w = 0.0;
           for (int i = 1 + lane; i < m - j; i += L) w += x[i] * y[i];
           w = grp_sum(w, L);
-          w = t * (y[0] + scal * w);
-          if (lane == 0) y[0] -= w;
+          w = t * (y0 + scal * w);
+          if (lane == 0) y[0] = y0 - w;
           const double ws = w * scal;
           for (int i = 1 + lane; i < m - j; i += L) y[i] -= ws * x[i];
         }
       }
     }
-    if (g.tid == 0) {
-      tau[j] = t;
-      x[0] = beta;
-    }
+    if (g.tid == 0) tau[j] = t;
     g.sync();
     pend_col = s > 0.0 ? j : -1;
     pend_scal = scal;
+    pend_beta = beta;
   }
   if (pend_col >= 0) {
     double* xp = A + pend_col + (int64_t)pend_col * lda;
     for (int i = 1 + g.tid; i < m - pend_col; i += g.nthr) xp[i] *= pend_scal;
+    if (g.tid == 0) xp[0] = pend_beta;
   }
   g.sync();
 }
@@ -270,6 +300,8 @@ CATN_D double* lds_base() {
 }
 #elif defined(__HIPCC__)
 CATN_D double* lds_base() { return nullptr; }   // host pass of the HIP build: unused
+#elif defined(CATN_EMU_MT)
+inline double* lds_base() { return emu_lds_base(); }
 #else
 inline double* lds_base() {
   static thread_local double buf[kLdsMaxDoubles];
@@ -559,8 +591,6 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
   }
   const double thr = thr_in >= 0.0 ? thr_in : 1e-36 * wg_sum(g, fpart);
   for (int sweep = 1; sweep <= maxsweeps; ++sweep) {
-    if (g.tid == 0) *g.flag = 0;
-    g.sync();
     int rotated = 0;
     for (int r = 0; r < N - 1; ++r) {
       for (int k = grp; k < npairs; k += ng) {
@@ -598,11 +628,7 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
       }
       g.sync();
     }
-    if (rotated) *g.flag = 1;
-    g.sync();
-    const int any = *g.flag;
-    g.sync();
-    if (!any) return sweep;
+    if (!(wg_max(g, rotated ? 1.0 : 0.0) > 0.0)) return sweep;
   }
   return -1;
 }
@@ -999,10 +1025,32 @@ struct Exec {
     if (st->record && status == STATUS_OK) ++tape_len;
   }
 
-  CATN_D SiteDesc site(int s) const {
+  // Operand descriptors of the current op, read by every lane before any
+  // lane can rewrite them: lane 0 publishes the op's output descriptors at
+  // the end of the op, and an op without an internal barrier would otherwise
+  // let a lagging wavefront read a descriptor its first wavefront already
+  // replaced.
+  int op_slot[3];
+  SiteDesc op_desc[3];
+  CATN_D SiteDesc load_site(int s) const {
     volatile SiteDesc* p = st->sites + s;
     SiteDesc d;
     d.off = p->off; d.dl = p->dl; d.d = p->d; d.dr = p->dr; d.live = p->live;
+    return d;
+  }
+  CATN_D void snapshot(const Op& op) {
+    const int sl[3] = {op.a, op.b, op.c};
+    for (int q = 0; q < 3; ++q) {
+      op_slot[q] = (sl[q] >= 0 && sl[q] < st->nsites) ? sl[q] : -1;
+      if (op_slot[q] >= 0) op_desc[q] = load_site(op_slot[q]);
+    }
+    g.sync();
+  }
+  CATN_D SiteDesc site(int s) const {
+    for (int q = 0; q < 3; ++q)
+      if (op_slot[q] == s) return op_desc[q];
+    SiteDesc d;
+    d.off = 0; d.dl = d.d = d.dr = 0; d.live = 0;
     return d;
   }
   CATN_D double* ptr(const SiteDesc& d) const { return st->arena + d.off; }
@@ -1700,6 +1748,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
     E.sc.cap = st->scratch_cap;
     E.sc.top = 0;
     E.sc.fail = false;
+    E.snapshot(op);
     switch (op.code) {
       case OP_CANO_R: op_cano_r(E, op); break;
       case OP_CANO_L: op_cano_l(E, op); break;

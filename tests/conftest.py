@@ -7,7 +7,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-EMUL_LIB = os.path.join(ROOT, "tests", "emul", "libcatn_emul.so")
+# CATN_EMU_LIB selects another emulator build for the "emu" cases, e.g. the
+# multi-threaded lane emulator (tests/emul/catn_emul_mt.cpp) under a sanitizer.
+EMUL_LIB = os.environ.get("CATN_EMU_LIB") or os.path.join(ROOT, "tests", "emul", "libcatn_emul.so")
 
 
 def pytest_configure(config):

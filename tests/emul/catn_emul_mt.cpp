@@ -1,0 +1,313 @@
+// Multi-threaded CPU emulator of libcatn_hip — TEST BUILD ONLY.
+//
+// Runs every workgroup of a launch with one host thread per lane (256 by
+// default, the product's workgroup size), so that the lane-parallel code
+// paths of engine.h / adjoint.h (lane groups, strided loops, cross-lane
+// reductions, LDS budgets) execute on the CPU the way they do on the device.
+// Built with AddressSanitizer it turns an out-of-bounds access of any lane
+// into a report with a stack trace instead of a GPU memory fault:
+//   * device buffers are heap blocks of exactly the requested size;
+//   * the dynamic LDS region is a heap block of exactly the launch's budget;
+//   * every barrier checks that all lanes arrive from the same call site
+//     (a divergent barrier is reported with both sites), and a barrier or a
+//     lane-group reduction that some lanes never reach is reported after a
+//     time limit instead of hanging.
+// Cross-lane reductions give every lane the same value in a fixed order, as
+// the device's butterfly reductions do.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#define CATN_EMU_MT 1
+#include "../../catn.jl_amd/csrc/engine.h"
+#include "../../catn.jl_amd/csrc/capi_impl.h"
+
+namespace catn {
+
+static double emu_timeout_s() {
+  const char* e = getenv("CATN_EMU_TIMEOUT");
+  return e ? atof(e) : 120.0;
+}
+
+struct EmuShared {
+  int nthr = 256;
+  int lds_cap = 0;
+  double* lds = nullptr;
+  std::vector<double> xb;          // per-lane exchange slots of the reductions
+  // checked barrier
+  std::mutex m;
+  std::condition_variable cv;
+  int count = 0;
+  long gen = 0;
+  std::vector<const void*> sites;
+  long divergent = 0;
+  // lane-group reductions: one slot per possible group leader
+  struct alignas(64) GSlot {
+    std::atomic<int> cnt{0};
+    std::atomic<long> gen{0};
+  };
+  std::vector<GSlot> gs;
+  std::vector<double> gx;
+  explicit EmuShared(int n, int lds_doubles)
+      : nthr(n), lds_cap(lds_doubles), xb(n), sites(n), gs(n), gx(n) {
+    lds = new double[lds_doubles > 0 ? lds_doubles : 1];
+    for (int i = 0; i < lds_doubles; ++i) lds[i] = 0.0;
+  }
+  ~EmuShared() { delete[] lds; }
+};
+
+struct EmuLane {
+  EmuShared* sh = nullptr;
+  int tid = 0;
+  const void* hist[32] = {};   // the lane's last barrier sites (ring)
+  long nb = 0;
+};
+static thread_local EmuLane t_lane;
+
+// A barrier site is the (file, line) of the sync() call, interned to a
+// stable pointer so that sites compare by address.
+struct Site {
+  const char* file;
+  int line;
+};
+static std::mutex g_site_m;
+static const Site* intern_site(const char* file, int line) {
+  static std::vector<Site*> all;
+  std::lock_guard<std::mutex> lk(g_site_m);
+  for (Site* s : all)
+    if (s->line == line && strcmp(s->file, file) == 0) return s;
+  Site* s = new Site{file, line};
+  all.push_back(s);
+  return s;
+}
+static const char* site_str(const void* p, char* buf, size_t n) {
+  const Site* s = (const Site*)p;
+  if (!s) return "(none)";
+  const char* f = strrchr(s->file, '/');
+  snprintf(buf, n, "%s:%d", f ? f + 1 : s->file, s->line);
+  return buf;
+}
+
+static void emu_die(const char* what) {
+  fprintf(stderr, "[catn-emu-mt] %s (lane %d)\n", what, t_lane.tid);
+  fflush(stderr);
+  abort();
+}
+
+static std::vector<EmuLane*>* g_lanes = nullptr;   // lanes of the running launch
+extern std::atomic<long> g_divergent;
+static void note_divergent() { g_divergent.fetch_add(1); }
+
+static void dump_history(int lane) {
+  if (!g_lanes || lane >= (int)g_lanes->size()) return;
+  EmuLane* L = (*g_lanes)[lane];
+  fprintf(stderr, "[catn-emu-mt] lane %d: last barrier sites (oldest first):", lane);
+  char b[256];
+  for (long i = L->nb > 32 ? L->nb - 32 : 0; i < L->nb; ++i)
+    fprintf(stderr, " %s", site_str(L->hist[i & 31], b, sizeof b));
+  fprintf(stderr, "\n");
+}
+
+static void barrier_at(const void* site) {
+  EmuShared* sh = t_lane.sh;
+  t_lane.hist[t_lane.nb++ & 31] = site;
+  std::unique_lock<std::mutex> lk(sh->m);
+  sh->sites[t_lane.tid] = site;
+  const long g = sh->gen;
+  if (++sh->count == sh->nthr) {
+    for (int i = 1; i < sh->nthr; ++i) {
+      if (sh->sites[i] != sh->sites[0]) {
+        note_divergent();
+        if (sh->divergent++ < 8)
+        {
+          char b0[256], b1[256];
+          fprintf(stderr, "[catn-emu-mt] divergent barrier: lane 0 at %s, lane %d at %s\n",
+                  site_str(sh->sites[0], b0, sizeof b0), i, site_str(sh->sites[i], b1, sizeof b1));
+        }
+        if (getenv("CATN_EMU_STRICT")) {
+          dump_history(0);
+          dump_history(i);
+          emu_die("divergent barrier");
+        }
+        break;
+      }
+    }
+    sh->count = 0;
+    sh->gen++;
+    sh->cv.notify_all();
+    return;
+  }
+  // plain wait (intercepted by ThreadSanitizer); the launch's watchdog
+  // reports a barrier that some lanes never reach
+  sh->cv.wait(lk, [&] { return sh->gen != g; });
+}
+
+// Watchdog of one launch: a barrier generation that does not advance within
+// the time limit while lanes wait in it is reported with the missing lanes'
+// last barrier sites.
+static void watchdog(EmuShared* sh, std::atomic<bool>* done) {
+  long last = -1;
+  auto since = std::chrono::steady_clock::now();
+  while (!done->load()) {
+    std::this_thread::sleep_for(std::chrono::milliseconds(200));
+    long g;
+    int cnt;
+    {
+      std::lock_guard<std::mutex> lk(sh->m);
+      g = sh->gen;
+      cnt = sh->count;
+    }
+    if (g != last || cnt == 0) {
+      last = g;
+      since = std::chrono::steady_clock::now();
+      continue;
+    }
+    if (std::chrono::duration<double>(std::chrono::steady_clock::now() - since).count() >
+        emu_timeout_s()) {
+      fprintf(stderr, "[catn-emu-mt] barrier not reached by all lanes: %d of %d arrived\n", cnt,
+              sh->nthr);
+      dump_history(0);
+      for (int i = 1; i < sh->nthr; ++i)
+        if (g_lanes && (*g_lanes)[i] && (*g_lanes)[0] &&
+            (*g_lanes)[i]->nb != (*g_lanes)[0]->nb) {
+          dump_history(i);
+          break;
+        }
+      fflush(stderr);
+      abort();
+    }
+  }
+}
+
+void emu_sync(const char* file, int line) { barrier_at(intern_site(file, line)); }
+
+static double wg_reduce(double v, bool is_max) {
+  EmuShared* sh = t_lane.sh;
+  sh->xb[t_lane.tid] = v;
+  static const Site* s1 = intern_site("emu_wg_reduce", 1);
+  static const Site* s2 = intern_site("emu_wg_reduce", 2);
+  barrier_at(s1);
+  double s = sh->xb[0];
+  for (int i = 1; i < sh->nthr; ++i) s = is_max ? fmax(s, sh->xb[i]) : s + sh->xb[i];
+  barrier_at(s2);
+  return s;
+}
+
+double emu_wg_sum(double v) { return wg_reduce(v, false); }
+double emu_wg_max(double v) { return wg_reduce(v, true); }
+
+static void group_wait(EmuShared::GSlot& s, int L) {
+  const long g0 = s.gen.load(std::memory_order_acquire);
+  if (s.cnt.fetch_add(1, std::memory_order_acq_rel) == L - 1) {
+    s.cnt.store(0, std::memory_order_relaxed);
+    s.gen.fetch_add(1, std::memory_order_acq_rel);
+    return;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  int spins = 0;
+  while (s.gen.load(std::memory_order_acquire) == g0) {
+    if (++spins > 64) {
+      std::this_thread::yield();
+      if ((spins & 4095) == 0 &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
+              emu_timeout_s()) {
+        fprintf(stderr, "[catn-emu-mt] lane group of %d not complete\n", L);
+        emu_die("group reduction hang");
+      }
+    }
+  }
+}
+
+double emu_grp_sum(double v, int L) {
+  if (L <= 1) return v;
+  EmuShared* sh = t_lane.sh;
+  const int tid = t_lane.tid;
+  if (L > 64 || (L & (L - 1))) emu_die("group size is not a power of two <= 64");
+  const int base = tid & ~(L - 1);
+  if (base + L > sh->nthr) emu_die("lane group extends past the workgroup");
+  sh->gx[tid] = v;
+  group_wait(sh->gs[base], L);
+  double s = 0.0;
+  for (int i = 0; i < L; ++i) s += sh->gx[base + i];
+  group_wait(sh->gs[base], L);
+  return s;
+}
+
+double* emu_lds_base() { return t_lane.sh->lds; }
+
+class MtExecutor : public Executor {
+ public:
+  MtExecutor() { threads = 256; }
+  const char* name() const override { return "cpu-emulator-mt"; }
+  void* dmalloc(size_t bytes) override { return calloc(1, bytes < 8 ? 8 : bytes); }
+  void dfree(void* p) override { free(p); }
+  void h2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
+  void d2h(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
+
+  template <class F>
+  void launch(int lds_cap, InstState* st, F body) {
+    EmuShared sh(threads, lds_cap);
+    double red[16];
+    int flag = 0;
+    std::vector<std::thread> th;
+    std::vector<EmuLane*> lanes(threads, nullptr);
+    g_lanes = &lanes;
+    std::atomic<int> ready{0};
+    th.reserve(threads);
+    for (int t = 0; t < threads; ++t) {
+      th.emplace_back([&, t] {
+        t_lane = EmuLane();
+        t_lane.sh = &sh;
+        t_lane.tid = t;
+        lanes[t] = &t_lane;
+        ready.fetch_add(1);
+        while (ready.load() < threads) std::this_thread::yield();
+        WG g;
+        g.tid = t;
+        g.nthr = threads;
+        g.red = red;
+        g.flag = &flag;
+        g.lds_cap = lds_cap;
+        g.sub = st->sub_ticks;
+        body(g);
+      });
+    }
+    std::atomic<bool> done{false};
+    std::thread wd(watchdog, &sh, &done);
+    for (auto& x : th) x.join();
+    done.store(true);
+    wd.join();
+    g_lanes = nullptr;
+  }
+
+  void run(const std::vector<InstState*>& states,
+           const std::vector<const std::vector<Op>*>& programs) override {
+    const int lds = lds_doubles_for_batch((int)states.size());
+    for (size_t b = 0; b < states.size(); ++b) {
+      const std::vector<Op>& p = *programs[b];
+      InstState* st = states[b];
+      launch(lds, st, [&](const WG& g) { run_program(g, st, p.data(), 0, (int)p.size()); });
+    }
+  }
+  void run_backward(const std::vector<InstState*>& states, int lds_cap) override {
+    if (lds_cap <= 0 || lds_cap > kLdsMaxDoubles) lds_cap = kLdsMaxDoubles;
+    for (InstState* st : states)
+      launch(lds_cap, st, [&](const WG& g) { catn::run_backward(g, st); });
+  }
+};
+
+Executor* make_executor(int) { return new MtExecutor(); }
+
+std::atomic<long> g_divergent{0};
+
+}  // namespace catn
+
+// barriers reached from different call sites since the library was loaded
+extern "C" long catn_emu_divergent_barriers(void) { return catn::g_divergent.load(); }

@@ -1,0 +1,70 @@
+"""Lane-parallel execution of the engine on the CPU (test-only emulator
+tests/emul/catn_emul_mt.cpp: 256 host threads per workgroup, one per lane,
+checked barriers, cross-lane reductions in a fixed order).
+
+The device code must give the same results whatever the lanes' relative
+progress: every barrier reached from one call site by all lanes (a divergent
+barrier desynchronises the workgroup on the GPU), no lane reading data that
+another lane rewrites without a barrier in between (lagging wavefronts). The
+forward lnZ and the reverse-sweep gradient of truncated networks therefore
+match the one-thread build to rounding, and no divergent barrier is seen.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oracle import configs as Cf
+from oracle import exact as X
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MT_LIB = os.path.join(ROOT, "tests", "emul", "libcatn_emul_mt.so")
+ST_LIB = os.path.join(ROOT, "tests", "emul", "libcatn_emul.so")
+
+
+@pytest.fixture(scope="module")
+def ctxs(catn):
+    if not (os.path.exists(MT_LIB) and os.path.exists(ST_LIB)):
+        pytest.skip("emulators not built (run __graft_entry__.build())")
+    return catn.Context(lib=MT_LIB), catn.Context(lib=ST_LIB)
+
+
+def _divergent():
+    lib = ctypes.CDLL(MT_LIB)
+    lib.catn_emu_divergent_barriers.restype = ctypes.c_long
+    return lib.catn_emu_divergent_barriers()
+
+
+def _run(catn, ctx, n, edges, w, h, beta, **kw):
+    tn = catn.ising_network(n, edges, w, h, beta, ctx=ctx, **kw)
+    lz, err, _ = catn.gradient_batch([tn])
+    dw, dh, _ = catn.ising_gradient(tn, len(edges), n)
+    return lz[0], err[0], dw, dh, tn.edge_log()
+
+
+@pytest.mark.parametrize("case", ["sg16_D6_chi12", "lattice4_Dmax3", "sg12_select2_norm2"])
+def test_lane_parallel_matches_one_thread(catn, ctxs, case):
+    mt, st = ctxs
+    if case == "sg16_D6_chi12":
+        n, edges, w, h = Cf.spin_glass_3reg(16, seed=4)
+        beta, kw = 1.0, dict(Dmax=6, chi=12)
+    elif case == "lattice4_Dmax3":
+        edges = X.square_edges(4)
+        n = 16
+        r = np.random.default_rng(3)
+        w, h = r.uniform(-1, 1, len(edges)), r.normal(0, 0.2, n)
+        beta, kw = 0.9, dict(Dmax=3, chi=32)
+    else:
+        n, edges, w, h = Cf.spin_glass_3reg(12, seed=2)
+        beta, kw = 1.0, dict(Dmax=4, chi=8, select=2, norm_method=2)
+    before = _divergent()
+    a = _run(catn, mt, n, edges, w, h, beta, **kw)
+    b = _run(catn, st, n, edges, w, h, beta, **kw)
+    assert _divergent() == before
+    assert a[4] == b[4]                              # same edge sequence
+    assert abs(a[0] - b[0]) / n < 1e-13
+    assert abs(a[1] - b[1]) <= 1e-9 * max(1.0, abs(b[1]))
+    scale = max(1.0, np.abs(b[2]).max(), np.abs(b[3]).max())
+    assert np.abs(a[2] - b[2]).max() < 1e-9 * scale
+    assert np.abs(a[3] - b[3]).max() < 1e-9 * scale
