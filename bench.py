@@ -206,9 +206,21 @@ def main():
         "batch": args.batch,
     }
     chosen = plain = None
+    refused = None
     if args.form in ("both", "chosen"):
-        chosen = run_chosen(catn, ctx, instances, args.steps, args.warmup)
-    if args.form in ("both", "plain"):
+        try:
+            chosen = run_chosen(catn, ctx, instances, args.steps, args.warmup)
+        except catn.CatnError as e:
+            # the device refuses gradient runs across a Dmax cut (before any launch)
+            if "Dmax cut" not in str(e):
+                raise
+            refused = str(e)
+            log(f"chosen form refused by the device: {e}")
+            if args.form == "chosen":
+                args.form = "plain"
+        if refused and plain is None:
+            plain = run_plain(catn, ctx, instances[0], args.steps, min(args.warmup, 1))
+    if args.form in ("both", "plain") and plain is None:
         plain = run_plain(catn, ctx, instances[0], args.plain_steps, min(args.warmup, 1))
 
     # FP64 oracle lnZ of a few instances at the same (Dmax, chi)
@@ -260,6 +272,8 @@ def main():
             out_cfg["chosen_over_plain"] = value / pv
     out_cfg["max_abs_dlnZ_per_N_vs_fp64_oracle"] = dlnz
     out_cfg["form"] = form
+    if refused:
+        out_cfg["chosen_form"] = "refused by the device: " + refused
     achieved = flops_per_launch / avg / 1e12 if avg > 0 else 0.0
     out = {
         "metric": "lnZ+gradlnZ solves/s at |dlnZ/N|<=1e-6, chosen vs plain GPU form",

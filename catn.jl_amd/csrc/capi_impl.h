@@ -44,6 +44,11 @@ class Executor {
   virtual void run_backward(const std::vector<InstState*>& states, int lds_cap) = 0;
   // frequency of the device wall clock read by the op timers (Hz)
   virtual double tick_hz() const { return 1.0; }
+  // Whether the reverse sweep may run over a Dmax cut (cut_bondim_opt!) on
+  // this executor. The gfx950 executor refuses it: gradient runs of
+  // networks with a Dmax cut faulted the GPU and the cause is not yet known,
+  // so such a run is rejected on the host before any of its launches.
+  virtual bool cut_reverse_enabled() const { return true; }
   // launches of the interpreter kernel and their summed duration (seconds,
   // measured with events on the stream the kernel runs on)
   int64_t launches = 0;
@@ -525,6 +530,12 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
       cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
     });
     ex->plan_seconds += wall() - t0;
+    if (!ex->cut_reverse_enabled())
+      for (size_t k = 0; k < active.size(); ++k)
+        if (cut[k] && active[k]->hs.record)
+          throw Error(-15, "gradient across a Dmax cut (cut_bondim_opt!) is disabled on this "
+                           "device; use the pinning route (magnetization / correlation) or a "
+                           "Dmax that the network never exceeds");
     run_nets(ex, active);
     for (size_t k = 0; k < active.size(); ++k) {
       catn_net* n = active[k];

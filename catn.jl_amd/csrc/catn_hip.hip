@@ -67,6 +67,7 @@ class HipExecutor : public Executor {
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
       tick_hz_ = 1000.0 * khz;
+    (void)hipGetLastError();   // an unsupported attribute must not surface at the first launch
     hipcheck(hipStreamCreate(&stream_), "hipStreamCreate");
     hipcheck(hipEventCreate(&ev0_), "hipEventCreate");
     hipcheck(hipEventCreate(&ev1_), "hipEventCreate");
@@ -80,6 +81,7 @@ class HipExecutor : public Executor {
              "hipFuncSetAttribute");
   }
   ~HipExecutor() override {
+    (void)hipStreamSynchronize(stream_);
     if (d_ops_) (void)hipFree(d_ops_);
     if (d_ranges_) (void)hipFree(d_ranges_);
     if (d_states_) (void)hipFree(d_states_);
@@ -88,6 +90,7 @@ class HipExecutor : public Executor {
     (void)hipStreamDestroy(stream_);
   }
   const char* name() const override { return name_.c_str(); }
+  bool cut_reverse_enabled() const override { return false; }
   double tick_hz() const override { return tick_hz_; }
   void* dmalloc(size_t bytes) override {
     void* p = nullptr;
@@ -95,7 +98,10 @@ class HipExecutor : public Executor {
     return p;
   }
   void dfree(void* p) override {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    (void)hipStreamSynchronize(stream_);   // nothing in flight may still use it
+    (void)hipFree(p);
+    (void)hipGetLastError();               // a failed free must not surface at a later launch
   }
   void h2d(void* dst, const void* src, size_t bytes) override {
     hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_), "h2d");
@@ -126,11 +132,10 @@ class HipExecutor : public Executor {
     hipcheck(hipMemcpyAsync(d_states_, packed.data(), sizeof(InstState) * nb, hipMemcpyHostToDevice, stream_), "states");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
     const int lds_cap = lds_doubles_for_batch(nb);
+    hipcheck(hipGetLastError(), "before launch exec_programs");
     hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
                        d_states_, d_ops_, d_ranges_, lds_cap);
-    hipcheck(hipGetLastError(), "launch exec_programs");
-    hipcheck(hipEventRecord(ev1_, stream_), "event record");
-    hipcheck(hipStreamSynchronize(stream_), "exec_programs");
+    finish_launch("exec_programs");
     float ms = 0.0f;
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
     launches += 1;
@@ -148,11 +153,10 @@ class HipExecutor : public Executor {
     for (int b = 0; b < nb; ++b) packed[b] = *states[b];
     hipcheck(hipMemcpyAsync(d_states_, packed.data(), sizeof(InstState) * nb, hipMemcpyHostToDevice, stream_), "states");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
+    hipcheck(hipGetLastError(), "before launch exec_backward");
     hipLaunchKernelGGL(exec_backward, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
                        d_states_, lds_cap);
-    hipcheck(hipGetLastError(), "launch exec_backward");
-    hipcheck(hipEventRecord(ev1_, stream_), "event record");
-    hipcheck(hipStreamSynchronize(stream_), "exec_backward");
+    finish_launch("exec_backward");
     float ms = 0.0f;
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
     bwd_launches += 1;
@@ -162,6 +166,17 @@ class HipExecutor : public Executor {
   }
 
  private:
+  // After a launch: the stream is always drained before any error leaves the
+  // executor, so no kernel can outlive the device memory the caller frees
+  // while it unwinds.
+  void finish_launch(const char* what) {
+    const hipError_t launch = hipGetLastError();
+    const hipError_t rec = hipEventRecord(ev1_, stream_);
+    const hipError_t sync = hipStreamSynchronize(stream_);
+    hipcheck(launch, what);
+    hipcheck(sync, what);
+    hipcheck(rec, "event record");
+  }
   void reserve(size_t nops, int nb) {
     if (nops > ops_cap_) {
       if (d_ops_) (void)hipFree(d_ops_);

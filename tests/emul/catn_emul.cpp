@@ -18,8 +18,18 @@ namespace catn {
 class CpuExecutor : public Executor {
  public:
   const char* name() const override { return "cpu-emulator"; }
-  void* dmalloc(size_t bytes) override { return calloc(1, bytes < 8 ? 8 : bytes); }
+  // CATN_EMU_POISON=1 fills new device blocks with 0xFF bytes (NaN doubles,
+  // -1 integers), as hipMalloc gives no zeroed memory: a read of memory the
+  // engine never wrote then shows up in the results.
+  void* dmalloc(size_t bytes) override {
+    void* p = calloc(1, bytes < 8 ? 8 : bytes);
+    if (p && getenv("CATN_EMU_POISON")) memset(p, 0xFF, bytes < 8 ? 8 : bytes);
+    return p;
+  }
   void dfree(void* p) override { free(p); }
+  // CATN_EMU_REFUSE_CUT_REVERSE=1 behaves like the gfx950 executor, which
+  // refuses gradient runs across a Dmax cut
+  bool cut_reverse_enabled() const override { return !getenv("CATN_EMU_REFUSE_CUT_REVERSE"); }
   void h2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
   void d2h(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
   void run(const std::vector<InstState*>& states,

@@ -58,7 +58,15 @@ struct EmuShared {
   explicit EmuShared(int n, int lds_doubles)
       : nthr(n), lds_cap(lds_doubles), xb(n), sites(n), gs(n), gx(n) {
     lds = new double[lds_doubles > 0 ? lds_doubles : 1];
-    for (int i = 0; i < lds_doubles; ++i) lds[i] = 0.0;
+    const uint64_t sentinel = 0x7ff4deadbeef0001ull;   // a NaN no kernel writes
+    for (int i = 0; i < lds_doubles; ++i) memcpy(lds + i, &sentinel, 8);
+  }
+  // one past the highest LDS double the launch wrote
+  int lds_high_water() const {
+    const uint64_t sentinel = 0x7ff4deadbeef0001ull;
+    for (int i = lds_cap - 1; i >= 0; --i)
+      if (memcmp(lds + i, &sentinel, 8) != 0) return i + 1;
+    return 0;
   }
   ~EmuShared() { delete[] lds; }
 };
@@ -103,6 +111,7 @@ static void emu_die(const char* what) {
 
 static std::vector<EmuLane*>* g_lanes = nullptr;   // lanes of the running launch
 extern std::atomic<long> g_divergent;
+extern std::atomic<long> g_lds_hwm;
 static void note_divergent() { g_divergent.fetch_add(1); }
 
 static void dump_history(int lane) {
@@ -246,8 +255,18 @@ class MtExecutor : public Executor {
  public:
   MtExecutor() { threads = 256; }
   const char* name() const override { return "cpu-emulator-mt"; }
-  void* dmalloc(size_t bytes) override { return calloc(1, bytes < 8 ? 8 : bytes); }
+  // CATN_EMU_POISON=1 fills new device blocks with 0xFF bytes (NaN doubles,
+  // -1 integers), as hipMalloc gives no zeroed memory: a read of memory the
+  // engine never wrote then shows up in the results.
+  void* dmalloc(size_t bytes) override {
+    void* p = calloc(1, bytes < 8 ? 8 : bytes);
+    if (p && getenv("CATN_EMU_POISON")) memset(p, 0xFF, bytes < 8 ? 8 : bytes);
+    return p;
+  }
   void dfree(void* p) override { free(p); }
+  // CATN_EMU_REFUSE_CUT_REVERSE=1 behaves like the gfx950 executor, which
+  // refuses gradient runs across a Dmax cut
+  bool cut_reverse_enabled() const override { return !getenv("CATN_EMU_REFUSE_CUT_REVERSE"); }
   void h2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
   void d2h(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
 
@@ -285,6 +304,10 @@ class MtExecutor : public Executor {
     done.store(true);
     wd.join();
     g_lanes = nullptr;
+    const int hw = sh.lds_high_water();
+    long cur = g_lds_hwm.load();
+    while (hw > cur && !g_lds_hwm.compare_exchange_weak(cur, hw)) {
+    }
   }
 
   void run(const std::vector<InstState*>& states,
@@ -306,8 +329,11 @@ class MtExecutor : public Executor {
 Executor* make_executor(int) { return new MtExecutor(); }
 
 std::atomic<long> g_divergent{0};
+std::atomic<long> g_lds_hwm{0};
 
 }  // namespace catn
 
 // barriers reached from different call sites since the library was loaded
 extern "C" long catn_emu_divergent_barriers(void) { return catn::g_divergent.load(); }
+// highest LDS extent (doubles) any launch wrote since the last call; resets
+extern "C" long catn_emu_lds_high_water(void) { return catn::g_lds_hwm.exchange(0); }

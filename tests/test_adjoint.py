@@ -34,6 +34,18 @@ def env(request, catn):
 
 EXACT = dict(Dmax=-1, chi=10_000)
 
+# Gradient runs across a Dmax cut faulted the GPU and the cause is not found
+# (the CPU emulators, also lane-parallel under ASan/TSan, run them cleanly);
+# the gfx950 executor refuses them before any launch, so their GPU cases are
+# skipped and the emulator cases keep the math covered.
+CUT_REVERSE_SKIP = ("reverse sweep across a Dmax cut is disabled on the gfx950 executor: "
+                    "such runs faulted the GPU, cause not found on the CPU")
+
+
+def _skip_cut_gradient_on_gpu(ctx):
+    if ctx.backend.startswith("hip"):
+        pytest.skip(CUT_REVERSE_SKIP)
+
 
 def grad(catn, ctx, n, edges, w, h, beta, **kw):
     tn = catn.ising_network(n, edges, w, h, beta, ctx=ctx, **kw)
@@ -163,6 +175,7 @@ def _frozen_fd(catn, ctx, n, edges, w, h, beta, idx_w, idx_h, eps=1e-4, **kw):
 @pytest.mark.parametrize("case", ["sg40_D8_chi16", "lattice6_pmJ_chi6", "lattice5_Dmax3"])
 def test_truncated_gradient_vs_finite_differences(env, case):
     catn, ctx = env
+    _skip_cut_gradient_on_gpu(ctx)
     if case == "sg40_D8_chi16":
         n, edges, w, h = Cf.spin_glass_3reg(40, seed=1)
         beta, kw = 1.0, dict(Dmax=8, chi=16)
@@ -201,6 +214,7 @@ def test_adjoint_close_to_pinning_when_truncated(env):
     """With truncation the two routes differentiate slightly different
     approximations; they agree to the truncation level."""
     catn, ctx = env
+    _skip_cut_gradient_on_gpu(ctx)
     n, edges, w, h = Cf.spin_glass_3reg(24, seed=5)
     kw = dict(Dmax=6, chi=12)
     m_adj = catn.magnetization(n, edges, w, h, 1.0, ctx=ctx, method="adjoint", **kw)
@@ -212,6 +226,7 @@ def test_adjoint_close_to_pinning_when_truncated(env):
 
 def test_recorded_forward_is_the_plain_forward(env):
     catn, ctx = env
+    _skip_cut_gradient_on_gpu(ctx)
     n, edges, w, h = Cf.spin_glass_3reg(30, seed=2)
     kw = dict(Dmax=8, chi=24)
     a = catn.ising_network(n, edges, w, h, 1.0, ctx=ctx, **kw)
@@ -245,3 +260,22 @@ def test_gradient_needs_a_fresh_network(env):
     tn2 = catn.ising_network(n, edges, w, h, beta, ctx=ctx, **EXACT)
     with pytest.raises(catn.CatnError):
         catn.ising_gradient(tn2, len(edges), n)     # no reverse sweep yet
+
+
+def test_device_refuses_gradient_across_a_cut_before_any_launch(catn, emu_ctx, monkeypatch):
+    """The refusal of the gfx950 executor, emulated: a gradient run whose
+    schedule reaches a Dmax cut raises before its first launch (no device op
+    executed), while a cut-free run is unaffected."""
+    monkeypatch.setenv("CATN_EMU_REFUSE_CUT_REVERSE", "1")
+    n, edges, w, h = Cf.spin_glass_3reg(40, seed=1)
+    tn = catn.ising_network(n, edges, w, h, 1.0, ctx=emu_ctx, Dmax=8, chi=16)
+    with pytest.raises(catn.CatnError, match="Dmax cut"):
+        catn.gradient_batch([tn])
+    assert tn.work()[0] == 0.0
+    n, edges, w, h, beta = _instance("lattice3", 1)
+    l, dw, dh = catn.lnz_and_grad(n, edges, w, h, beta, ctx=emu_ctx, **EXACT)
+    assert np.isfinite(l) and np.all(np.isfinite(dw))
+    # the plain forward with cuts is not affected
+    tn = catn.ising_network(*Cf.spin_glass_3reg(40, seed=1), 1.0, ctx=emu_ctx, Dmax=8, chi=16)
+    lz, _, _ = catn.contraction_batch([tn])
+    assert np.isfinite(lz[0])
