@@ -132,7 +132,37 @@ def run_chosen(catn, ctx, instances, steps, warmup):
                 bwd_launches=bl, bwd_seconds=bs, fwd_flops=fwd_flops, fwd_bytes=fwd_bytes)
 
 
-def run_plain(catn, ctx, inst, steps, warmup):
+def op_profile(nets):
+    """Device op profile summed over `nets`: seconds and counts per opcode and
+    per SVD size bucket, sub-step seconds, and the spread of per-network time."""
+    ops, buckets, sub, per_net = {}, {}, {}, []
+    for t in nets:
+        o, b = t.profile()
+        tot = 0.0
+        for k, (c, sec) in o.items():
+            a = ops.setdefault(k, [0, 0.0])
+            a[0] += c
+            a[1] += sec
+            tot += sec
+        per_net.append(tot)
+        for k, v in b.items():
+            if isinstance(v, tuple):
+                a = buckets.setdefault(k, [0, 0.0])
+                a[0] += v[0]
+                a[1] += v[1]
+            elif isinstance(v, dict):
+                d = sub.setdefault(k, {})
+                for kk, vv in v.items():
+                    d[kk] = d.get(kk, 0.0) + float(vv)
+            else:
+                sub[k] = sub.get(k, 0.0) + float(v)
+    per_net = np.array(per_net)
+    return {"networks": len(nets), "ops": ops, "svd_buckets": buckets, "sub_steps": sub,
+            "per_network_seconds": {"min": float(per_net.min()), "median": float(np.median(per_net)),
+                                    "max": float(per_net.max())}}
+
+
+def run_plain(catn, ctx, inst, steps, warmup, profile_path=""):
     n, edges, w, h = inst
     times, res = [], None
     flops = bytes_ = 0.0
@@ -155,6 +185,9 @@ def run_plain(catn, ctx, inst, steps, warmup):
             res = (lz.copy(), g)
         log(f"plain step {step} {'timed' if timed else 'warmup'}: {t1 - t0:.3f} s "
             f"(801 pinned networks), lnZ/N={lz[0] / n:.12f}")
+        if profile_path and step == warmup + steps - 1:
+            with open(profile_path, "w") as f:
+                json.dump(op_profile(nets), f, indent=1)
         del nets
     launches, ksec = ctx.kernel_stats()
     return dict(times=times, lz=res[0], grad=res[1], launches=launches, seconds=ksec,
@@ -171,6 +204,9 @@ def main():
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--op-profile", default="",
+                    help="write the device op profile of the last plain step (per opcode, per "
+                         "SVD size bucket, per-network spread) to this JSON file")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc) and exit")
@@ -219,9 +255,11 @@ def main():
             if args.form == "chosen":
                 args.form = "plain"
         if refused and plain is None:
-            plain = run_plain(catn, ctx, instances[0], args.steps, min(args.warmup, 1))
+            plain = run_plain(catn, ctx, instances[0], args.steps, min(args.warmup, 1),
+                              args.op_profile)
     if args.form in ("both", "plain") and plain is None:
-        plain = run_plain(catn, ctx, instances[0], args.plain_steps, min(args.warmup, 1))
+        plain = run_plain(catn, ctx, instances[0], args.plain_steps, min(args.warmup, 1),
+                          args.op_profile)
 
     # FP64 oracle lnZ of a few instances at the same (Dmax, chi)
     ncheck = 2

@@ -283,13 +283,16 @@ class TensorNetwork:
         ops = {self.OPNAMES[k]: (int(c[k]), float(t[k]) / hz.value) for k in range(16) if c[k]}
         buckets = {f"<={4 << k}" if k < 7 else ">256": (int(sc[k]), float(st[k]) / hz.value)
                    for k in range(8) if sc[k]}
-        sub = np.zeros(8, np.int64)
+        sub = np.zeros(16, np.int64)
         self.ctx._check(self.ctx.lib.catn_net_svd_profile(self.h, sub.ctypes.data_as(_I64P)))
         hzv = hz.value
         buckets["_large_svd"] = {"qr_s": sub[0] / hzv, "jacobi_s": sub[1] / hzv,
-                                 "finish_s": sub[2] / hzv, "sweeps": int(sub[3])}
+                                 "finish_s": sub[2] / hzv, "sweeps": int(sub[3]),
+                                 "global_sweeps": int(sub[11])}
         buckets["_certified_qr"] = {"qr_s": sub[4] / hzv, "inverse_s": sub[5] / hzv,
                                     "qform_s": sub[6] / hzv}
+        buckets["_householder"] = {"panel_s": sub[8] / hzv, "update_s": sub[9] / hzv,
+                                   "apply_q_s": sub[10] / hzv}
         buckets["_theta_s"] = sub[7] / hzv
         return ops, buckets
 

@@ -883,7 +883,7 @@ int catn_net_profile(catn_net* net, int64_t* op_ticks, int32_t* op_count, int64_
 }
 
 int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks) {
-  for (int k = 0; k < 8; ++k) sub_ticks[k] = net->hs.sub_ticks[k];
+  for (int k = 0; k < 16; ++k) sub_ticks[k] = net->hs.sub_ticks[k];
   return 0;
 }
 

@@ -511,7 +511,10 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     const int jb = (n - p) < kQB ? (n - p) : kQB;
     const int rows = m - p;
     double* Ap = A + p + lda * (int64_t)p;
+    int64_t t0 = now_ticks();
     wg_geqr2(g, Ap, rows, jb, lda, tau + p);
+    sub_tick(g, 8, t0);
+    t0 = now_ticks();
     wg_panel_v(g, Ap, rows, jb, lda, V);
     double* T = Tall + (int64_t)kQB * p;   // kQB x jb block, ld kQB
     wg_larft(g, V, rows, jb, tau + p, T, Y);
@@ -525,6 +528,7 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
       // C -= V W2
       wg_gemm(g, rows, nc, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Cp, lda);
     }
+    sub_tick(g, 9, t0);
   }
   return true;
 }
@@ -543,6 +547,7 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
   double* W = sc.alloc((int64_t)kQB * k);
   double* W2 = sc.alloc((int64_t)kQB * k);
   if (sc.fail) return;
+  const int64_t t0 = now_ticks();
   const int last = ((n - 1) / kQB) * kQB;
   for (int p = last; p >= 0; p -= kQB) {
     const int jb = (n - p) < kQB ? (n - p) : kQB;
@@ -555,6 +560,7 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
     wg_gemm_nn_t(g, jb, k, T, W, W2, false);
     wg_gemm(g, rows, k, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Xp, ldx);
   }
+  sub_tick(g, 10, t0);
   sc.top = save;
 }
 
@@ -564,8 +570,13 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
 // when not converged within maxsweeps. Round-robin ordering: n/2 disjoint
 // column pairs per step, one aligned group of L lanes per pair.
 // ---------------------------------------------------------------------------
+// lanes_per_pair = 0 spreads all lanes over the n/2 pairs of a round (short
+// LDS-resident columns); a wavefront-sized value gives each pair one
+// wavefront that walks its rows contiguously, which global-memory operands
+// need: coalesced loads and full-width reductions, each wavefront taking its
+// share of the round's pairs in turn.
 CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double* V,
-                       int64_t ldv, int maxsweeps, double thr_in = -1.0) {
+                       int64_t ldv, int maxsweeps, double thr_in = -1.0, int lanes_per_pair = 0) {
   for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     V[i + (int64_t)j * ldv] = (i == j) ? 1.0 : 0.0;
@@ -576,6 +587,7 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
   const int npairs = N / 2;
   int L = g.nthr / npairs;
   L = L < 1 ? 1 : pow2floor(L);
+  if (lanes_per_pair > L) L = pow2floor(lanes_per_pair < g.nthr ? lanes_per_pair : g.nthr);
   if (L > 64) L = 64;
   const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
   // pair convergence test on the cosine |a_p.a_q| / (|a_p||a_q|); a rotation
@@ -633,9 +645,14 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
   return -1;
 }
 
+// Jacobi on an operand in global memory: one wavefront per column pair.
+constexpr int kJacobiGlobalLanes = 64;
+
 CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double* V,
                       int64_t ldv, int maxsweeps) {
-  return jacobi_core(g, A, m, n, lda, V, ldv, maxsweeps);
+  const int sw = jacobi_core(g, A, m, n, lda, V, ldv, maxsweeps, -1.0, kJacobiGlobalLanes);
+  if (g.tid == 0 && g.sub && sw > 0) g.sub[11] += sw;
+  return sw;
 }
 
 // ---------------------------------------------------------------------------
@@ -734,13 +751,109 @@ CATN_DN int wg_jacobi_r_lds(const WG& g, const double* R, int n, double* U, int6
   return sw;
 }
 
+// Householder QR with column pivoting (LAPACK dgeqp2 structure) of A (m x n,
+// m >= n, lda): A P = Q R. R overwrites the upper triangle, the reflectors
+// (v(0) = 1 implicit) the part below it, tau[j] their factors; perm[j] is the
+// original index of column j. The pivot is the column of largest remaining
+// norm (first on ties); the trailing norms are recomputed exactly in each
+// update pass, so no norm downdating is involved. cn: n doubles of scratch.
+CATN_DN void wg_geqp2(const WG& g, double* A, int m, int n, int64_t lda, double* tau, int* perm,
+                      double* cn) {
+  {
+    int L = pow2floor(g.nthr / n < 1 ? 1 : g.nthr / n);
+    if (L > 64) L = 64;
+    const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
+    for (int c = grp; c < n; c += ng) {
+      const double* y = A + (int64_t)c * lda;
+      double w = 0.0;
+      for (int i = lane; i < m; i += L) w += y[i] * y[i];
+      w = grp_sum(w, L);
+      if (lane == 0) { cn[c] = w; perm[c] = c; }
+    }
+  }
+  g.sync();
+  for (int j = 0; j < n; ++j) {
+    // pivot: every lane scans the same norms, so the choice is uniform
+    int p = j;
+    double best = cn[j];
+    for (int c = j + 1; c < n; ++c)
+      if (cn[c] > best) { best = cn[c]; p = c; }
+    g.sync();   // all lanes have read cn before lane 0 swaps it
+    if (p != j) {
+      double* a = A + (int64_t)j * lda;
+      double* b = A + (int64_t)p * lda;
+      for (int i = g.tid; i < m; i += g.nthr) {
+        const double t = a[i];
+        a[i] = b[i];
+        b[i] = t;
+      }
+      if (g.tid == 0) {
+        const double t = cn[j]; cn[j] = cn[p]; cn[p] = t;
+        const int q = perm[j]; perm[j] = perm[p]; perm[p] = q;
+      }
+    }
+    g.sync();
+    double* x = A + j + (int64_t)j * lda;
+    double part = 0.0;
+    for (int i = 1 + g.tid; i < m - j; i += g.nthr) part += x[i] * x[i];
+    const double s = wg_sum(g, part);
+    const double alpha = x[0];
+    double t = 0.0, beta = alpha, scal = 0.0;
+    const int ncol = n - j - 1;
+    if (s > 0.0) {
+      const double nrm = sqrt(alpha * alpha + s);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      t = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    if (ncol > 0) {
+      int L = pow2floor(g.nthr / ncol < 1 ? 1 : g.nthr / ncol);
+      if (L > 64) L = 64;
+      const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
+      for (int c = grp; c < ncol; c += ng) {
+        double* y = A + j + (int64_t)(j + 1 + c) * lda;
+        const double y0 = y[0];
+        double w = 0.0;
+        if (s > 0.0) {
+          for (int i = 1 + lane; i < m - j; i += L) w += x[i] * y[i];
+          w = grp_sum(w, L);
+          w = t * (y0 + scal * w);
+        }
+        const double ws = w * scal;
+        double nq = 0.0;   // squared norm of the updated column below row j
+        for (int i = 1 + lane; i < m - j; i += L) {
+          const double v = y[i] - ws * x[i];
+          if (s > 0.0) y[i] = v;
+          nq += v * v;
+        }
+        nq = grp_sum(nq, L);
+        if (lane == 0) {
+          y[0] = y0 - w;
+          cn[j + 1 + c] = nq;
+        }
+      }
+    }
+    g.sync();   // every group has read x (unscaled) before it is scaled
+    if (s > 0.0)
+      for (int i = 1 + g.tid; i < m - j; i += g.nthr) x[i] *= scal;
+    if (g.tid == 0) {
+      tau[j] = t;
+      x[0] = beta;
+    }
+    g.sync();
+  }
+}
+
 CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
                         bool transpose_in, SvdResult& out, int* sweeps_used) {
   // SVD of B = A (m x n) or B = A^T when transpose_in (then m, n are B's).
   // Requires m >= n. Produces U (m x n), S, V (n x n) of B, sorted.
-  // Operands that fit run entirely in LDS; otherwise Householder QR (when
-  // m > n) reduces B to its n x n R factor, whose Jacobi runs in LDS when R
-  // and V fit, else block-pair by block-pair.
+  // Operands that fit run entirely in LDS. Otherwise (the QR-preconditioned
+  // one-sided Jacobi of Drmac and Veselic): blocked Householder QR reduces B
+  // to R0 (n x n; B itself when square), a column-pivoted QR gives
+  // R0 P = Q2 R, and the Jacobi sweeps run on R^T, whose graded columns
+  // converge in a few sweeps: R^T Vj = U' S, so B = (Q1 Q2 Vj) S (P U')^T.
+  // The Jacobi runs in LDS when R and its rotations fit, else in global memory.
   const int64_t lds_direct = (int64_t)m * n + (int64_t)n * n + 2 * (int64_t)n;
   if (lds_direct <= g.lds_cap) return wg_svd_lds(g, sc, A, m, n, lda, transpose_in, out, sweeps_used);
   double* W = sc.alloc((int64_t)m * n);
@@ -756,7 +869,7 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
   const bool use_qr = m > n;
   int sw = 0;
   double* tau = nullptr;
-  double* R = W;
+  double* R = W;   // n x n, ld n (B itself when square)
   int64_t t0 = now_ticks();
   double* Tall = nullptr;
   bool blocked = false;
@@ -773,10 +886,24 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
     }
     g.sync();
   }
+  // R0 P = Q2 R (in place), then X = R^T
+  double* tau2 = sc.alloc(n);
+  double* cn = sc.alloc(n);
+  int* perm = (int*)sc.alloc(n);
+  double* X = sc.alloc((int64_t)n * n);
+  double* Y = sc.alloc((int64_t)n * n);
+  double* Vj = sc.alloc((int64_t)n * n);
+  if (sc.fail) return -1;
+  wg_geqp2(g, R, n, n, n, tau2, perm, cn);
+  for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    X[idx] = (j <= i) ? R[j + (int64_t)i * n] : 0.0;
+  }
+  g.sync();
   sub_tick(g, 0, t0);
   t0 = now_ticks();
   if (2 * (int64_t)n * n + 2 * (int64_t)n <= g.lds_cap) {
-    sw = wg_jacobi_r_lds(g, R, n, U, m, Ss, Vs, sweeps_used);
+    sw = wg_jacobi_r_lds(g, X, n, Y, n, Ss, Vj, sweeps_used);
     sub_tick(g, 1, t0);
     t0 = now_ticks();
   } else {
@@ -784,12 +911,20 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
     double* key = sc.alloc(n);
     int* rk = (int*)sc.alloc(n);
     if (sc.fail) return -1;
-    sw = wg_jacobi(g, R, n, n, n, V, n, 60);
+    sw = wg_jacobi(g, X, n, n, n, V, n, 60);
     if (sweeps_used) *sweeps_used = sw;
     sub_tick(g, 1, t0);
     t0 = now_ticks();
-    jacobi_finish(g, R, n, n, n, V, n, key, rk, U, m, Ss, Vs);
+    jacobi_finish(g, X, n, n, n, V, n, key, rk, Y, n, Ss, Vj);
   }
+  // V of B: rows permuted back; U of B: Q1 Q2 Vj
+  for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
+    const int i = (int)(idx % n), c = (int)(idx / n);
+    Vs[perm[i] + (int64_t)c * n] = Y[idx];
+    U[i + (int64_t)c * m] = Vj[idx];
+  }
+  g.sync();
+  wg_apply_q(g, R, n, n, n, tau2, U, n, m);
   if (use_qr) {
     for (int64_t idx = g.tid; idx < (int64_t)(m - n) * n; idx += g.nthr) {
       const int i = n + (int)(idx % (m - n)), c = (int)(idx / (m - n));
@@ -852,7 +987,11 @@ struct Factor {
 // 1 / ||R^{-1}||_F of an upper-triangular n x n R (ld ldr); 0 if singular.
 // X = R^{-1} (n x n, ld n) is built row by row from the bottom: row i needs
 // only rows > i, so each step is one parallel pass over the columns j > i.
+CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int64_t ldr,
+                                        double* X);
+
 CATN_DN double wg_inv_fro_bound(const WG& g, const double* R, int n, int64_t ldr, double* X) {
+  if (n > 32) return wg_inv_fro_bound_blocked(g, R, n, ldr, X);
   double part = 0.0;
   for (int i = n - 1; i >= 0; --i) {
     const double rii = R[i + (int64_t)i * ldr];
@@ -869,6 +1008,58 @@ CATN_DN double wg_inv_fro_bound(const WG& g, const double* R, int n, int64_t ldr
       }
       X[i + (int64_t)j * n] = x;
       part += x * x;
+    }
+    g.sync();
+  }
+  const double tot = wg_sum(g, part);
+  if (!(tot < INFINITY) || tot <= 0.0) return 0.0;
+  return 1.0 / sqrt(tot);
+}
+
+// The same bound for larger R, by blocks of kInvB rows from the bottom:
+// X(I, I) = R(I, I)^{-1} by back substitution, one lane per column of the
+// block; then X(I, c) = -X(I, I) (R(I, i1:c) X(i1:c, c)) for the columns c
+// right of the block, one lane per column. Two barriers per block instead of
+// one per row.
+constexpr int kInvB = 16;
+
+CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int64_t ldr,
+                                        double* X) {
+  for (int i = 0; i < n; ++i)          // uniform: every lane reads the same diagonal
+    if (R[i + (int64_t)i * ldr] == 0.0) return 0.0;
+  double part = 0.0;
+  const int nb = (n + kInvB - 1) / kInvB;
+  for (int ib = nb - 1; ib >= 0; --ib) {
+    const int i0 = ib * kInvB;
+    const int i1 = (i0 + kInvB) < n ? (i0 + kInvB) : n;
+    const int bs = i1 - i0;
+    // diagonal block: column j of R(I,I)^{-1}, rows i0 + j down to i0
+    for (int j = g.tid; j < bs; j += g.nthr) {
+      const int c = i0 + j;
+      for (int r = i0 + bs - 1; r > c; --r) X[r + (int64_t)c * n] = 0.0;
+      for (int r = c; r >= i0; --r) {
+        double acc = r == c ? 1.0 : 0.0;
+        for (int q = r + 1; q <= c; ++q) acc -= R[r + (int64_t)q * ldr] * X[q + (int64_t)c * n];
+        const double x = acc / R[r + (int64_t)r * ldr];
+        X[r + (int64_t)c * n] = x;
+        part += x * x;
+      }
+    }
+    g.sync();
+    // columns right of the block
+    for (int c = i1 + g.tid; c < n; c += g.nthr) {
+      double t[kInvB];
+      for (int r = 0; r < bs; ++r) t[r] = 0.0;
+      for (int q = i1; q <= c; ++q) {
+        const double xq = X[q + (int64_t)c * n];
+        for (int r = 0; r < bs; ++r) t[r] += R[(i0 + r) + (int64_t)q * ldr] * xq;
+      }
+      for (int r = 0; r < bs; ++r) {
+        double acc = 0.0;
+        for (int q = r; q < bs; ++q) acc -= X[(i0 + r) + (int64_t)(i0 + q) * n] * t[q];
+        X[(i0 + r) + (int64_t)c * n] = acc;
+        part += acc * acc;
+      }
     }
     g.sync();
   }

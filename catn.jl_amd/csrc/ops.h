@@ -123,8 +123,10 @@ struct InstState {
   double alg_flops, alg_bytes;
   int64_t scratch_hwm;
   // sub-step ticks: SVD path 0 QR, 1 Jacobi, 2 finish + Q apply, 3 sweeps;
-  // certified-QR path 4 QR, 5 inverse bound, 6 Q formation; 7 swap theta
-  int64_t sub_ticks[8];
+  // certified-QR path 4 QR, 5 inverse bound, 6 Q formation; 7 swap theta;
+  // blocked Householder 8 panel factorisations, 9 T factors + trailing
+  // updates; 10 blocked Q application; 11 Jacobi sweeps in global memory
+  int64_t sub_ticks[16];
   int32_t last_cut;    // kept count of the last OP_CUT (read by the planner)
   int32_t record;      // 1: write a tape record per executed op
   TapeRec* tape;       // tape records (device), tape_cap entries

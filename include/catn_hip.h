@@ -133,8 +133,10 @@ int catn_net_work(catn_net* net, double* flops, double* bytes, int64_t* scratch_
 /* Number of interpreter-kernel launches on this context and their summed
  * duration in seconds (HIP events on the context's stream); reset != 0 zeroes. */
 int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t reset);
-/* Sub-step ticks (8 entries): SVD path [QR, Jacobi, finish + Q apply, Jacobi
- * sweep count], certified-QR path [QR, inverse bound, Q formation], swap theta. */
+/* Sub-step ticks (16 entries): SVD path [QR, Jacobi, finish + Q apply, Jacobi
+ * sweep count], certified-QR path [QR, inverse bound, Q formation], swap theta,
+ * blocked Householder [panels, T factors + trailing updates], blocked Q
+ * application, Jacobi sweeps on global-memory operands (count); rest zero. */
 int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks);
 /* Host time spent planning and in device round trips (launch, copies,
  * waits) since the last catn_kernel_stats reset, in seconds. */
