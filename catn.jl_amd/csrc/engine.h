@@ -456,16 +456,18 @@ CATN_D void wg_larft(const WG& g, const double* V, int rows, int jb, const doubl
                      double* Y) {
   // Y = V^T V (jb x jb)
   wg_gemm_skinny_tn(g, jb, jb, rows, V, rows, V, rows, Y, jb);
-  if (g.tid == 0) {
-    for (int i = 0; i < jb; ++i) {
-      for (int j = 0; j < i; ++j) {
-        double s = 0.0;
-        for (int l = j; l < i; ++l) s += T[j + kQB * l] * Y[l + jb * i];
-        T[j + kQB * i] = -tau[i] * s;
-      }
-      T[i + kQB * i] = tau[i];
-      for (int j = i + 1; j < jb; ++j) T[j + kQB * i] = 0.0;
+  // T(j, i) = -tau_i sum_{l=j}^{i-1} T(j, l) Y(l, i): row j of T needs only its
+  // own earlier entries, so lane j builds row j in registers.
+  for (int j = g.tid; j < jb; j += g.nthr) {
+    double row[kQB];
+    for (int i = 0; i < kQB; ++i) row[i] = 0.0;
+    row[j] = tau[j];
+    for (int i = j + 1; i < jb; ++i) {
+      double acc = 0.0;
+      for (int l = j; l < i; ++l) acc += row[l] * Y[l + jb * i];
+      row[i] = -tau[i] * acc;
     }
+    for (int i = 0; i < jb; ++i) T[j + kQB * i] = row[i];
   }
   g.sync();
 }
@@ -1081,22 +1083,21 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   // one QR'd (iso_left & !trans, or !iso_left & trans); otherwise identity
   const bool need_q = iso_left != trans;
   const int64_t tmr = (int64_t)tm * r, rr = (int64_t)r * r;
-  // small operands: W, R, R^{-1} (and Q) in LDS, unblocked Householder
-  const bool in_lds = tmr * (need_q ? 2 : 1) + 2 * rr + r + 64 <= g.lds_cap;
-  double *W, *R, *X, *Q = nullptr, *tau, *Tall = nullptr;
+  // small operands: W, R, R^{-1} in LDS, unblocked Householder; Q is formed
+  // straight into its scratch destination either way
+  const bool in_lds = tmr + 2 * rr + r + 64 <= g.lds_cap;
+  double *W, *R, *X, *tau, *Tall = nullptr;
   if (in_lds) {
     W = lds_base();
     R = W + tmr;
     X = R + rr;
     tau = X + rr;
-    if (need_q) Q = tau + r;
   } else {
     W = sc.alloc(tmr);
     tau = sc.alloc(r);
     R = sc.alloc(rr);
     X = sc.alloc(rr);
     Tall = sc.alloc((int64_t)kQB * r);
-    if (need_q) Q = sc.alloc(tmr);
     if (sc.fail) return false;
   }
   double fpart = 0.0;
@@ -1130,6 +1131,12 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
     return false;
   }
   F.r = r;
+  // outputs in scratch (LDS is reused by the caller's GEMMs)
+  double* L = sc.alloc((int64_t)m * r);
+  double* Rt = sc.alloc((int64_t)r * n);
+  double* Q = nullptr;
+  if (need_q) Q = (iso_left && !trans) ? L : sc.alloc(tmr);   // Q of the tall orientation
+  if (sc.fail) return false;
   if (need_q) {
     for (int64_t idx = g.tid; idx < tmr; idx += g.nthr) {
       const int i = (int)(idx % tm), j = (int)(idx / tm);
@@ -1140,12 +1147,7 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
     else wg_apply_q_blocked(g, sc, W, tm, r, tm, tau, blocked ? Tall : nullptr, Q, r, tm);
     sub_tick(g, 6, t0);
   }
-  // outputs in scratch (LDS is reused by the caller's GEMMs)
-  double* L = sc.alloc((int64_t)m * r);
-  double* Rt = sc.alloc((int64_t)r * n);
-  if (sc.fail) return false;
-  if (iso_left && !trans) {          // A = Q R: L = Q (m x n), Rt = R (n x n)
-    for (int64_t idx = g.tid; idx < tmr; idx += g.nthr) L[idx] = Q[idx];
+  if (iso_left && !trans) {          // A = Q R: L = Q (m x n, formed in place), Rt = R (n x n)
     for (int64_t idx = g.tid; idx < rr; idx += g.nthr) Rt[idx] = R[idx];
   } else if (iso_left && trans) {    // rank m: L = I_m, Rt = A (m x n)
     for (int64_t idx = g.tid; idx < (int64_t)m * m; idx += g.nthr)
