@@ -35,7 +35,7 @@ __all__ = [
     "Context", "TensorNetwork", "MPSNet", "ising_network", "tensor_network", "contraction",
     "contraction_batch", "free_energy", "magnetization", "correlation", "tsvd",
     "default_context", "LIB_PATH", "CatnError", "gradient_batch", "ising_gradient",
-    "lnz_and_grad", "lnz_and_grad_batch",
+    "lnz_and_grad", "lnz_and_grad_batch", "rsvd",
 ]
 
 
@@ -49,7 +49,7 @@ class _Params(C.Structure):
         ("norm_method", C.c_int32), ("select", C.c_int32), ("reverse", C.c_int32),
         ("svdopt", C.c_int32), ("swapopt", C.c_int32), ("compress", C.c_int32),
         ("cut_bond", C.c_int32), ("seed", C.c_int32), ("arena_elems", C.c_int64),
-        ("scratch_elems", C.c_int64),
+        ("scratch_elems", C.c_int64), ("rsvd_both", C.c_int32), ("rsvd_any", C.c_int32),
     ]
 
 
@@ -106,6 +106,8 @@ _SIGS = {
     "catn_node_site": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_int64]),
     "catn_tsvd": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_double, C.c_int32, _I32P, _DP,
                             _DP, _DP, _DP]),
+    "catn_rsvd": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_int32, C.c_int32, C.c_int32,
+                            C.c_int32, _I32P, _DP, _DP, _DP]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -197,7 +199,7 @@ def default_context() -> Context:
 
 def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse=True,
             svdopt=True, swapopt=True, compress=False, cut_bond=False, seed=1,
-            arena_elems=0, scratch_elems=0):
+            arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000):
     p = _Params()
     ctx.lib.catn_params_default(C.byref(p))
     p.Dmax, p.chi, p.cutoff = int(Dmax), int(chi), float(cutoff)
@@ -205,6 +207,7 @@ def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse
     p.reverse, p.svdopt, p.swapopt = int(bool(reverse)), int(bool(svdopt)), int(bool(swapopt))
     p.compress, p.cut_bond, p.seed = int(bool(compress)), int(bool(cut_bond)), int(seed)
     p.arena_elems, p.scratch_elems = int(arena_elems), int(scratch_elems)
+    p.rsvd_both, p.rsvd_any = int(rsvd_both), int(rsvd_any)
     return p
 
 
@@ -523,6 +526,22 @@ def tsvd(A, cutoff=1e-15, maxdim=None, *, ctx=None):
     U = Ub[: m * kk].reshape((m, kk), order="F")
     V = Vb[: n * kk].reshape((n, kk), order="F")
     return U, S[:kk].copy(), V, disc.value, S
+
+
+def rsvd(A, k, oversample=10, power=10, *, seed=1, ctx=None):
+    """rsvd(A, k, oversample, power; rng) (src/linalg_utils.jl:44-58) on the
+    device -> (U, S, V). The Gaussian sketch comes from a counter-based
+    generator seeded with `seed`."""
+    ctx = ctx or default_context()
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    m, n = A.shape
+    p = min(n, oversample * k)
+    kk = C.c_int32()
+    Ub, Vb, S = np.zeros(m * p), np.zeros(n * p), np.zeros(p)
+    ctx._check(ctx.lib.catn_rsvd(ctx.h, m, n, _dptr(A.ravel(order="F")), int(k), int(oversample),
+                                 int(power), int(seed), C.byref(kk), _dptr(Ub), _dptr(S), _dptr(Vb)))
+    q = kk.value
+    return Ub[: m * q].reshape((m, q), order="F"), S[:q].copy(), Vb[: n * q].reshape((n, q), order="F")
 
 
 class MPSNet:

@@ -37,31 +37,6 @@ namespace catn {
 
 constexpr double kBroaden = 1e-12;
 
-// C = alpha * op(A) op(B) + beta * C; tiled GEMM for products large enough to
-// amortise its barriers, else a direct loop (every lane owns whole outputs).
-CATN_DN void wg_gemm_any(const WG& g, int M, int N, int K, double alpha, const double* A,
-                         int64_t lda, bool tA, const double* B, int64_t ldb, bool tB, double beta,
-                         double* C, int64_t ldc) {
-  if (M <= 0 || N <= 0) return;
-  if ((int64_t)M * N >= 2048 && K >= 8) {
-    wg_gemm(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, ldc);
-    return;
-  }
-  const int64_t tot = (int64_t)M * N;
-  for (int64_t idx = g.tid; idx < tot; idx += g.nthr) {
-    const int i = (int)(idx % M), j = (int)(idx / M);
-    double s = 0.0;
-    for (int p = 0; p < K; ++p) {
-      const double a = tA ? A[p + lda * (int64_t)i] : A[i + lda * (int64_t)p];
-      const double b = tB ? B[j + ldb * (int64_t)p] : B[p + ldb * (int64_t)j];
-      s += a * b;
-    }
-    double* c = C + i + ldc * (int64_t)j;
-    *c = beta == 0.0 ? alpha * s : alpha * s + beta * *c;
-  }
-  g.sync();
-}
-
 CATN_D void wg_axpy(const WG& g, double* y, const double* x, double a, int64_t n) {
   for (int64_t i = g.tid; i < n; i += g.nthr) y[i] += a * x[i];
 }
@@ -423,6 +398,10 @@ CATN_DN void bwd_cano_l(const BwdCtx& c, const TapeRec& rc) {
 CATN_DN void bwd_swap(const BwdCtx& c, const TapeRec& rc) {
   Exec& E = *c.E;
   const WG& g = E.g;
+  if (rc.path == 2) {   // randomized SVD: no reverse step
+    E.status = STATUS_UNSUPPORTED;
+    return;
+  }
   const TapeSite &L = rc.in[0], &Rr = rc.in[1], &ol = rc.out[0], &orr = rc.out[1];
   const int d0 = L.dl, d2 = L.d, dd = L.dr, d1 = Rr.d, d3 = Rr.dr;
   const int m = d0 * d1, n = d2 * d3, k = rc.k;

@@ -170,6 +170,7 @@ struct catn_net {
       res_cap = 4096;
       hs = catn::InstState{};
       hs.psi = 1.0; hs.sign = 1.0;
+      hs.seed = pl.p.seed;
       hs.arena_cap = arena_cap;
       hs.scratch_cap = scratch_cap;
       hs.arena = (double*)ex->dmalloc(sizeof(double) * arena_cap);
@@ -590,6 +591,7 @@ void catn_params_default(catn_params* p) {
   p->Dmax = 32; p->chi = 32; p->cutoff = 1e-15; p->norm_method = 1; p->select = 1;
   p->reverse = 1; p->svdopt = 1; p->swapopt = 1; p->compress = 0; p->cut_bond = 0;
   p->seed = 1; p->arena_elems = 0; p->scratch_elems = 0;
+  p->rsvd_both = 7000; p->rsvd_any = 20000;
 }
 
 const char* catn_last_error(void) { return catn::g_last_error.c_str(); }
@@ -624,6 +626,8 @@ static void apply_params(catn_net* net, const catn_params* p) {
   q.select = p->select; q.reverse = p->reverse != 0; q.svdopt = p->svdopt != 0;
   q.swapopt = p->swapopt != 0; q.compress = p->compress != 0; q.cut_bond = p->cut_bond != 0;
   q.seed = p->seed;
+  q.rsvd_both = p->rsvd_both;
+  q.rsvd_any = p->rsvd_any;
   net->arena_cap = p->arena_elems;
   net->scratch_cap = p->scratch_elems;
 }
@@ -1071,6 +1075,39 @@ int catn_tsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, double cutof
   if (S) ctx->ex->d2h(S, net.hs.arena + dsd.off, sizeof(double) * (size_t)dsd.dl);
   if (k) *k = kk;
   if (discarded) *discarded = disc;
+  return 0;
+  CATN_CATCH
+}
+
+int catn_rsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, int32_t k, int32_t oversample,
+              int32_t power, int32_t seed, int32_t* kk, double* U, double* S, double* V) {
+  CATN_TRY
+  if (m <= 0 || n <= 0 || k <= 0 || oversample <= 0 || power < 0)
+    throw catn::Error(-2, "rsvd: dimensions, k and oversample must be positive, power >= 0");
+  const int64_t p = std::min<int64_t>(n, (int64_t)oversample * k);
+  if (p > m) throw catn::Error(-2, "rsvd: the sketch has more columns than A has rows");
+  catn_net net;
+  net.attach(ctx);
+  catn_params prm;
+  catn_params_default(&prm);
+  prm.seed = seed;
+  apply_params(&net, &prm);
+  const int64_t need = (int64_t)m * n;
+  net.arena_cap = 2 * need + 4 * ((int64_t)m + n) * p + 4096;
+  net.scratch_cap = 4 * need + 8 * ((int64_t)m + n) * p + 8 * p * p + 4096;
+  const int a = net.new_slot(m, 1, n, A);
+  const int su = net.new_slot(0, 0, 0, nullptr);
+  const int sv = net.new_slot(0, 0, 0, nullptr);
+  const int ss = net.new_slot(0, 0, 0, nullptr);
+  net.pl.nresults = 1;
+  net.pl.emit(catn::OP_TSVD, a, su, sv, k, 0, ss, 0.0, 1, oversample, power);
+  catn::run_one(&net);
+  const int kq = net.read_result(0);
+  catn::SiteDesc du = net.read_desc(su), dv = net.read_desc(sv), dsd = net.read_desc(ss);
+  if (U) ctx->ex->d2h(U, net.hs.arena + du.off, sizeof(double) * (size_t)m * kq);
+  if (V) ctx->ex->d2h(V, net.hs.arena + dv.off, sizeof(double) * (size_t)n * kq);
+  if (S) ctx->ex->d2h(S, net.hs.arena + dsd.off, sizeof(double) * (size_t)kq);
+  if (kk) *kk = kq;
   return 0;
   CATN_CATCH
 }

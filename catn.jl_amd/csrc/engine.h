@@ -420,6 +420,31 @@ CATN_D void wg_mm(const WG& g, int M, int N, int K, const double* A, int64_t lda
     wg_gemm_nn(g, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
+// C = alpha * op(A) op(B) + beta * C; tiled GEMM for products large enough to
+// amortise its barriers, else a direct loop (every lane owns whole outputs).
+CATN_DN void wg_gemm_any(const WG& g, int M, int N, int K, double alpha, const double* A,
+                         int64_t lda, bool tA, const double* B, int64_t ldb, bool tB, double beta,
+                         double* C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return;
+  if ((int64_t)M * N >= 2048 && K >= 8 && g.lds_cap >= 2 * kTK * kTM + 64) {
+    wg_gemm(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, ldc);
+    return;
+  }
+  const int64_t tot = (int64_t)M * N;
+  for (int64_t idx = g.tid; idx < tot; idx += g.nthr) {
+    const int i = (int)(idx % M), j = (int)(idx / M);
+    double s = 0.0;
+    for (int p = 0; p < K; ++p) {
+      const double a = tA ? A[p + lda * (int64_t)i] : A[i + lda * (int64_t)p];
+      const double b = tB ? B[j + ldb * (int64_t)p] : B[p + ldb * (int64_t)j];
+      s += a * b;
+    }
+    double* c = C + i + ldc * (int64_t)j;
+    *c = beta == 0.0 ? alpha * s : alpha * s + beta * *c;
+  }
+  g.sync();
+}
+
 // output index of the swap theta: X[(i + d0 j), (a + d1 b)] -> M[(i + d0 a), (j + d2 b)]
 struct IdxTheta {
   int d0, d1, d2;
@@ -960,6 +985,78 @@ CATN_DN int wg_svd(const WG& g, Scratch& sc, const double* A, int m, int n, int6
 
 
 // ---------------------------------------------------------------------------
+// Randomized SVD (src/linalg_utils.jl:44-58).
+//
+// The sketch is standard normal, drawn by a counter-based generator: entry i
+// of stream `stream` is a SplitMix64 hash of (seed, stream, i) turned into two
+// uniforms and a Box-Muller normal, so every lane draws its entries
+// independently and a replay draws the same sketch. (The reference draws
+// from Julia's global RNG; its stream is not reproducible elsewhere.)
+// ---------------------------------------------------------------------------
+CATN_D uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+CATN_D double gauss_at(uint64_t seed, uint64_t stream, uint64_t i) {
+  const uint64_t h1 = splitmix64(splitmix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) ^ (2 * i));
+  const uint64_t h2 = splitmix64(h1 ^ (2 * i + 1));
+  const double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);   // (0, 1)
+  const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);           // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+CATN_DN bool wg_thin_qr(const WG& g, Scratch& sc, double* Wk, int m, int n, double** Qo, double** Ro);
+
+// Y (m x p, ld m) := thin Q of Y, through scratch that is released again
+CATN_DN bool wg_orthonormalise(const WG& g, Scratch& sc, double* Y, int m, int p) {
+  const int64_t save = sc.top;
+  double *Q = nullptr, *R = nullptr;
+  if (!wg_thin_qr(g, sc, Y, m, p, &Q, &R)) return false;
+  wg_copy(g, Y, Q, (int64_t)m * p);
+  g.sync();
+  sc.top = save;
+  return true;
+}
+
+// rsvd of A (m x n, lda): U (m x kk, ld m), S (kk), V (n x kk, ld n) in
+// scratch, kk = min(k, p) with p = min(n, oversample k). Requires p <= m.
+// Returns kk, or -1 (scratch exhausted or the small SVD did not converge).
+CATN_DN int wg_rsvd(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda, int k,
+                    int oversample, int power, uint64_t seed, uint64_t stream, double** Uo,
+                    double** So, double** Vo) {
+  const int p = n < oversample * k ? n : oversample * k;
+  if (p > m || p <= 0) return -1;
+  double* G = sc.alloc((int64_t)n * p);
+  double* Y = sc.alloc((int64_t)m * p);
+  double* Z = sc.alloc((int64_t)n * p);
+  if (sc.fail) return -1;
+  for (int64_t idx = g.tid; idx < (int64_t)n * p; idx += g.nthr) G[idx] = gauss_at(seed, stream, idx);
+  g.sync();
+  wg_gemm_any(g, m, p, n, 1.0, A, lda, false, G, n, false, 0.0, Y, m);          // Y = A G
+  for (int it = 0; it < power; ++it) {
+    wg_gemm_any(g, n, p, m, 1.0, A, lda, true, Y, m, false, 0.0, Z, n);         // Z = A' Y
+    wg_gemm_any(g, m, p, n, 1.0, A, lda, false, Z, n, false, 0.0, Y, m);        // Y = A Z
+    if (!wg_orthonormalise(g, sc, Y, m, p)) return -1;
+  }
+  if (!wg_orthonormalise(g, sc, Y, m, p)) return -1;                            // Q = thinQ(Y)
+  double* B = Z;                                                                  // p x n (reuses Z)
+  wg_gemm_any(g, p, n, m, 1.0, Y, m, true, A, lda, false, 0.0, B, p);           // B = Q' A
+  SvdResult F;
+  if (wg_svd(g, sc, B, p, n, p, F, nullptr) < 0) return -1;
+  const int kk = k < F.r ? k : F.r;
+  double* U = sc.alloc((int64_t)m * kk);
+  if (sc.fail) return -1;
+  wg_gemm_any(g, m, kk, p, 1.0, Y, m, false, F.U, p, false, 0.0, U, m);         // U = Q U_B
+  *Uo = U;
+  *So = F.S;
+  *Vo = F.V;   // n x r (ld n); the first kk columns are the result
+  return kk;
+}
+
+// ---------------------------------------------------------------------------
 // Certified full-rank factorisation (no truncation possible).
 //
 // When every singular value of A (m x n) provably exceeds the truncation
@@ -1355,6 +1452,41 @@ CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool 
   return true;
 }
 
+// swap!'s randomized-SVD branch: rsvd(M, chi, 10, 10) (src/mps_node.jl:155)
+// with the reference's hand-over of the factors: going right U | S V',
+// going left U S | V'. No cutoff truncation, no discarded weight (the
+// reference returns 0.0 for this branch); path 2 in the tape.
+CATN_DN bool swap_rsvd(Exec& E, const double* M, int m, int n, bool iso_left, int chi, Split& sp) {
+  const WG& g = E.g;
+  double *U, *S, *V;
+  const int kk = wg_rsvd(g, E.sc, M, m, n, m, chi, 10, 10, (uint64_t)E.st->seed,
+                         (uint64_t)E.top, &U, &S, &V);
+  if (kk < 0) {
+    E.status = E.sc.fail ? STATUS_SCRATCH_FULL : (m < (n < 10 * chi ? n : 10 * chi)
+                                                       ? STATUS_UNSUPPORTED : STATUS_NOCONV);
+    return false;
+  }
+  double* L = E.sc.alloc((int64_t)m * kk);
+  double* R = E.sc.alloc((int64_t)kk * n);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return false; }
+  for (int64_t idx = g.tid; idx < (int64_t)m * kk; idx += g.nthr) {
+    const int p = (int)(idx / m);
+    L[idx] = iso_left ? U[idx] : U[idx] * S[p];
+  }
+  for (int64_t idx = g.tid; idx < (int64_t)kk * n; idx += g.nthr) {
+    const int p = (int)(idx % kk);
+    const int64_t c = idx / kk;
+    const double v = V[c + (int64_t)n * p];
+    R[idx] = iso_left ? S[p] * v : v;
+  }
+  g.sync();
+  const double p = n < 10.0 * chi ? n : 10.0 * chi;
+  E.work(2.0 * m * n * p * (1 + 2 * 10) + 2.0 * (10 + 1) * qr_flops(m, p) + 2.0 * p * m * n +
+             svd_flops(p, n) + 2.0 * m * p * kk, 0.0);
+  sp.k = kk; sp.L = L; sp.R = R; sp.disc = 0.0; sp.qr = 2;
+  return true;
+}
+
 CATN_DN void op_cano_r(Exec& E, const Op& op) {
   const WG& g = E.g;
   SiteDesc a = E.site(op.a), b = E.site(op.b);
@@ -1437,10 +1569,8 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   const int d0 = L.dl, d2 = L.d, dd = L.dr, d1 = Rr.d, d3 = Rr.dr;
   if (Rr.dl != dd) { E.status = STATUS_SHAPE; return; }
   const int m = d0 * d1, n = d2 * d3;
-  if (op.f && ((m > 7000 && n > 7000) || m > 20000 || n > 20000)) {
-    E.status = STATUS_UNSUPPORTED;   // randomized SVD branch (src/mps_node.jl:154-156)
-    return;
-  }
+  // randomized SVD for large thetas (src/mps_node.jl:154-156)
+  const bool use_rsvd = op.f && ((m > op.g && n > op.g) || m > op.h || n > op.h);
   double* M = E.sc.alloc((int64_t)m * n);
   if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
   const double* tl = E.ptr(L);
@@ -1466,7 +1596,11 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   g.sync();
   sub_tick(g, 7, tth);
   Split sp;
-  if (!wg_split(E, M, m, n, m, op.c != 0, 0, op.x, op.d, sp)) return;
+  if (use_rsvd) {
+    if (!swap_rsvd(E, M, m, n, op.c != 0, op.d, sp)) return;
+  } else if (!wg_split(E, M, m, n, m, op.c != 0, 0, op.x, op.d, sp)) {
+    return;
+  }
   const int k = sp.k;
   if (op.e) E.acc(&E.st->err, sp.disc);
   const int64_t offl = E.alloc((int64_t)m * k);
@@ -1897,6 +2031,25 @@ CATN_DN void op_tsvd(Exec& E, const Op& op) {
   const WG& g = E.g;
   SiteDesc s = E.site(op.a);
   const int m = s.dl * s.d, n = s.dr;
+  if (op.g == 1) {   // rsvd(A, k = op.d, oversample = op.h, power = op.i)
+    double *U, *S, *V;
+    const int kk = wg_rsvd(g, E.sc, E.ptr(s), m, n, m, op.d, op.h, op.i, (uint64_t)E.st->seed,
+                           0, &U, &S, &V);
+    if (kk < 0) { E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return; }
+    const int64_t offu = E.alloc((int64_t)m * kk);
+    const int64_t offv = E.alloc((int64_t)n * kk);
+    const int64_t offs = E.alloc(kk);
+    if (E.status) return;
+    wg_copy(g, E.st->arena + offu, U, (int64_t)m * kk);
+    wg_copy(g, E.st->arena + offv, V, (int64_t)n * kk);
+    wg_copy(g, E.st->arena + offs, S, kk);
+    E.set_site(op.b, offu, m, 1, kk);
+    E.set_site(op.c, offv, n, 1, kk);
+    E.set_site(op.f, offs, kk, 1, 1);
+    if (g.tid == 0) { E.st->results[op.e] = kk; E.st->dresults[op.e] = 0.0; }
+    g.sync();
+    return;
+  }
   SvdResult R;
   int sw = 0;
   if (wg_svd(g, E.sc, E.ptr(s), m, n, m, R, &sw) < 0) {

@@ -57,14 +57,19 @@ enum OpCode : int32_t {
   // (src/mps_node.jl:319-328, src/tensor_network.jl:489-501)
   OP_SCALAR_LOGNORM = 14,
   // standalone truncated SVD of the matrix held in slot a viewed as
-  // (dl*d) x dr: U -> slot b (m x k), S*V' kept in slot c (k x n), S -> result
-  // (src/linalg_utils.jl:15-28); d = maxdim, e = result index
+  // (dl*d) x dr: U -> slot b (m x k), V -> slot c (n x k), S -> slot f, k ->
+  // result e (src/linalg_utils.jl:15-28); d = maxdim. With g = 1 the
+  // randomized SVD rsvd(A, d, h, i) instead (src/linalg_utils.jl:44-58).
   OP_TSVD = 15,
 };
 
 struct Op {
   int32_t code;
   int32_t a, b, c, d, e, f;
+  // OP_SWAP: g, h = the randomized-SVD size switch (both dims > g, or either
+  // > h; src/mps_node.jl:154). OP_TSVD: g = 1 for rsvd (k = d), h = oversample,
+  // i = power iterations (src/linalg_utils.jl:44).
+  int32_t g, h, i;
   int32_t pad;
   double x;  // cutoff
 };
@@ -86,7 +91,7 @@ struct TapeSite {
 // count, norms, phases).
 struct TapeRec {
   int32_t code, nin, nout, k;
-  int32_t path;    // split ops: 1 certified QR, 0 Jacobi SVD
+  int32_t path;    // split ops: 1 certified QR, 0 Jacobi SVD, 2 randomized SVD
   int32_t flag;    // op specific (direction, counted log term, ...)
   int32_t aux0;
   int32_t aux1;    // LDS budget (doubles) of the launch that executed the op
@@ -137,6 +142,7 @@ struct InstState {
   int64_t bwd_ticks[16];
   double replay_dev;
   int32_t lds_max;     // largest LDS budget (doubles) of the launches that ran ops
+  int32_t seed;        // seed of the randomized-SVD sketches (TensorNetwork seed)
 };
 
 enum Status : int32_t {

@@ -39,6 +39,10 @@ struct Params {
   bool compress = false;
   bool cut_bond = false;
   int seed = 1;
+  // randomized-SVD size switch of swap! (src/mps_node.jl:154): both dims above
+  // rsvd_both, or either above rsvd_any
+  int rsvd_both = 7000;
+  int rsvd_any = 20000;
 };
 
 struct PNode {
@@ -66,9 +70,10 @@ class Planner {
 
   // ---- emission helpers ---------------------------------------------------
   void emit(int code, int a = 0, int b = 0, int c = 0, int d = 0, int e = 0, int f = 0,
-            double x = 0.0) {
+            double x = 0.0, int g = 0, int h = 0, int i = 0) {
     Op o;
-    o.code = code; o.a = a; o.b = b; o.c = c; o.d = d; o.e = e; o.f = f; o.pad = 0; o.x = x;
+    o.code = code; o.a = a; o.b = b; o.c = c; o.d = d; o.e = e; o.f = f;
+    o.g = g; o.h = h; o.i = i; o.pad = 0; o.x = x;
     prog.push_back(o);
   }
 
@@ -111,7 +116,7 @@ class Planner {
     }
     const int lo = std::min(i, j), hi = std::max(i, j);
     emit(OP_SWAP, n.slot[lo - 1], n.slot[hi - 1], i < j ? 1 : 0, p.chi, acc_err ? 1 : 0,
-         p.swapopt ? 1 : 0, p.cutoff);
+         p.swapopt ? 1 : 0, p.cutoff, p.rsvd_both, p.rsvd_any);
     std::swap(n.phys[lo - 1], n.phys[hi - 1]);
     n.cano = j;
     std::swap(n.nb[i - 1], n.nb[j - 1]);

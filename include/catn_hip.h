@@ -34,9 +34,13 @@ typedef struct catn_params {
   int32_t norm_method; /* 0 none, 1 L2, 2 max-abs */
   int32_t select;      /* 0 min-dim, 1 min-dim+triangle, 2 sequential */
   int32_t reverse, svdopt, swapopt, compress, cut_bond;
-  int32_t seed;
+  int32_t seed;          /* seed of the randomized-SVD sketches */
   int64_t arena_elems;   /* device arena per network (0 = default) */
   int64_t scratch_elems; /* device scratch per network (0 = default) */
+  /* randomized-SVD size switch of swap! (src/mps_node.jl:154): rsvd when both
+   * theta dimensions exceed rsvd_both or either exceeds rsvd_any (defaults
+   * 7000 and 20000, the reference's constants) */
+  int32_t rsvd_both, rsvd_any;
 } catn_params;
 
 /* Fill `p` with the reference defaults (Dmax=32, chi=32, cutoff=1e-15, ...). */
@@ -168,6 +172,14 @@ int catn_node_get(catn_net* net, int32_t id, int32_t* order, int32_t* cano, int3
                   int32_t* dims, int32_t cap);
 /* Download site k (1-based) data (dl*d*dr doubles). */
 int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t cap);
+
+/* rsvd(A, k, oversample, power; rng) (src/linalg_utils.jl:44-58) on the device:
+ * Gaussian sketch of p = min(n, oversample*k) columns drawn by a counter-based
+ * generator from `seed` (the reference draws from Julia's RNG), `power`
+ * iterations Y = thinQ(A (A' Y)), Q = thinQ(Y), B = Q' A, SVD of B, top
+ * kk = min(k, p) triplets. Outputs U (m x kk), S (kk), V (n x kk). */
+int catn_rsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, int32_t k, int32_t oversample,
+              int32_t power, int32_t seed, int32_t* kk, double* U, double* S, double* V);
 
 /* tsvd(A; cutoff, maxdim) (src/linalg_utils.jl:15-28) on the device.
  * A is m x n column-major. Outputs U (m x k), S (min(m,n), full spectrum,
