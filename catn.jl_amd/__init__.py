@@ -50,6 +50,7 @@ class _Params(C.Structure):
         ("svdopt", C.c_int32), ("swapopt", C.c_int32), ("compress", C.c_int32),
         ("cut_bond", C.c_int32), ("seed", C.c_int32), ("arena_elems", C.c_int64),
         ("scratch_elems", C.c_int64), ("rsvd_both", C.c_int32), ("rsvd_any", C.c_int32),
+        ("precision", C.c_int32),
     ]
 
 
@@ -199,7 +200,7 @@ def default_context() -> Context:
 
 def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse=True,
             svdopt=True, swapopt=True, compress=False, cut_bond=False, seed=1,
-            arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000):
+            arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000, dtype="f64"):
     p = _Params()
     ctx.lib.catn_params_default(C.byref(p))
     p.Dmax, p.chi, p.cutoff = int(Dmax), int(chi), float(cutoff)
@@ -208,6 +209,11 @@ def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse
     p.compress, p.cut_bond, p.seed = int(bool(compress)), int(bool(cut_bond)), int(seed)
     p.arena_elems, p.scratch_elems = int(arena_elems), int(scratch_elems)
     p.rsvd_both, p.rsvd_any = int(rsvd_both), int(rsvd_any)
+    levels = {"f64": 0, "f32": 1, "f32x6": 2}
+    if dtype not in levels:
+        raise CatnError(f"dtype must be 'f64' (Float64), 'f32' (Float32 storage, split-BF16 "
+                        f"3-pass theta) or 'f32x6' (6-pass theta), got {dtype!r}")
+    p.precision = levels[dtype]
     return p
 
 

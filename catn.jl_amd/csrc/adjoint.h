@@ -185,7 +185,8 @@ CATN_DN bool split_replay(Exec& E, const double* A, int m, int n, int64_t lda, b
   t.r = m < n ? m : n;
   if (path == 1) {
     const int maxd = rule == 1 ? (1 << 30) : maxdim;
-    if (!wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, t.F) || t.F.r != k) {
+    if (!wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, t.F, E.noise_floor()) ||
+        t.F.r != k) {
       E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_REPLAY;
       return false;
     }
@@ -310,9 +311,10 @@ CATN_DN bool split_adjoint(Exec& E, const SplitReplay& t, const double* Lb, cons
 // ---------------------------------------------------------------------------
 struct BwdCtx {
   Exec* E;
-  double* arena;
   double* adj;
-  CATN_D const double* x(const TapeSite& s) const { return arena + s.off; }
+  // FP64 view of a recorded tensor version (converted before the record runs
+  // when the network stores FP32)
+  CATN_D const double* x(const TapeSite& s) const { return E->view(s.off); }
   CATN_D double* a(const TapeSite& s) const { return adj + s.off; }
 };
 
@@ -409,7 +411,16 @@ CATN_DN void bwd_swap(const BwdCtx& c, const TapeRec& rc) {
   if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
   const double* tl = c.x(L);
   const double* tr = c.x(Rr);
-  if ((int64_t)m * n >= 2048 && dd >= 8 && g.lds_cap >= 2 * kTK * kTM + 64) {
+  if (E.st->f32) {   // the forward's split-BF16 theta, recomputed bit for bit
+    IdxTheta o;
+    o.d0 = d0; o.d1 = d1; o.d2 = d2; o.m = m;
+    if (E.st->f32 == 2)
+      wg_gemm_bf16<3>(g, d0 * d2, d1 * d3, dd, E.arena_f() + L.off, (int64_t)d0 * d2,
+                      E.arena_f() + Rr.off, dd, M, o);
+    else
+      wg_gemm_bf16<2>(g, d0 * d2, d1 * d3, dd, E.arena_f() + L.off, (int64_t)d0 * d2,
+                      E.arena_f() + Rr.off, dd, M, o);
+  } else if ((int64_t)m * n >= 2048 && dd >= 8 && g.lds_cap >= 2 * kTK * kTM + 64) {
     IdxTheta o;
     o.d0 = d0; o.d1 = d1; o.d2 = d2; o.m = m;
     wg_gemm_t(g, d0 * d2, d1 * d3, dd, 1.0, tl, (int64_t)d0 * d2, false, tr, dd, false, 0.0, M, o);
@@ -757,7 +768,6 @@ CATN_DN int run_backward(const WG& g, InstState* st) {
   g.sync();
   BwdCtx c;
   c.E = &E;
-  c.arena = st->arena;
   c.adj = st->adj;
   int64_t shwm = 0;
   for (int ti = st->tape_len - 1; ti >= 0 && E.status == STATUS_OK; --ti) {
@@ -772,6 +782,19 @@ CATN_DN int run_backward(const WG& g, InstState* st) {
     E.sc.cap = st->scratch_cap;
     E.sc.top = 0;
     E.sc.fail = false;
+    E.nconv = 0;
+    E.nstage = 0;
+    if (st->f32) {   // FP64 copies of the record's tensor versions
+      for (int q = 0; q < rc.nin && E.status == STATUS_OK; ++q)
+        E.convert_in(rc.in[q].off, tsize(rc.in[q]));
+      for (int q = 0; q < rc.nout && E.status == STATUS_OK; ++q)
+        E.convert_in(rc.out[q].off, tsize(rc.out[q]));
+      g.sync();
+      if (E.status != STATUS_OK) {
+        if (g.tid == 0) st->fail_op = ti;
+        break;
+      }
+    }
     switch (rc.code) {
       case OP_CANO_R: bwd_cano_r(c, rc); break;
       case OP_CANO_L: bwd_cano_l(c, rc); break;

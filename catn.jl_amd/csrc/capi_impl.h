@@ -109,6 +109,9 @@ struct catn_net {
   std::vector<double> iw, ih;
   int ipos1 = 0, ival1 = 0, ipos2 = 0, ival2 = 0;
   bool have_adjoint = false;
+  bool f32 = false;        // FP32 site storage (catn_params.precision = 1 or 2)
+  int bf16_terms = 2;      // BF16 terms per FP32 operand of the theta GEMM
+  size_t elem_bytes() const { return f32 ? sizeof(float) : sizeof(double); }
 
   ~catn_net() {
     release();
@@ -171,9 +174,10 @@ struct catn_net {
       hs = catn::InstState{};
       hs.psi = 1.0; hs.sign = 1.0;
       hs.seed = pl.p.seed;
+      hs.f32 = f32 ? (bf16_terms == 3 ? 2 : 1) : 0;
       hs.arena_cap = arena_cap;
       hs.scratch_cap = scratch_cap;
-      hs.arena = (double*)ex->dmalloc(sizeof(double) * arena_cap);
+      hs.arena = (double*)ex->dmalloc(elem_bytes() * arena_cap);   // floats when f32
       hs.scratch = (double*)ex->dmalloc(sizeof(double) * scratch_cap);
       hs.sites = (catn::SiteDesc*)ex->dmalloc(sizeof(catn::SiteDesc) * slot_cap);
       hs.results = (int32_t*)ex->dmalloc(sizeof(int32_t) * res_cap);
@@ -207,7 +211,14 @@ struct catn_net {
         descs.push_back({s.slot, d});
       }
       if (top > arena_cap) throw catn::Error(-4, "arena capacity exceeded while uploading sites");
-      if (!buf.empty()) ex->h2d(hs.arena + base, buf.data(), sizeof(double) * buf.size());
+      if (!buf.empty()) {
+        if (f32) {
+          std::vector<float> fb(buf.begin(), buf.end());
+          ex->h2d((float*)hs.arena + base, fb.data(), sizeof(float) * fb.size());
+        } else {
+          ex->h2d(hs.arena + base, buf.data(), sizeof(double) * buf.size());
+        }
+      }
       // staged slots are always the contiguous id range [first, nslots)
       const int first = descs.front().first;
       std::vector<catn::SiteDesc> block(descs.size());
@@ -591,7 +602,7 @@ void catn_params_default(catn_params* p) {
   p->Dmax = 32; p->chi = 32; p->cutoff = 1e-15; p->norm_method = 1; p->select = 1;
   p->reverse = 1; p->svdopt = 1; p->swapopt = 1; p->compress = 0; p->cut_bond = 0;
   p->seed = 1; p->arena_elems = 0; p->scratch_elems = 0;
-  p->rsvd_both = 7000; p->rsvd_any = 20000;
+  p->rsvd_both = 7000; p->rsvd_any = 20000; p->precision = 0;
 }
 
 const char* catn_last_error(void) { return catn::g_last_error.c_str(); }
@@ -628,6 +639,10 @@ static void apply_params(catn_net* net, const catn_params* p) {
   q.seed = p->seed;
   q.rsvd_both = p->rsvd_both;
   q.rsvd_any = p->rsvd_any;
+  if (p->precision < 0 || p->precision > 2)
+    throw catn::Error(-2, "precision must be 0 (FP64), 1 (FP32, split-BF16 3-pass) or 2 (FP32, 6-pass)");
+  net->f32 = p->precision != 0;
+  net->bf16_terms = p->precision == 2 ? 3 : 2;
   net->arena_cap = p->arena_elems;
   net->scratch_cap = p->scratch_elems;
 }
@@ -1043,7 +1058,13 @@ int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t ca
   catn::SiteDesc d = net->read_desc(nd.slot[k - 1]);
   const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
   if (cnt > cap) throw catn::Error(-2, "output buffer too small");
-  catn::exec_of(net)->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
+  if (net->f32) {
+    std::vector<float> fb(cnt);
+    catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off, sizeof(float) * cnt);
+    for (size_t q = 0; q < cnt; ++q) out[q] = fb[q];
+  } else {
+    catn::exec_of(net)->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
+  }
   return 0;
   CATN_CATCH
 }

@@ -445,6 +445,115 @@ CATN_DN void wg_gemm_any(const WG& g, int M, int N, int K, double alpha, const d
   g.sync();
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 GEMM for FP32 operands: C[out(r, c)] = sum_k A(r, k) B(k, c) with
+// A (M x K, lda) and B (K x N, ldb) in FP32, products on the BF16 matrix
+// cores, accumulated in FP32, widened to FP64. Each operand is split into
+// BF16 terms by successive round-to-nearest residuals:
+//   NT = 2: x = hi + lo (to 2^-17 relative); hi*hi + hi*lo + lo*hi, 3 passes;
+//   NT = 3: x = hi + mid + lo (to FP32 precision); the six products whose
+//           order is at most 2^-16 relative, 6 passes.
+// One wavefront per 16 x 16 output tile, v_mfma_f32_16x16x16_bf16 per
+// 16 x 16 x 16 block and pass.
+// ---------------------------------------------------------------------------
+CATN_D uint32_t f32_bits(float x) {
+  union { float f; uint32_t u; } v;
+  v.f = x;
+  return v.u;
+}
+CATN_D float f32_from_bits(uint32_t u) {
+  union { float f; uint32_t u; } v;
+  v.u = u;
+  return v.f;
+}
+// round to nearest even BF16 (the operands are finite)
+CATN_D uint16_t bf16_rne(float x) {
+  const uint32_t u = f32_bits(x);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+CATN_D float bf16_val(uint16_t h) { return f32_from_bits((uint32_t)h << 16); }
+template <int NT>
+CATN_D void bf16_split(float x, uint16_t* t) {
+  float r = x;
+  for (int q = 0; q < NT; ++q) {
+    t[q] = bf16_rne(r);
+    r -= bf16_val(t[q]);
+  }
+}
+// the (a-term, b-term) pairs of each pass
+template <int NT>
+struct Bf16Passes;
+template <>
+struct Bf16Passes<2> {
+  static constexpr int n = 3;
+  CATN_D static int a(int p) { return p == 2 ? 1 : 0; }
+  CATN_D static int b(int p) { return p == 1 ? 1 : 0; }
+};
+template <>
+struct Bf16Passes<3> {
+  static constexpr int n = 6;   // (0,0) (0,1) (1,0) (0,2) (1,1) (2,0)
+  CATN_D static int a(int p) { return (0x210100 >> (4 * p)) & 15; }
+  CATN_D static int b(int p) { return (0x012010 >> (4 * p)) & 15; }
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef short catn_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float catn_f32x4 __attribute__((ext_vector_type(4)));
+#endif
+
+template <int NT, class Out>
+CATN_DN void wg_gemm_bf16(const WG& g, int M, int N, int K, const float* A, int64_t lda,
+                          const float* B, int64_t ldb, double* C, Out out) {
+  typedef Bf16Passes<NT> P;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // 16 x 16 x 16 MFMA operand layout: lane l holds A(l % 16, 4 (l / 16) + v),
+  // B(4 (l / 16) + v, l % 16) and result C(4 (l / 16) + v, l % 16), v = 0..3
+  const int nw = g.nthr >> 6, w = g.tid >> 6, l = g.tid & 63;
+  const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+  const int lr = l & 15, lk = (l >> 4) << 2;
+  for (int t = w; t < tm * tn; t += nw) {
+    const int i0 = (t % tm) << 4, j0 = (t / tm) << 4;
+    catn_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      catn_bf16x4 at[NT], bt[NT];
+      for (int v = 0; v < 4; ++v) {
+        const int k = k0 + lk + v;
+        const int i = i0 + lr, j = j0 + lr;
+        const float a = (i < M && k < K) ? A[i + lda * (int64_t)k] : 0.0f;
+        const float b = (k < K && j < N) ? B[k + ldb * (int64_t)j] : 0.0f;
+        uint16_t ta[NT], tb[NT];
+        bf16_split<NT>(a, ta);
+        bf16_split<NT>(b, tb);
+        for (int q = 0; q < NT; ++q) {
+          at[q][v] = (short)ta[q];
+          bt[q][v] = (short)tb[q];
+        }
+      }
+      for (int p = 0; p < P::n; ++p)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(at[P::a(p)], bt[P::b(p)], acc, 0, 0, 0);
+    }
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + lk + v, j = j0 + lr;
+      if (i < M && j < N) C[out(i, j)] = (double)acc[v];
+    }
+  }
+#else
+  // the same products in FP32, k in order (the emulator's rendering)
+  for (int64_t idx = g.tid; idx < (int64_t)M * N; idx += g.nthr) {
+    const int i = (int)(idx % M), j = (int)(idx / M);
+    float acc = 0.0f;
+    for (int k = 0; k < K; ++k) {
+      uint16_t ta[NT], tb[NT];
+      bf16_split<NT>(A[i + lda * (int64_t)k], ta);
+      bf16_split<NT>(B[k + ldb * (int64_t)j], tb);
+      for (int p = 0; p < P::n; ++p) acc += bf16_val(ta[P::a(p)]) * bf16_val(tb[P::b(p)]);
+    }
+    C[out(i, j)] = (double)acc;
+  }
+#endif
+  g.sync();
+}
+
 // output index of the swap theta: X[(i + d0 j), (a + d1 b)] -> M[(i + d0 a), (j + d2 b)]
 struct IdxTheta {
   int d0, d1, d2;
@@ -1169,7 +1278,7 @@ CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int
 
 CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m, int n,
                                 int64_t lda, double cutoff, int maxdim, bool iso_left,
-                                Factor& F) {
+                                Factor& F, double noise = 0.0) {
   const int r = m < n ? m : n;
   if (r <= 0 || r > maxdim) return false;
   const int64_t save = sc.top;
@@ -1221,7 +1330,7 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   const double lb = wg_inv_fro_bound(g, R, r, r, X);
   sub_tick(g, 5, t0);
   t0 = now_ticks();
-  const double thr = fmax(100.0 * cutoff, 1e-12 * fro);
+  const double thr = fmax(fmax(100.0 * cutoff, 1e-12 * fro), noise * fro);
   if (!(lb > thr)) {
     g.sync();
     sc.top = save;
@@ -1343,12 +1452,78 @@ struct Exec {
     d.off = 0; d.dl = d.d = d.dr = 0; d.live = 0;
     return d;
   }
-  CATN_D double* ptr(const SiteDesc& d) const { return st->arena + d.off; }
+
+  // FP32 storage: an op reads FP64 copies of its input tensors (made before
+  // the op runs) and writes its outputs into FP64 staging buffers, which are
+  // rounded into the FP32 arena after the op. Both live in the op's scratch.
+  // With FP64 storage the arena is read and written in place.
+  static constexpr int kStage = 6;
+  int nconv, nstage;
+  int64_t conv_off[kStage], stage_off[kStage], stage_n[kStage];
+  double* conv_buf[kStage];
+  double* stage_buf[kStage];
+  CATN_D float* arena_f() const { return (float*)st->arena; }
+  // Singular values below noise_floor() * ||A||_F count as zero when the
+  // tensors are stored in FP32: the storage rounds every element (relative
+  // 2^-24) and the 3-pass split-BF16 theta product carries relative errors
+  // near 2^-16, so smaller singular values are rounding noise. Keeping them
+  // would grow bond dimensions in exact runs and, through the 1/sigma terms
+  // of the SVD adjoint, turn the FP32 orthogonality error of U into O(1)
+  // gradient errors. With FP64 storage the reference's cutoff alone decides.
+  CATN_D double noise_floor() const {
+    return st->f32 == 2 ? 0x1p-20 : st->f32 == 1 ? 0x1p-13 : 0.0;
+  }
+  // FP64 copy of the tensor version at arena offset `off` (n elements) into
+  // scratch; every lane calls it. False when the scratch is exhausted.
+  CATN_D bool convert_in(int64_t off, int64_t n) {
+    for (int q = 0; q < nconv; ++q)
+      if (conv_off[q] == off) return true;
+    if (nconv >= kStage) { status = STATUS_UNSUPPORTED; return false; }
+    double* buf = sc.alloc(n > 0 ? n : 1);
+    if (sc.fail) { status = STATUS_SCRATCH_FULL; return false; }
+    const float* src = arena_f() + off;
+    for (int64_t i = g.tid; i < n; i += g.nthr) buf[i] = (double)src[i];
+    conv_off[nconv] = off;
+    conv_buf[nconv] = buf;
+    ++nconv;
+    return true;
+  }
+  CATN_D const double* ptr(const SiteDesc& d) const { return view(d.off); }
+  CATN_D const double* view(int64_t off) const {
+    if (!st->f32) return st->arena + off;
+    for (int q = 0; q < nconv; ++q)
+      if (conv_off[q] == off) return conv_buf[q];
+    return conv_buf[0];   // not reached: every op input is converted before the op
+  }
   CATN_D int64_t alloc(int64_t n) {
     int64_t a = (top + 31) & ~int64_t(31);
     if (a + n > st->arena_cap) { status = STATUS_ARENA_FULL; return 0; }
     top = a + n;
+    if (st->f32) {
+      if (nstage >= kStage) { status = STATUS_UNSUPPORTED; return 0; }
+      double* buf = sc.alloc(n > 0 ? n : 1);
+      if (sc.fail) { status = STATUS_SCRATCH_FULL; return 0; }
+      stage_off[nstage] = a;
+      stage_n[nstage] = n;
+      stage_buf[nstage] = buf;
+      ++nstage;
+    }
     return a;
+  }
+  // where an op writes the output allocated at `off`
+  CATN_D double* out(int64_t off) const {
+    if (!st->f32) return st->arena + off;
+    for (int q = 0; q < nstage; ++q)
+      if (stage_off[q] == off) return stage_buf[q];
+    return stage_buf[0];   // not reached: outputs come from alloc()
+  }
+  // round the staged outputs into the FP32 arena (every lane; after the op)
+  CATN_D void flush_out() {
+    if (!st->f32) return;
+    float* dst = arena_f();
+    for (int q = 0; q < nstage; ++q)
+      for (int64_t i = g.tid; i < stage_n[q]; i += g.nthr)
+        dst[stage_off[q] + i] = (float)stage_buf[q][i];
   }
   CATN_D void set_site(int s, int64_t off, int dl, int d, int dr) {
     if (g.tid == 0) {
@@ -1390,6 +1565,12 @@ CATN_D int keep_cano(const double* S, int r, double cutoff) {
   return c == 0 ? c0 : c;
 }
 
+CATN_D double fro_of(const double* S, int r) {
+  double s = 0.0;
+  for (int i = 0; i < r; ++i) s += S[i] * S[i];
+  return sqrt(s);
+}
+
 CATN_D double tail_sum(const double* S, int from, int r) {
   double s = 0.0;
   for (int i = from; i < r; ++i) s += S[i];
@@ -1417,7 +1598,8 @@ CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool 
   const int maxd = rule == 1 ? (1 << 30) : maxdim;
   const int r0 = m < n ? m : n;
   E.bucket = size_bucket(r0);
-  if (wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, F)) {
+  const double noise = E.noise_floor();
+  if (wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, F, noise)) {
     sp.k = F.r; sp.L = F.L; sp.R = F.Rt; sp.disc = 0.0; sp.qr = 1;
     const double tm = m > n ? m : n;
     E.work(qr_flops(tm, r0) + r0 * (double)r0 * r0 / 3.0 + (iso_left != (m < n) ? qr_flops(tm, r0) : 0.0),
@@ -1433,6 +1615,7 @@ CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool 
     return false;
   }
   E.work(svd_flops(m, n), 0.0);
+  if (noise > 0.0) cutoff = fmax(cutoff, noise * fro_of(R.S, R.r));
   const int k = rule == 1 ? keep_cano(R.S, R.r, cutoff) : keep_tsvd(R.S, R.r, cutoff, maxdim);
   double* L = E.sc.alloc((int64_t)m * k);
   double* Rm = E.sc.alloc((int64_t)k * n);
@@ -1499,9 +1682,9 @@ CATN_DN void op_cano_r(Exec& E, const Op& op) {
   const int64_t offa = E.alloc((int64_t)m * k);
   const int64_t offb = E.alloc((int64_t)k * nb);
   if (E.status) return;
-  wg_copy(g, E.st->arena + offa, sp.L, (int64_t)m * k);
+  wg_copy(g, E.out(offa), sp.L, (int64_t)m * k);
   // new b = R * b   (k x n) * (n x nb)
-  wg_mm(g, k, (int)nb, n, sp.R, k, E.ptr(b), n, E.st->arena + offb, k);
+  wg_mm(g, k, (int)nb, n, sp.R, k, E.ptr(b), n, E.out(offb), k);
   E.work(2.0 * k * n * (double)nb, (double)m * n + (double)n * nb + (double)m * k + (double)k * nb);
   E.rec_begin(OP_CANO_R); E.rec_in(a); E.rec_in(b);
   E.rec_out(offa, a.dl, a.d, k); E.rec_out(offb, k, b.d, b.dr);
@@ -1536,8 +1719,8 @@ CATN_DN void op_cano_l(Exec& E, const Op& op) {
   const int64_t offa = E.alloc((int64_t)k * m);
   const int64_t offb = E.alloc(rb * k);
   if (E.status) return;
-  double* na = E.st->arena + offa;
-  double* nbp = E.st->arena + offb;
+  double* na = E.out(offa);
+  double* nbp = E.out(offb);
   // new a[p, y, z] = L[y + d*z, p]
   for (int64_t idx = g.tid; idx < (int64_t)k * m; idx += g.nthr) {
     const int p = (int)(idx % k);
@@ -1578,7 +1761,17 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   // M[(i + d0*a), (j + d2*b)] = sum_k tl[i, j, k] tr[k, a, b]: the GEMM
   // (tl as (d0 d2) x dd) * (tr as dd x (d1 d3)) written through the index map
   const int64_t tth = now_ticks();
-  if ((int64_t)m * n >= 2048 && dd >= 8 && g.lds_cap >= 2 * kTK * kTM + 64) {
+  if (E.st->f32) {
+    // FP32 storage: split-BF16 MFMA products straight from the FP32 arena
+    IdxTheta o;
+    o.d0 = d0; o.d1 = d1; o.d2 = d2; o.m = m;
+    if (E.st->f32 == 2)
+      wg_gemm_bf16<3>(g, d0 * d2, d1 * d3, dd, E.arena_f() + L.off, (int64_t)d0 * d2,
+                      E.arena_f() + Rr.off, dd, M, o);
+    else
+      wg_gemm_bf16<2>(g, d0 * d2, d1 * d3, dd, E.arena_f() + L.off, (int64_t)d0 * d2,
+                      E.arena_f() + Rr.off, dd, M, o);
+  } else if ((int64_t)m * n >= 2048 && dd >= 8 && g.lds_cap >= 2 * kTK * kTM + 64) {
     IdxTheta o;
     o.d0 = d0; o.d1 = d1; o.d2 = d2; o.m = m;
     wg_gemm_t(g, d0 * d2, d1 * d3, dd, 1.0, tl, (int64_t)d0 * d2, false, tr, dd, false, 0.0, M, o);
@@ -1606,8 +1799,8 @@ CATN_DN void op_swap(Exec& E, const Op& op) {
   const int64_t offl = E.alloc((int64_t)m * k);
   const int64_t offr = E.alloc((int64_t)k * n);
   if (E.status) return;
-  wg_copy(g, E.st->arena + offl, sp.L, (int64_t)m * k);
-  wg_copy(g, E.st->arena + offr, sp.R, (int64_t)k * n);
+  wg_copy(g, E.out(offl), sp.L, (int64_t)m * k);
+  wg_copy(g, E.out(offr), sp.R, (int64_t)k * n);
   E.work(2.0 * m * n * (double)dd,
          (double)d0 * d2 * dd + (double)dd * d1 * d3 + (double)m * k + (double)k * n);
   E.rec_begin(OP_SWAP); E.rec_in(L); E.rec_in(Rr);
@@ -1626,7 +1819,7 @@ CATN_DN void op_permute321(Exec& E, const Op& op) {
   const int64_t off = E.alloc(tot);
   if (E.status) return;
   const double* s = E.ptr(a);
-  double* dst = E.st->arena + off;
+  double* dst = E.out(off);
   // dst[z, y, x] = s[x, y, z]; dst dims (dr, d, dl)
   for (int64_t idx = g.tid; idx < tot; idx += g.nthr) {
     const int z = (int)(idx % a.dr);
@@ -1688,7 +1881,7 @@ CATN_D void scale_normalize(Exec& E, int slot, const SiteDesc& s, int method, do
   E.work(3.0 * n, 2.0 * n);
   const int64_t off = E.alloc(n);
   if (E.status) return;
-  double* y = E.st->arena + off;
+  double* y = E.out(off);
   const bool counted = nrm > cutoff;   // src/mps_node.jl:522-524, :569-571
   const double inv = counted ? 1.0 / nrm : 1.0;
   for (int64_t i = g.tid; i < n; i += g.nthr) y[i] = x[i] * inv;
@@ -1718,7 +1911,7 @@ CATN_DN void op_eat_b(Exec& E, const Op& op) {
   const int64_t rows = (int64_t)p.dl * p.d;
   const int64_t off = E.alloc(rows);
   if (E.status) return;
-  double* np_ = E.st->arena + off;
+  double* np_ = E.out(off);
   const double* pp = E.ptr(p);
   for (int64_t r = g.tid; r < rows; r += g.nthr) {
     double s = 0.0;
@@ -1745,7 +1938,7 @@ CATN_DN void op_eat_c(Exec& E, const Op& op) {
   const int64_t rows = (int64_t)p.dl * p.d;
   const int64_t off = E.alloc(rows * Dr);
   if (E.status) return;
-  wg_mm(g, (int)rows, Dr, Dl, E.ptr(p), rows, mat, Dl, E.st->arena + off, rows);
+  wg_mm(g, (int)rows, Dr, Dl, E.ptr(p), rows, mat, Dl, E.out(off), rows);
   E.work(2.0 * Dl * Dr * d + 2.0 * rows * Dl * Dr,
          (double)Dl * d + (double)d * Dr + rows * Dl + rows * Dr);
   E.rec_begin(OP_EAT_C); E.rec_in(p); E.rec_in(t); E.rec_in(h); E.rec_out(off, p.dl, p.d, Dr);
@@ -1767,7 +1960,7 @@ CATN_DN void op_merge_fuse(Exec& E, const Op& op) {
   const int m = a.dl * a.d, n = b.d * b.dr, k = a.dr;
   const int64_t off = E.alloc((int64_t)m * n);
   if (E.status) return;
-  wg_mm(g, m, n, k, E.ptr(a), m, E.ptr(b), k, E.st->arena + off, m);
+  wg_mm(g, m, n, k, E.ptr(a), m, E.ptr(b), k, E.out(off), m);
   E.work(2.0 * m * n * (double)k, (double)m * k + (double)k * n + (double)m * n);
   E.rec_begin(OP_MERGE_FUSE); E.rec_in(a); E.rec_in(b); E.rec_out(off, a.dl, a.d * b.d, b.dr);
   E.rec_commit();
@@ -1849,7 +2042,8 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
     E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_NOCONV; return;
   }
   E.bucket = size_bucket(R.r);
-  const int nk = keep_tsvd(R.S, R.r, op.x, 1 << 30);
+  const double cut = fmax(op.x, E.noise_floor() * fro_of(R.S, R.r));
+  const int nk = keep_tsvd(R.S, R.r, cut, 1 << 30);
   int myd = nk < op.d ? nk : op.d;
   if (myd == 0) myd = 1;
   E.acc(&E.st->err, tail_sum(R.S, myd, R.r));
@@ -1882,8 +2076,8 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
   const int64_t offa = E.alloc((int64_t)dal * myd * dar);
   const int64_t offb = E.alloc((int64_t)dbl * myd * dbr);
   if (E.status) return;
-  double* na = E.st->arena + offa;
-  double* nb = E.st->arena + offb;
+  double* na = E.out(offa);
+  double* nb = E.out(offb);
   // A'[a, p, c] = Fi[a + dal*c, p]
   for (int64_t idx = g.tid; idx < (int64_t)dal * myd * dar; idx += g.nthr) {
     const int a = (int)(idx % dal);
@@ -1980,8 +2174,8 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
   const int64_t offa = E.alloc((int64_t)m * k);
   const int64_t offb = E.alloc((int64_t)k * n);
   if (E.status) return;
-  wg_copy(g, E.st->arena + offa, Gl, (int64_t)m * k);
-  wg_copy(g, E.st->arena + offb, Gr, (int64_t)k * n);
+  wg_copy(g, E.out(offa), Gl, (int64_t)m * k);
+  wg_copy(g, E.out(offb), Gr, (int64_t)k * n);
   E.work(opt ? (fl ? 2.0 * m * dd * (double)k : 0.0) + (fr ? 2.0 * n * dd * (double)k : 0.0)
              : 2.0 * m * n * (double)dd,
          (double)m * dd + (double)dd * n + (double)m * k + (double)k * n);
@@ -2003,8 +2197,8 @@ CATN_DN void op_raw_split(Exec& E, const Op& op) {
   const int64_t offu = E.alloc((int64_t)m * k);
   const int64_t offr = E.alloc((int64_t)k * n);
   if (E.status) return;
-  wg_copy(g, E.st->arena + offu, sp.L, (int64_t)m * k);
-  wg_copy(g, E.st->arena + offr, sp.R, (int64_t)k * n);
+  wg_copy(g, E.out(offu), sp.L, (int64_t)m * k);
+  wg_copy(g, E.out(offr), sp.R, (int64_t)k * n);
   E.work(0.0, (double)m * n + (double)m * k + (double)k * n);
   E.rec_begin(OP_RAW_SPLIT); E.rec_commit();
   E.set_site(op.b, offu, dleft, di, k);
@@ -2040,9 +2234,9 @@ CATN_DN void op_tsvd(Exec& E, const Op& op) {
     const int64_t offv = E.alloc((int64_t)n * kk);
     const int64_t offs = E.alloc(kk);
     if (E.status) return;
-    wg_copy(g, E.st->arena + offu, U, (int64_t)m * kk);
-    wg_copy(g, E.st->arena + offv, V, (int64_t)n * kk);
-    wg_copy(g, E.st->arena + offs, S, kk);
+    wg_copy(g, E.out(offu), U, (int64_t)m * kk);
+    wg_copy(g, E.out(offv), V, (int64_t)n * kk);
+    wg_copy(g, E.out(offs), S, kk);
     E.set_site(op.b, offu, m, 1, kk);
     E.set_site(op.c, offv, n, 1, kk);
     E.set_site(op.f, offs, kk, 1, 1);
@@ -2061,9 +2255,9 @@ CATN_DN void op_tsvd(Exec& E, const Op& op) {
   const int64_t offv = E.alloc((int64_t)n * k);
   const int64_t offs = E.alloc(R.r);
   if (E.status) return;
-  wg_copy(g, E.st->arena + offu, R.U, (int64_t)m * k);
-  wg_copy(g, E.st->arena + offv, R.V, (int64_t)n * k);
-  wg_copy(g, E.st->arena + offs, R.S, R.r);
+  wg_copy(g, E.out(offu), R.U, (int64_t)m * k);
+  wg_copy(g, E.out(offv), R.V, (int64_t)n * k);
+  wg_copy(g, E.out(offs), R.S, R.r);
   E.set_site(op.b, offu, m, 1, k);
   E.set_site(op.c, offv, n, 1, k);
   E.set_site(op.f, offs, R.r, 1, 1);
@@ -2095,6 +2289,24 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
     E.sc.top = 0;
     E.sc.fail = false;
     E.snapshot(op);
+    E.nconv = 0;
+    E.nstage = 0;
+    if (st->f32) {
+      // FP64 copies of the op's input tensors (slots a, b; c for the eat! folds)
+      const int nin = (op.code == OP_EAT_B || op.code == OP_EAT_C) ? 3
+                      : (op.code == OP_PERMUTE321 || op.code == OP_NORMALIZE ||
+                         op.code == OP_RAW_SPLIT || op.code == OP_SCALAR_LOGNORM ||
+                         op.code == OP_TSVD) ? 1 : 2;
+      for (int q = 0; q < nin && E.status == STATUS_OK; ++q) {
+        const SiteDesc d = E.op_desc[q];
+        if (E.op_slot[q] >= 0) E.convert_in(d.off, (int64_t)d.dl * d.d * d.dr);
+      }
+      g.sync();
+      if (E.status != STATUS_OK) {
+        if (g.tid == 0) st->fail_op = i;
+        break;
+      }
+    }
     switch (op.code) {
       case OP_CANO_R: op_cano_r(E, op); break;
       case OP_CANO_L: op_cano_l(E, op); break;
@@ -2117,6 +2329,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
     if (E.sc.top > shwm) shwm = E.sc.top;
     if (E.status != STATUS_OK && g.tid == 0) st->fail_op = i;
     g.sync();
+    if (E.status == STATUS_OK) E.flush_out();
     if (g.tid == 0) {
       const int64_t dt = now_ticks() - t0;
       const int c = op.code & 15;

@@ -143,6 +143,11 @@ struct InstState {
   double replay_dev;
   int32_t lds_max;     // largest LDS budget (doubles) of the launches that ran ops
   int32_t seed;        // seed of the randomized-SVD sketches (TensorNetwork seed)
+  // 1 or 2: site tensors are stored in FP32 (the arena holds floats at the
+  // same element offsets; the adjoint arena stays FP64), ops compute in FP64
+  // and the swap theta GEMM runs as split-BF16 products accumulated in FP32,
+  // with 2 (3 passes) or, for 2, 3 BF16 terms per operand (6 passes)
+  int32_t f32;
 };
 
 enum Status : int32_t {

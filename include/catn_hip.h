@@ -41,6 +41,12 @@ typedef struct catn_params {
    * theta dimensions exceed rsvd_both or either exceeds rsvd_any (defaults
    * 7000 and 20000, the reference's constants) */
   int32_t rsvd_both, rsvd_any;
+  /* 0: FP64 site tensors (Float64). 1: FP32 storage (the reference's
+   * Float32 networks, cu(tn)): site tensors rounded to FP32 between ops, ops
+   * computing in FP64, the swap theta GEMM as split-BF16 (hi/lo, 3-pass) MFMA
+   * products accumulated in FP32. 2: the same with a three-term BF16 split
+   * (6 passes, FP32-accurate products). Log-norms accumulate in FP64. */
+  int32_t precision;
 } catn_params;
 
 /* Fill `p` with the reference defaults (Dmax=32, chi=32, cutoff=1e-15, ...). */

@@ -43,7 +43,8 @@ def _run(catn, ctx, n, edges, w, h, beta, **kw):
     return lz[0], err[0], dw, dh, tn.edge_log()
 
 
-@pytest.mark.parametrize("case", ["sg16_D6_chi12", "lattice4_Dmax3", "sg12_select2_norm2"])
+@pytest.mark.parametrize("case", ["sg16_D6_chi12", "lattice4_Dmax3", "sg12_select2_norm2",
+                                  "sg16_f32"])
 def test_lane_parallel_matches_one_thread(catn, ctxs, case):
     mt, st = ctxs
     if case == "sg16_D6_chi12":
@@ -55,6 +56,9 @@ def test_lane_parallel_matches_one_thread(catn, ctxs, case):
         r = np.random.default_rng(3)
         w, h = r.uniform(-1, 1, len(edges)), r.normal(0, 0.2, n)
         beta, kw = 0.9, dict(Dmax=3, chi=32)
+    elif case == "sg16_f32":
+        n, edges, w, h = Cf.spin_glass_3reg(16, seed=4)
+        beta, kw = 1.0, dict(Dmax=6, chi=12, dtype="f32x6")
     else:
         n, edges, w, h = Cf.spin_glass_3reg(12, seed=2)
         beta, kw = 1.0, dict(Dmax=4, chi=8, select=2, norm_method=2)
@@ -63,8 +67,10 @@ def test_lane_parallel_matches_one_thread(catn, ctxs, case):
     b = _run(catn, st, n, edges, w, h, beta, **kw)
     assert _divergent() == before
     assert a[4] == b[4]                              # same edge sequence
-    assert abs(a[0] - b[0]) / n < 1e-13
+    tol = 1e-7 if kw.get("dtype") else 1e-13   # FP32 storage: rounding of the stored sites
+    assert abs(a[0] - b[0]) / n < tol
     assert abs(a[1] - b[1]) <= 1e-9 * max(1.0, abs(b[1]))
     scale = max(1.0, np.abs(b[2]).max(), np.abs(b[3]).max())
-    assert np.abs(a[2] - b[2]).max() < 1e-9 * scale
-    assert np.abs(a[3] - b[3]).max() < 1e-9 * scale
+    gtol = 1e-4 if kw.get("dtype") else 1e-9
+    assert np.abs(a[2] - b[2]).max() < gtol * scale
+    assert np.abs(a[3] - b[3]).max() < gtol * scale
