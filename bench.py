@@ -162,16 +162,30 @@ def op_profile(nets):
                                     "max": float(per_net.max())}}
 
 
-def run_plain(catn, ctx, inst, steps, warmup, profile_path=""):
+def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
     n, edges, w, h = inst
     times, res = [], None
     flops = bytes_ = 0.0
+    tuned = None
+    if tune:
+        # launch shape (lanes, LDS per workgroup) picked by timing this solve's
+        # batch once per shape, before the warmup and the timed steps
+        def build():
+            nets = pinned_batch(catn, ctx, n, edges, w, h)
+            _prefetch(nets)
+            return nets
+        tuned = catn.autotune(ctx, build)
+        log("autotune: " + ", ".join(f"({r['threads']},{r['lds_doubles']}) {r['seconds']:.3f} s"
+                                     for r in tuned["table"])
+            + f" -> ({tuned['threads']},{tuned['lds_doubles']})")
+    plan0 = 0.0
     for step in range(warmup + steps):
         nets = pinned_batch(catn, ctx, n, edges, w, h)
         _prefetch(nets)
         timed = step >= warmup
         if step == warmup:
             ctx.kernel_stats(reset=True)
+            plan0 = ctx.host_stats()[0]
         t0 = time.perf_counter()
         lz, _, _ = catn.contraction_batch(nets)
         g = gradient_from_lnz(lz, n, edges)
@@ -190,8 +204,58 @@ def run_plain(catn, ctx, inst, steps, warmup, profile_path=""):
                 json.dump(op_profile(nets), f, indent=1)
         del nets
     launches, ksec = ctx.kernel_stats()
+    plan = (ctx.host_stats()[0] - plan0) / max(steps, 1)
     return dict(times=times, lz=res[0], grad=res[1], launches=launches, seconds=ksec,
-                flops=flops, bytes=bytes_)
+                flops=flops, bytes=bytes_, tuned=tuned, plan_s=plan)
+
+
+TRAFFIC_FILE = "exec_programs_sg100_plain_traffic.json"
+KERNEL_KEY = "exec_programs"
+
+
+def _pmc_rows(path, counter):
+    import csv
+    with open(path, newline="") as f:
+        return [r for r in csv.DictReader(f) if r["Counter_Name"] == counter]
+
+
+def summarize_pmc(paths, copy_mib, out_path):
+    """HBM traffic per exec_programs launch from rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE runs (KiB per dispatch), each scaled by what the same counter
+    reads for a plain device copy of copy_mib MiB (the largest dispatch of the
+    calibration run is the copy)."""
+    fetch = [float(r["Counter_Value"]) for r in _pmc_rows(paths[0], "FETCH_SIZE")
+             if KERNEL_KEY in r["Kernel_Name"]]
+    write = [float(r["Counter_Value"]) for r in _pmc_rows(paths[1], "WRITE_SIZE")
+             if KERNEL_KEY in r["Kernel_Name"]]
+    true_kib = copy_mib * 1024.0
+    cf = max(float(r["Counter_Value"]) for r in _pmc_rows(paths[2], "FETCH_SIZE"))
+    cw = max(float(r["Counter_Value"]) for r in _pmc_rows(paths[3], "WRITE_SIZE"))
+    sf, sw = true_kib / cf, true_kib / cw
+    n = len(fetch)
+    per_launch = 1024.0 * (sf * sum(fetch) + sw * sum(write)) / max(n, 1)
+    summary = {
+        "kernel": KERNEL_KEY, "workload": "plain form, spin glass n=100, Dmax=32, chi=256, FP64, "
+                                          "one solve (801 pinned networks)",
+        "dispatches": n,
+        "fetch_kib_per_dispatch": [round(x, 1) for x in fetch],
+        "write_kib_per_dispatch": [round(x, 1) for x in write],
+        "copy_mib": copy_mib, "copy_fetch_kib": cf, "copy_write_kib": cw,
+        "fetch_scale": sf, "write_scale": sw,
+        "traffic_bytes_per_launch": per_launch,
+    }
+    with open(out_path, "w") as f:
+        json.dump(summary, f, indent=1)
+    print(json.dumps(summary))
+
+
+def traffic_per_launch(path, form):
+    """HBM bytes per launch of the dominant kernel, from the committed counter
+    summary of the plain-form solve (None for another form or without one)."""
+    if form != "plain" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return float(json.load(f)["traffic_bytes_per_launch"])
 
 
 def main():
@@ -204,13 +268,25 @@ def main():
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the automatic launch shape instead of timing the candidates")
     ap.add_argument("--op-profile", default="",
                     help="write the device op profile of the last plain step (per opcode, per "
                          "SVD size bucket, per-network spread) to this JSON file")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc) and exit")
+    ap.add_argument("--summarize-pmc", nargs=4, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="only turn the counter_collection.csv files of four rocprofv3 --pmc "
+                         "runs (FETCH_SIZE, WRITE_SIZE of this bench; the same of "
+                         "--copy-kernel-mib) into the traffic summary JSON (--traffic) and exit")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", TRAFFIC_FILE),
+                    help="traffic summary read into roofline.traffic (written by --summarize-pmc)")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc, args.copy_kernel_mib or 1024, args.traffic)
+        return
     if args.copy_kernel_mib:
         import torch
         nbytes = args.copy_kernel_mib << 20
@@ -256,10 +332,10 @@ def main():
                 args.form = "plain"
         if refused and plain is None:
             plain = run_plain(catn, ctx, instances[0], args.steps, min(args.warmup, 1),
-                              args.op_profile)
+                              args.op_profile, not args.no_autotune)
     if args.form in ("both", "plain") and plain is None:
         plain = run_plain(catn, ctx, instances[0], args.plain_steps, min(args.warmup, 1),
-                          args.op_profile)
+                          args.op_profile, not args.no_autotune)
 
     # FP64 oracle lnZ of a few instances at the same (Dmax, chi)
     ncheck = 2
@@ -306,6 +382,12 @@ def main():
         pv = len(plain["times"]) / sum(plain["times"])
         out_cfg["plain_solves_per_s"] = pv
         out_cfg["plain_kernel"] = {"launches": plain["launches"], "seconds": plain["seconds"]}
+        out_cfg["plain_host_planning_s_per_step"] = plain["plan_s"]
+        t, l, _ = ctx.launch()
+        out_cfg["launch_shape"] = {"threads": t, "lds_doubles": l or "automatic"}
+        if plain["tuned"]:
+            out_cfg["autotune_s"] = {f"{r['threads']}x{r['lds_doubles']}": round(r["seconds"], 4)
+                                     for r in plain["tuned"]["table"]}
         if chosen is not None:
             out_cfg["chosen_over_plain"] = value / pv
     out_cfg["max_abs_dlnZ_per_N_vs_fp64_oracle"] = dlnz
@@ -333,7 +415,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic_per_launch(args.traffic, form),
             "kernel": "exec_programs (forward interpreter)",
             "algorithmic_flops_per_launch": flops_per_launch,
             "algorithmic_bytes_per_launch": bytes_per_launch,

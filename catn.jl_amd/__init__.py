@@ -23,7 +23,9 @@ the library is missing or no GPU is present, creating a Context raises.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
+import time
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -92,6 +94,8 @@ _SIGS = {
     "catn_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
     "catn_host_stats": (C.c_int, [_P, _DP, _DP]),
     "catn_set_threads": (C.c_int, [_P, C.c_int32]),
+    "catn_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "catn_get_launch": (C.c_int, [_P, _I32P, _I32P, _I32P]),
     "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "catn_node_cano_to": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "catn_node_left_canonical": (C.c_int, [_P, C.c_int32]),
@@ -175,6 +179,17 @@ class Context:
     def set_threads(self, threads: int):
         """Workgroup size of the interpreter kernel (64, 128 or 256)."""
         self._check(self.lib.catn_set_threads(self.h, int(threads)))
+
+    def set_launch(self, threads: int = 256, lds_doubles: int = 0):
+        """Launch shape of the interpreter: lanes per workgroup and LDS budget
+        per workgroup in doubles (0 = automatic by batch size)."""
+        self._check(self.lib.catn_set_launch(self.h, int(threads), int(lds_doubles)))
+
+    def launch(self):
+        """(threads, lds_doubles, lds_limit) of the current launch shape."""
+        t, l, lim = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.catn_get_launch(self.h, C.byref(t), C.byref(l), C.byref(lim)))
+        return t.value, l.value, lim.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -426,6 +441,51 @@ def gradient_batch(tns: Sequence[TensorNetwork], tape_cap: int = 0):
     for k, t in enumerate(tns):
         t.lnZ, t.psi = float(lz[k]), float(psi[k])
     return lz, err, psi
+
+
+# launch shapes tried by autotune(): (lanes per workgroup, LDS doubles per
+# workgroup); 0 = the automatic budget. 20352, 10176, 6784 and 5056 doubles
+# let 1, 2, 3 and 4 workgroups share one CU's 160 KB of LDS.
+AUTOTUNE_SHAPES = ((256, 0), (256, 5056), (256, 6784), (256, 10176), (256, 20352),
+                   (128, 5056), (128, 10176))
+
+
+def autotune(ctx: Context, build, run=None, shapes=AUTOTUNE_SHAPES, repeats=1, rtol=1e-10):
+    """Pick the interpreter's launch shape for a workload by timing it.
+
+    build() returns a fresh batch of networks for ctx (a run consumes its
+    batch); run(nets) executes one (default contraction_batch) and returns
+    lnZ per network. Each shape within the device's LDS limit is timed,
+    best of `repeats`; the fastest shape whose lnZ agree with the first
+    shape's to rtol (the LDS budget decides which ops run LDS-resident,
+    which may change rounding, never the algorithm) is set on ctx.
+    Returns {"threads", "lds_doubles", "seconds", "table": [...]}."""
+    run = run or (lambda nets: contraction_batch(nets)[0])
+    limit = ctx.launch()[2]
+    table, ref = [], None
+    for threads, lds in shapes:
+        if lds > limit:
+            continue
+        ctx.set_launch(threads, lds)
+        best, lz = math.inf, None
+        for _ in range(max(1, int(repeats))):
+            nets = build()
+            t0 = time.perf_counter()
+            lz = np.asarray(run(nets), dtype=float)
+            best = min(best, time.perf_counter() - t0)
+            for t in nets:
+                t.free()
+        if ref is None:
+            ref = lz
+        dev = float(np.max(np.abs(lz - ref) / np.maximum(np.abs(ref), 1e-300))) if lz.size else 0.0
+        table.append({"threads": threads, "lds_doubles": lds, "seconds": best, "lnz_rel_dev": dev})
+    ok = [r for r in table if r["lnz_rel_dev"] <= rtol]
+    if not ok:
+        raise CatnError("autotune: no launch shape ran")
+    win = min(ok, key=lambda r: r["seconds"])
+    ctx.set_launch(win["threads"], win["lds_doubles"])
+    return {"threads": win["threads"], "lds_doubles": win["lds_doubles"],
+            "seconds": win["seconds"], "table": table}
 
 
 def ising_gradient(tn: TensorNetwork, nedges: int, n: int):

@@ -60,6 +60,15 @@ class Executor {
   double plan_seconds = 0.0, run_seconds = 0.0;
   // workgroup size of the interpreter (64, 128 or 256 lanes)
   int threads = 256;
+  // LDS budget per workgroup in doubles; 0 = lds_doubles_for_batch(batch)
+  int lds_override = 0;
+  int lds_for(int nb) const {
+    if (lds_override > 0) return lds_override;
+    const int a = lds_doubles_for_batch(nb), lim = lds_limit();
+    return a < lim ? a : lim;
+  }
+  // the largest LDS budget (doubles) a launch of this executor may take
+  virtual int lds_limit() const { return kLdsMaxDoubles; }
 };
 
 Executor* make_executor(int device);   // provided by the executor TU
@@ -945,6 +954,26 @@ int catn_set_threads(catn_ctx* ctx, int32_t threads) {
   if (threads != 64 && threads != 128 && threads != 256)
     return catn::fail(-2, "catn_set_threads: threads must be 64, 128 or 256");
   ctx->ex->threads = threads;
+  return 0;
+}
+
+int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles) {
+  if (threads != 64 && threads != 128 && threads != 256)
+    return catn::fail(-2, "catn_set_launch: threads must be 64, 128 or 256");
+  const int lim = ctx->ex->lds_limit();
+  if (lds_doubles != 0 && (lds_doubles < catn::kLdsMinDoubles || lds_doubles > lim))
+    return catn::fail(-2, "catn_set_launch: lds_doubles must be 0 (automatic) or in [" +
+                              std::to_string(catn::kLdsMinDoubles) + ", " + std::to_string(lim) +
+                              "] on this device");
+  ctx->ex->threads = threads;
+  ctx->ex->lds_override = lds_doubles;
+  return 0;
+}
+
+int catn_get_launch(catn_ctx* ctx, int32_t* threads, int32_t* lds_doubles, int32_t* lds_limit) {
+  if (threads) *threads = ctx->ex->threads;
+  if (lds_doubles) *lds_doubles = ctx->ex->lds_override;
+  if (lds_limit) *lds_limit = ctx->ex->lds_limit();
   return 0;
 }
 

@@ -17,6 +17,8 @@
 namespace catn {
 
 constexpr int kThreads = 256;
+// static LDS of both kernels: red[16] doubles + flag, padded
+constexpr int kStaticLdsBytes = 256;
 
 __global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, const Op* ops,
                                                              const int* ranges, int lds_cap) {
@@ -68,16 +70,25 @@ class HipExecutor : public Executor {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
       tick_hz_ = 1000.0 * khz;
     (void)hipGetLastError();   // an unsupported attribute must not surface at the first launch
+    // dynamic LDS a workgroup may take: the device limit less the kernels'
+    // static LDS (reduction slots and flag), at most kLdsMaxDoubles
+    int max_shared = 0;
+    hipcheck(hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, device),
+             "hipDeviceGetAttribute(MaxSharedMemoryPerBlock)");
+    lds_limit_ = (max_shared - kStaticLdsBytes) / (int)sizeof(double);
+    if (lds_limit_ > kLdsMaxDoubles) lds_limit_ = kLdsMaxDoubles;
+    if (lds_limit_ < kLdsMinDoubles)
+      throw Error(-100, "device LDS per workgroup is below the interpreter's minimum");
     hipcheck(hipStreamCreate(&stream_), "hipStreamCreate");
     hipcheck(hipEventCreate(&ev0_), "hipEventCreate");
     hipcheck(hipEventCreate(&ev1_), "hipEventCreate");
     hipcheck(hipFuncSetAttribute((const void*)exec_programs,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(sizeof(double) * kLdsMaxDoubles)),
+                                 (int)(sizeof(double) * lds_limit_)),
              "hipFuncSetAttribute");
     hipcheck(hipFuncSetAttribute((const void*)exec_backward,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(sizeof(double) * kLdsMaxDoubles)),
+                                 (int)(sizeof(double) * lds_limit_)),
              "hipFuncSetAttribute");
   }
   ~HipExecutor() override {
@@ -92,6 +103,7 @@ class HipExecutor : public Executor {
   const char* name() const override { return name_.c_str(); }
   bool cut_reverse_enabled() const override { return false; }
   double tick_hz() const override { return tick_hz_; }
+  int lds_limit() const override { return lds_limit_; }
   void* dmalloc(size_t bytes) override {
     void* p = nullptr;
     hipcheck(hipMalloc(&p, bytes < 8 ? 8 : bytes), "hipMalloc");
@@ -123,6 +135,8 @@ class HipExecutor : public Executor {
       ranges[2 * b + 1] = (int)all.size();
     }
     if (all.empty()) return;
+    const int lds_cap = lds_for(nb);
+    if (lds_cap > lds_limit_) throw Error(-2, "launch LDS budget above the device limit");
     reserve(all.size(), nb);
     // stage every instance state into one contiguous array: one copy each way
     std::vector<InstState> packed(nb);
@@ -131,7 +145,6 @@ class HipExecutor : public Executor {
     hipcheck(hipMemcpyAsync(d_ranges_, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice, stream_), "ranges");
     hipcheck(hipMemcpyAsync(d_states_, packed.data(), sizeof(InstState) * nb, hipMemcpyHostToDevice, stream_), "states");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
-    const int lds_cap = lds_doubles_for_batch(nb);
     hipcheck(hipGetLastError(), "before launch exec_programs");
     hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
                        d_states_, d_ops_, d_ranges_, lds_cap);
@@ -147,7 +160,7 @@ class HipExecutor : public Executor {
   void run_backward(const std::vector<InstState*>& states, int lds_cap) override {
     const int nb = (int)states.size();
     if (nb == 0) return;
-    if (lds_cap <= 0 || lds_cap > kLdsMaxDoubles) lds_cap = kLdsMaxDoubles;
+    if (lds_cap <= 0 || lds_cap > lds_limit_) lds_cap = lds_limit_;
     reserve(0, nb);
     std::vector<InstState> packed(nb);
     for (int b = 0; b < nb; ++b) packed[b] = *states[b];
@@ -200,6 +213,7 @@ class HipExecutor : public Executor {
   InstState* d_states_ = nullptr;
   size_t ops_cap_ = 0, st_cap_ = 0;
   double tick_hz_ = 1e8;
+  int lds_limit_ = 0;
 };
 
 Executor* make_executor(int device) { return new HipExecutor(device); }

@@ -285,12 +285,14 @@ CATN_DN void wg_apply_q(const WG& g, const double* A, int m, int n, int64_t lda,
 // the kernel's dynamic shared array (so pointers derived from it compile to
 // ds_* instructions after inlining); the CPU emulator uses a static buffer.
 // ---------------------------------------------------------------------------
-constexpr int kLdsMaxDoubles = 10240;   // 80 KB upper bound (launch chooses <=)
+constexpr int kLdsMaxDoubles = 20480;   // 160 KB: the most a launch may choose
+constexpr int kLdsMinDoubles = 2048;    // the least a launch shape may set
 
-// LDS budget per workgroup (doubles). MI355X has 160 KB of LDS per CU: up to
-// 512 workgroups get ~80 KB each (2 per CU on 256 CUs); larger batches get
-// ~40 KB so that 4 workgroups co-reside per CU. The static part (reduction
-// slots, flags) is kept out of the budget.
+// Default LDS budget per workgroup (doubles). MI355X has 160 KB of LDS per CU:
+// up to 512 workgroups get ~80 KB each (2 per CU on 256 CUs); larger batches
+// get ~40 KB so that 4 workgroups co-reside per CU. The static part (reduction
+// slots, flags) is kept out of the budget. A launch shape set on the context
+// (catn_set_launch, picked by the autotuner) overrides it.
 inline int lds_doubles_for_batch(int nb) { return nb <= 512 ? 10176 : 5056; }
 
 #if defined(__HIP_DEVICE_COMPILE__)

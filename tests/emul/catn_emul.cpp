@@ -41,7 +41,7 @@ class CpuExecutor : public Executor {
     g.nthr = 1;
     g.red = red;
     g.flag = &flag;
-    g.lds_cap = lds_doubles_for_batch((int)states.size());
+    g.lds_cap = lds_for((int)states.size());
     if (getenv("CATN_EMU_LDS")) g.lds_cap = atoi(getenv("CATN_EMU_LDS"));   // a device batch's budget
     for (size_t b = 0; b < states.size(); ++b) {
       const std::vector<Op>& p = *programs[b];

@@ -312,7 +312,7 @@ class MtExecutor : public Executor {
 
   void run(const std::vector<InstState*>& states,
            const std::vector<const std::vector<Op>*>& programs) override {
-    int lds = lds_doubles_for_batch((int)states.size());
+    int lds = lds_for((int)states.size());
     if (getenv("CATN_EMU_LDS")) lds = atoi(getenv("CATN_EMU_LDS"));   // a device batch's budget
     for (size_t b = 0; b < states.size(); ++b) {
       const std::vector<Op>& p = *programs[b];
