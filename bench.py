@@ -132,6 +132,66 @@ def run_chosen(catn, ctx, instances, steps, warmup):
                 bwd_launches=bl, bwd_seconds=bs, fwd_flops=fwd_flops, fwd_bytes=fwd_bytes)
 
 
+CUT_FREE = dict(n=80, seed=0, Dmax=-1, chi=64)
+
+
+def run_cut_free(catn, ctx, steps, warmup, batch):
+    """Both forms on a spin glass whose contraction never reaches a Dmax cut
+    (n=80 random 3-regular, seed 0, Dmax=-1, chi=64): the device runs the
+    chosen form's reverse sweep only on such networks. Chosen: `batch` copies
+    of the instance by recorded forward + reverse sweep per step. Plain: the
+    1 + 2n + 4|E| = 641 pinned contractions of one solve per step."""
+    from oracle import configs as Cf
+    from oracle import catn_oracle as O
+    n, edges, w, h = Cf.spin_glass_3reg(CUT_FREE["n"], seed=CUT_FREE["seed"])
+    kw = dict(Dmax=CUT_FREE["Dmax"], chi=CUT_FREE["chi"], arena_elems=3 << 20,
+              scratch_elems=1 << 21)
+    ct, pt = [], []
+    g_adj = g_pin = lz_c = None
+    for step in range(warmup + steps):
+        nets = [catn.ising_network(n, edges, w, h, BETA, ctx=ctx, **kw) for _ in range(batch)]
+        _prefetch(nets)
+        t0 = time.perf_counter()
+        lz, _, _ = catn.gradient_batch(nets)
+        grads = [catn.ising_gradient(t, len(edges), n)[:2] for t in nets]
+        t1 = time.perf_counter()
+        del nets
+        pn = [catn.ising_network(n, edges, w, h, BETA, ctx=ctx, **kw)]
+        for k in range(1, n + 1):
+            for v in (1, 2):
+                pn.append(catn.ising_network(n, edges, w, h, BETA, pos1=k, val1=v, ctx=ctx, **kw))
+        for (a, b) in edges:
+            for v1 in (1, 2):
+                for v2 in (1, 2):
+                    pn.append(catn.ising_network(n, edges, w, h, BETA, pos1=a, val1=v1, pos2=b,
+                                                 val2=v2, ctx=ctx, **kw))
+        _prefetch(pn)
+        t2 = time.perf_counter()
+        plz, _, _ = catn.contraction_batch(pn)
+        gp = gradient_from_lnz(plz, n, edges)
+        t3 = time.perf_counter()
+        del pn
+        log(f"cut-free step {step}: chosen {t1 - t0:.3f} s ({batch} solves), "
+            f"plain {t3 - t2:.3f} s (1 solve)")
+        if step >= warmup:
+            ct.append(t1 - t0)
+            pt.append(t3 - t2)
+            g_adj, g_pin, lz_c = grads[0], gp, float(lz[0])
+    ref = O.contraction(O.ising_network(n, edges, w, h, BETA, Dmax=CUT_FREE["Dmax"],
+                                        chi=CUT_FREE["chi"]))[0]
+    cv = batch * len(ct) / sum(ct)
+    pv = len(pt) / sum(pt)
+    return {
+        "workload": f"spin glass n={n} random 3-regular (seed {CUT_FREE['seed']}), w=+-1, "
+                    f"h~N(0,0.1^2), beta=1.0, Dmax=-1 (no cut), chi={CUT_FREE['chi']}, FP64",
+        "chosen_solves_per_s": cv, "chosen_batch": batch,
+        "plain_solves_per_s": pv, "chosen_over_plain": cv / pv,
+        "max_abs_dlnZ_per_N_vs_fp64_oracle": abs(lz_c - ref) / n,
+        "adjoint_vs_pinning_max_abs_dw": float(np.max(np.abs(g_adj[0] - g_pin[0]))),
+        "adjoint_vs_pinning_max_abs_dh": float(np.max(np.abs(g_adj[1] - g_pin[1]))),
+    }
+
+
 def op_profile(nets):
     """Device op profile summed over `nets`: seconds and counts per opcode and
     per SVD size bucket, sub-step seconds, and the spread of per-network time."""
@@ -268,6 +328,8 @@ def main():
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cut-free-batch", type=int, default=128,
+                    help="instances per chosen-form step of the cut-free comparison (0: skip it)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the automatic launch shape instead of timing the candidates")
     ap.add_argument("--op-profile", default="",
@@ -336,6 +398,9 @@ def main():
     if args.form in ("both", "plain") and plain is None:
         plain = run_plain(catn, ctx, instances[0], args.plain_steps, min(args.warmup, 1),
                           args.op_profile, not args.no_autotune)
+
+    if refused and args.form == "both" and args.cut_free_batch > 0:
+        out_cfg["cut_free_comparison"] = run_cut_free(catn, ctx, 2, 1, args.cut_free_batch)
 
     # FP64 oracle lnZ of a few instances at the same (Dmax, chi)
     ncheck = 2

@@ -22,6 +22,8 @@
 
 #include "../../include/catn_hip.h"
 #include "ops.h"
+#include <unordered_map>
+
 #include "planner.h"
 
 namespace catn {
@@ -288,14 +290,19 @@ inline double wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-inline void run_nets(Executor* ex, const std::vector<catn_net*>& nets) {
+// share[k] >= 0: network k runs the program planned for network share[k]
+// (equal planner states give equal programs), so it is uploaded once
+inline void run_nets(Executor* ex, const std::vector<catn_net*>& nets,
+                     const std::vector<int>* share = nullptr) {
   const double t0 = wall();
   std::vector<InstState*> st;
   std::vector<const std::vector<Op>*> pr;
-  for (catn_net* n : nets) {
+  for (size_t k = 0; k < nets.size(); ++k) {
+    catn_net* n = nets[k];
     n->ensure_device();
     st.push_back(&n->hs);
-    pr.push_back(&n->pl.prog);
+    const int l = share ? (*share)[k] : -1;
+    pr.push_back(l >= 0 ? &nets[l]->pl.prog : &n->pl.prog);
   }
   ex->run(st, pr);
   for (catn_net* n : nets) {
@@ -547,8 +554,32 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     std::vector<catn_net*> still;
     std::vector<int> cut(active.size());
     const double t0 = wall();
-    parallel_for((int)active.size(), [&](int k) {
-      cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
+    // Plan each group of networks with equal planner states once (pinned
+    // networks of one ising instance differ only in site values): the first
+    // member plans, the others take a copy of its planner after the step.
+    const int na = (int)active.size();
+    std::vector<uint64_t> fp(na);
+    std::vector<int> share(na, -1);
+    parallel_for(na, [&](int k) { fp[k] = active[k]->pl.fingerprint(); });
+    {
+      std::unordered_map<uint64_t, int> first;
+      for (int k = 0; k < na; ++k) {
+        auto it = first.find(fp[k]);
+        if (it == first.end()) first.emplace(fp[k], k);
+        else share[k] = it->second;
+      }
+    }
+    parallel_for(na, [&](int k) {   // equal fingerprints must be equal states
+      if (share[k] >= 0 && !active[k]->pl.same_state(active[share[k]]->pl)) share[k] = -1;
+    });
+    parallel_for(na, [&](int k) {
+      if (share[k] < 0) cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
+    });
+    parallel_for(na, [&](int k) {
+      if (share[k] >= 0) {
+        active[k]->pl = active[share[k]]->pl;
+        cut[k] = cut[share[k]];
+      }
     });
     ex->plan_seconds += wall() - t0;
     if (!ex->cut_reverse_enabled())
@@ -557,7 +588,7 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
           throw Error(-15, "gradient across a Dmax cut (cut_bondim_opt!) is disabled on this "
                            "device; use the pinning route (magnetization / correlation) or a "
                            "Dmax that the network never exceeds");
-    run_nets(ex, active);
+    run_nets(ex, active, &share);
     for (size_t k = 0; k < active.size(); ++k) {
       catn_net* n = active[k];
       if (cut[k]) {

@@ -9,6 +9,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "engine.h"
@@ -127,9 +128,18 @@ class HipExecutor : public Executor {
            const std::vector<const std::vector<Op>*>& programs) override {
     const int nb = (int)states.size();
     if (nb == 0) return;
+    // concatenated programs; a program shared by several networks is uploaded once
     std::vector<Op> all;
     std::vector<int> ranges(2 * nb);
+    std::unordered_map<const std::vector<Op>*, int> at;
     for (int b = 0; b < nb; ++b) {
+      auto it = at.find(programs[b]);
+      if (it != at.end()) {
+        ranges[2 * b] = ranges[2 * it->second];
+        ranges[2 * b + 1] = ranges[2 * it->second + 1];
+        continue;
+      }
+      at.emplace(programs[b], b);
       ranges[2 * b] = (int)all.size();
       all.insert(all.end(), programs[b]->begin(), programs[b]->end());
       ranges[2 * b + 1] = (int)all.size();

@@ -51,6 +51,32 @@ struct PNode {
   std::vector<int> phys;   // physical dim per MPS position
   int cano = 0;            // 1-based canonical centre (reference convention)
   int order() const { return (int)slot.size(); }
+  bool operator==(const PNode& o) const {
+    return cano == o.cano && slot == o.slot && nb == o.nb && phys == o.phys;
+  }
+};
+
+inline bool same_params(const Params& a, const Params& b) {
+  return a.Dmax == b.Dmax && a.chi == b.chi && a.cutoff == b.cutoff &&
+         a.norm_method == b.norm_method && a.select == b.select && a.reverse == b.reverse &&
+         a.svdopt == b.svdopt && a.swapopt == b.swapopt && a.compress == b.compress &&
+         a.cut_bond == b.cut_bond && a.seed == b.seed && a.rsvd_both == b.rsvd_both &&
+         a.rsvd_any == b.rsvd_any;
+}
+
+// 64-bit mixing of a sequence of integers (splitmix64 finaliser per word)
+struct Mix {
+  uint64_t h = 0x9E3779B97F4A7C15ull;
+  void add(uint64_t v) {
+    uint64_t z = h ^ (v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2));
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    h = z ^ (z >> 31);
+  }
+  void add(const std::vector<int>& v) {
+    add(v.size());
+    for (int x : v) add((uint64_t)(uint32_t)x);
+  }
 };
 
 struct PlanError : std::runtime_error {
@@ -389,6 +415,37 @@ class Planner {
   void start() {
     if ((p.select == 0 || p.select == 1) && edge_count.empty()) select_edge_init();
     state_ = ST_EDGE;
+  }
+
+  // Planning is a deterministic function of the planner state (and of the
+  // cut results applied between segments), so networks whose states are equal
+  // get the same next segment. fingerprint() buckets states; same_state()
+  // confirms equality field by field before a plan is shared.
+  uint64_t fingerprint() const {
+    Mix m;
+    m.add((uint64_t)p.Dmax); m.add((uint64_t)p.chi); m.add((uint64_t)p.select);
+    m.add((uint64_t)p.norm_method); m.add((uint64_t)state_); m.add((uint64_t)cur_i_);
+    m.add((uint64_t)cur_j_); m.add((uint64_t)dup_pos_); m.add(dups_);
+    m.add((uint64_t)nresults); m.add((uint64_t)(int64_t)maxdim_intermediate);
+    m.add(nodes.size());
+    for (const auto& kv : nodes) {
+      m.add((uint64_t)kv.first); m.add((uint64_t)kv.second.cano);
+      m.add(kv.second.slot); m.add(kv.second.nb); m.add(kv.second.phys);
+    }
+    m.add(edge_count.size());
+    for (const auto& kv : edge_count) {
+      m.add((uint64_t)kv.first); m.add(kv.second.size());
+      for (const auto& e : kv.second) { m.add((uint64_t)e.first); m.add((uint64_t)e.second); }
+    }
+    return m.h;
+  }
+  bool same_state(const Planner& o) const {
+    return same_params(p, o.p) && state_ == o.state_ && cur_i_ == o.cur_i_ &&
+           cur_j_ == o.cur_j_ && dups_ == o.dups_ && dup_pos_ == o.dup_pos_ &&
+           cut_pending_ == o.cut_pending_ && nresults == o.nresults &&
+           maxdim_intermediate == o.maxdim_intermediate && acc_err == o.acc_err &&
+           std::equal(pending_cut_, pending_cut_ + 5, o.pending_cut_) && nodes == o.nodes &&
+           edge_count == o.edge_count && edge_log == o.edge_log && prog.size() == o.prog.size();
   }
 
   StepResult step() {
