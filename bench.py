@@ -132,7 +132,7 @@ def run_chosen(catn, ctx, instances, steps, warmup):
                 bwd_launches=bl, bwd_seconds=bs, fwd_flops=fwd_flops, fwd_bytes=fwd_bytes)
 
 
-CUT_FREE = dict(n=80, seed=0, Dmax=-1, chi=64)
+CUT_FREE = dict(n=80, seed=0, Dmax=-1, chi=128)
 
 
 def run_cut_free(catn, ctx, steps, warmup, batch):
