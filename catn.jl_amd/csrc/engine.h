@@ -634,10 +634,29 @@ CATN_D void wg_panel_v(const WG& g, const double* A, int rows, int jb, int64_t l
   g.sync();
 }
 
+// wg_geqr2 of an operand in global memory. When it fits the LDS budget it is
+// factorised in LDS (the 3 barriers per column then wait on LDS, not on
+// global memory) and copied back. The LDS must be free: every caller of
+// wg_geqrf keeps its operands in scratch.
+CATN_DN void wg_geqr2_staged(const WG& g, double* A, int m, int n, int64_t lda, double* tau) {
+  if ((int64_t)m * n > g.lds_cap) {
+    wg_geqr2(g, A, m, n, lda, tau);
+    return;
+  }
+  double* P = lds_base();
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr)
+    P[idx] = A[(idx % m) + lda * (idx / m)];
+  g.sync();
+  wg_geqr2(g, P, m, n, m, tau);
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr)
+    A[(idx % m) + lda * (idx / m)] = P[idx];
+  g.sync();
+}
+
 CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t lda, double* tau,
                       double* Tall) {
   if (n <= kQB) {
-    wg_geqr2(g, A, m, n, lda, tau);
+    wg_geqr2_staged(g, A, m, n, lda, tau);
     return false;   // unblocked: no T factors
   }
   double* V = sc.alloc((int64_t)m * kQB);
@@ -650,7 +669,7 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     const int rows = m - p;
     double* Ap = A + p + lda * (int64_t)p;
     int64_t t0 = now_ticks();
-    wg_geqr2(g, Ap, rows, jb, lda, tau + p);
+    wg_geqr2_staged(g, Ap, rows, jb, lda, tau + p);
     sub_tick(g, 8, t0);
     t0 = now_ticks();
     wg_panel_v(g, Ap, rows, jb, lda, V);
@@ -713,6 +732,9 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
 // wavefront that walks its rows contiguously, which global-memory operands
 // need: coalesced loads and full-width reductions, each wavefront taking its
 // share of the round's pairs in turn.
+// rows per lane held in registers by the Jacobi pair step (m, n <= kJacReg * L)
+constexpr int kJacReg = 8;
+
 CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double* V,
                        int64_t ldv, int maxsweeps, double thr_in = -1.0, int lanes_per_pair = 0) {
   for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
@@ -751,7 +773,46 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
         if (q >= n) continue;
         double* ap = A + (int64_t)p * lda;
         double* aq = A + (int64_t)q * lda;
+        double* vp = V + (int64_t)p * ldv;
+        double* vq = V + (int64_t)q * ldv;
         double al = 0.0, be = 0.0, ga = 0.0;
+        if (m <= kJacReg * L && n <= kJacReg * L) {
+          // the pair's columns (and V's) stay in registers: one round of
+          // loads in flight, rotation from registers, one round of stores
+          // (named scalars: indexed arrays were placed on the stack)
+#define CATN_JR8(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+#define CATN_JLOAD(u)                                  \
+  const int i##u = lane + u * L;                       \
+  const double xa##u = i##u < m ? ap[i##u] : 0.0;      \
+  const double ya##u = i##u < m ? aq[i##u] : 0.0;      \
+  const double xv##u = i##u < n ? vp[i##u] : 0.0;      \
+  const double yv##u = i##u < n ? vq[i##u] : 0.0;
+#define CATN_JDOT(u) al += xa##u * xa##u; be += ya##u * ya##u; ga += xa##u * ya##u;
+#define CATN_JROT(u)                                   \
+  if (i##u < m) {                                      \
+    ap[i##u] = c * xa##u - s * ya##u;                  \
+    aq[i##u] = s * xa##u + c * ya##u;                  \
+  }                                                    \
+  if (i##u < n) {                                      \
+    vp[i##u] = c * xv##u - s * yv##u;                  \
+    vq[i##u] = s * xv##u + c * yv##u;                  \
+  }
+          CATN_JR8(CATN_JLOAD)
+          CATN_JR8(CATN_JDOT)
+          al = grp_sum(al, L); be = grp_sum(be, L); ga = grp_sum(ga, L);
+          if (al <= thr || be <= thr) continue;
+          if (fabs(ga) <= tol * sqrt(al) * sqrt(be)) continue;
+          rotated = 1;
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+          CATN_JR8(CATN_JROT)
+#undef CATN_JROT
+#undef CATN_JDOT
+#undef CATN_JLOAD
+#undef CATN_JR8
+          continue;
+        }
         for (int i = lane; i < m; i += L) {
           const double x = ap[i], y = aq[i];
           al += x * x; be += y * y; ga += x * y;
@@ -768,8 +829,6 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
           ap[i] = c * x - s * y;
           aq[i] = s * x + c * y;
         }
-        double* vp = V + (int64_t)p * ldv;
-        double* vq = V + (int64_t)q * ldv;
         for (int i = lane; i < n; i += L) {
           const double x = vp[i], y = vq[i];
           vp[i] = c * x - s * y;
@@ -783,8 +842,11 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
   return -1;
 }
 
-// Jacobi on an operand in global memory: one wavefront per column pair.
-constexpr int kJacobiGlobalLanes = 64;
+// Jacobi on an operand in global memory: a 16-lane group per column pair, so
+// a wavefront rotates four pairs at once (each lane holds up to kJacReg rows
+// of both columns in registers for n <= 128); 16 consecutive doubles per
+// group keep the loads coalesced.
+constexpr int kJacobiGlobalLanes = 16;
 
 CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double* V,
                       int64_t ldv, int maxsweeps) {
@@ -1467,13 +1529,16 @@ struct Exec {
   CATN_D float* arena_f() const { return (float*)st->arena; }
   // Singular values below noise_floor() * ||A||_F count as zero when the
   // tensors are stored in FP32: the storage rounds every element (relative
-  // 2^-24) and the 3-pass split-BF16 theta product carries relative errors
-  // near 2^-16, so smaller singular values are rounding noise. Keeping them
+  // 2^-24), so much smaller singular values are rounding noise. Keeping them
   // would grow bond dimensions in exact runs and, through the 1/sigma terms
   // of the SVD adjoint, turn the FP32 orthogonality error of U into O(1)
-  // gradient errors. With FP64 storage the reference's cutoff alone decides.
+  // gradient errors. 6-pass products (2): 2^-20, 16x the storage resolution.
+  // 3-pass products (1): 2^-18; a floor at their own product error (2^-16 and
+  // above) drops real spectrum: 2^-13 cost 2e-4 in lnZ/N on a 14 x 14 lattice
+  // at beta 1.1, 2^-18 keeps it within 3.4e-6 on 8..16-wide lattices. With
+  // FP64 storage the reference's cutoff alone decides.
   CATN_D double noise_floor() const {
-    return st->f32 == 2 ? 0x1p-20 : st->f32 == 1 ? 0x1p-13 : 0.0;
+    return st->f32 == 2 ? 0x1p-20 : st->f32 == 1 ? 0x1p-18 : 0.0;
   }
   // FP64 copy of the tensor version at arena offset `off` (n elements) into
   // scratch; every lane calls it. False when the scratch is exhausted.
