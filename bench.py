@@ -132,15 +132,17 @@ def run_chosen(catn, ctx, instances, steps, warmup):
                 bwd_launches=bl, bwd_seconds=bs, fwd_flops=fwd_flops, fwd_bytes=fwd_bytes)
 
 
-CUT_FREE = dict(n=80, seed=0, Dmax=-1, chi=128)
+CUT_FREE = dict(n=90, seed=1, Dmax=-1, chi=256)
 
 
 def run_cut_free(catn, ctx, steps, warmup, batch):
     """Both forms on a spin glass whose contraction never reaches a Dmax cut
-    (n=80 random 3-regular, seed 0, Dmax=-1, chi=64): the device runs the
-    chosen form's reverse sweep only on such networks. Chosen: `batch` copies
-    of the instance by recorded forward + reverse sweep per step. Plain: the
-    1 + 2n + 4|E| = 641 pinned contractions of one solve per step."""
+    and that chi=256 does not truncate (n=90 random 3-regular, seed 1,
+    Dmax=-1; lnZ equal at chi 256 and 1024), so the adjoint and the pinning
+    differences compute the same gradient: the device runs the chosen form's
+    reverse sweep only on networks without a cut. Chosen: `batch` copies of
+    the instance by recorded forward + reverse sweep per step. Plain: the
+    1 + 2n + 4|E| = 721 pinned contractions of one solve per step."""
     from oracle import configs as Cf
     from oracle import catn_oracle as O
     n, edges, w, h = Cf.spin_glass_3reg(CUT_FREE["n"], seed=CUT_FREE["seed"])
@@ -235,9 +237,11 @@ def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
             _prefetch(nets)
             return nets
         tuned = catn.autotune(ctx, build)
-        log("autotune: " + ", ".join(f"({r['threads']},{r['lds_doubles']}) {r['seconds']:.3f} s"
+        log("autotune: " + ", ".join(f"({r['threads']},{r['lds_doubles']},{r['panel']},"
+                                     f"{r['jacobi_lanes']}) {r['seconds']:.3f} s"
                                      for r in tuned["table"])
-            + f" -> ({tuned['threads']},{tuned['lds_doubles']})")
+            + f" -> ({tuned['threads']},{tuned['lds_doubles']},{tuned['panel']},"
+              f"{tuned['jacobi_lanes']})")
     plan0 = 0.0
     for step in range(warmup + steps):
         nets = pinned_batch(catn, ctx, n, edges, w, h)
@@ -448,11 +452,13 @@ def main():
         out_cfg["plain_solves_per_s"] = pv
         out_cfg["plain_kernel"] = {"launches": plain["launches"], "seconds": plain["seconds"]}
         out_cfg["plain_host_planning_s_per_step"] = plain["plan_s"]
-        t, l, _ = ctx.launch()
-        out_cfg["launch_shape"] = {"threads": t, "lds_doubles": l or "automatic"}
+        t, l, _, qb, jl = ctx.launch()
+        out_cfg["launch_shape"] = {"threads": t, "lds_doubles": l or "automatic", "panel": qb,
+                                   "jacobi_lanes": jl}
         if plain["tuned"]:
-            out_cfg["autotune_s"] = {f"{r['threads']}x{r['lds_doubles']}": round(r["seconds"], 4)
-                                     for r in plain["tuned"]["table"]}
+            out_cfg["autotune_s"] = {
+                f"{r['threads']}x{r['lds_doubles']}x{r['panel']}x{r['jacobi_lanes']}":
+                    round(r["seconds"], 4) for r in plain["tuned"]["table"]}
         if chosen is not None:
             out_cfg["chosen_over_plain"] = value / pv
     out_cfg["max_abs_dlnZ_per_N_vs_fp64_oracle"] = dlnz

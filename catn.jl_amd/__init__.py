@@ -94,8 +94,8 @@ _SIGS = {
     "catn_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
     "catn_host_stats": (C.c_int, [_P, _DP, _DP]),
     "catn_set_threads": (C.c_int, [_P, C.c_int32]),
-    "catn_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
-    "catn_get_launch": (C.c_int, [_P, _I32P, _I32P, _I32P]),
+    "catn_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "catn_get_launch": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P]),
     "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "catn_node_cano_to": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "catn_node_left_canonical": (C.c_int, [_P, C.c_int32]),
@@ -180,16 +180,20 @@ class Context:
         """Workgroup size of the interpreter kernel (64, 128 or 256)."""
         self._check(self.lib.catn_set_threads(self.h, int(threads)))
 
-    def set_launch(self, threads: int = 256, lds_doubles: int = 0):
-        """Launch shape of the interpreter: lanes per workgroup and LDS budget
-        per workgroup in doubles (0 = automatic by batch size)."""
-        self._check(self.lib.catn_set_launch(self.h, int(threads), int(lds_doubles)))
+    def set_launch(self, threads: int = 256, lds_doubles: int = 0, panel: int = 0,
+                   jacobi_lanes: int = 0):
+        """Launch shape of the interpreter: lanes per workgroup, LDS budget per
+        workgroup in doubles (0 = automatic by batch size), Householder panel
+        width (0 = 16) and global-Jacobi lanes per column pair (0 = 16)."""
+        self._check(self.lib.catn_set_launch(self.h, int(threads), int(lds_doubles), int(panel),
+                                             int(jacobi_lanes)))
 
     def launch(self):
-        """(threads, lds_doubles, lds_limit) of the current launch shape."""
-        t, l, lim = C.c_int32(), C.c_int32(), C.c_int32()
-        self._check(self.lib.catn_get_launch(self.h, C.byref(t), C.byref(l), C.byref(lim)))
-        return t.value, l.value, lim.value
+        """(threads, lds_doubles, lds_limit, panel, jacobi_lanes) of the current
+        launch shape."""
+        v = [C.c_int32() for _ in range(5)]
+        self._check(self.lib.catn_get_launch(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def close(self):
         if getattr(self, "h", None):
@@ -448,25 +452,30 @@ def gradient_batch(tns: Sequence[TensorNetwork], tape_cap: int = 0):
 # let 1, 2, 3 and 4 workgroups share one CU's 160 KB of LDS.
 AUTOTUNE_SHAPES = ((256, 0), (256, 5056), (256, 6784), (256, 10176), (256, 20352),
                    (128, 5056), (128, 10176))
+# tile parameters tried at the fastest shape: (Householder panel width,
+# global-Jacobi lanes per column pair)
+AUTOTUNE_TILES = ((16, 16), (32, 16), (8, 16), (16, 32), (16, 8), (16, 64))
 
 
-def autotune(ctx: Context, build, run=None, shapes=AUTOTUNE_SHAPES, repeats=1, rtol=1e-10):
+def autotune(ctx: Context, build, run=None, shapes=AUTOTUNE_SHAPES, tiles=AUTOTUNE_TILES,
+             repeats=1, rtol=1e-10):
     """Pick the interpreter's launch shape for a workload by timing it.
 
     build() returns a fresh batch of networks for ctx (a run consumes its
     batch); run(nets) executes one (default contraction_batch) and returns
-    lnZ per network. Each shape within the device's LDS limit is timed,
-    best of `repeats`; the fastest shape whose lnZ agree with the first
-    shape's to rtol (the LDS budget decides which ops run LDS-resident,
-    which may change rounding, never the algorithm) is set on ctx.
-    Returns {"threads", "lds_doubles", "seconds", "table": [...]}."""
+    lnZ per network. Stage 1 times each (lanes, LDS budget) shape within the
+    device's LDS limit at the default tiles; stage 2 times each (panel width,
+    Jacobi lanes) tile pair at the fastest shape. Times are best of
+    `repeats`. A candidate counts only if its lnZ agree with the first
+    candidate's to rtol (shape and tiles change rounding, never the
+    algorithm). The winner is set on ctx. Returns {"threads",
+    "lds_doubles", "panel", "jacobi_lanes", "seconds", "table": [...]}."""
     run = run or (lambda nets: contraction_batch(nets)[0])
     limit = ctx.launch()[2]
-    table, ref = [], None
-    for threads, lds in shapes:
-        if lds > limit:
-            continue
-        ctx.set_launch(threads, lds)
+    table, ref = [], [None]
+
+    def timed(threads, lds, panel, jl):
+        ctx.set_launch(threads, lds, panel, jl)
         best, lz = math.inf, None
         for _ in range(max(1, int(repeats))):
             nets = build()
@@ -475,17 +484,29 @@ def autotune(ctx: Context, build, run=None, shapes=AUTOTUNE_SHAPES, repeats=1, r
             best = min(best, time.perf_counter() - t0)
             for t in nets:
                 t.free()
-        if ref is None:
-            ref = lz
-        dev = float(np.max(np.abs(lz - ref) / np.maximum(np.abs(ref), 1e-300))) if lz.size else 0.0
-        table.append({"threads": threads, "lds_doubles": lds, "seconds": best, "lnz_rel_dev": dev})
+        if ref[0] is None:
+            ref[0] = lz
+        dev = (float(np.max(np.abs(lz - ref[0]) / np.maximum(np.abs(ref[0]), 1e-300)))
+               if lz.size else 0.0)
+        row = {"threads": threads, "lds_doubles": lds, "panel": panel, "jacobi_lanes": jl,
+               "seconds": best, "lnz_rel_dev": dev}
+        table.append(row)
+        return row
+
+    for threads, lds in shapes:
+        if lds <= limit:
+            timed(threads, lds, 16, 16)
     ok = [r for r in table if r["lnz_rel_dev"] <= rtol]
     if not ok:
         raise CatnError("autotune: no launch shape ran")
-    win = min(ok, key=lambda r: r["seconds"])
-    ctx.set_launch(win["threads"], win["lds_doubles"])
-    return {"threads": win["threads"], "lds_doubles": win["lds_doubles"],
-            "seconds": win["seconds"], "table": table}
+    s1 = min(ok, key=lambda r: r["seconds"])
+    for panel, jl in tiles:
+        if (panel, jl) != (16, 16) and jl <= s1["threads"]:
+            timed(s1["threads"], s1["lds_doubles"], panel, jl)
+    win = min((r for r in table if r["lnz_rel_dev"] <= rtol), key=lambda r: r["seconds"])
+    ctx.set_launch(win["threads"], win["lds_doubles"], win["panel"], win["jacobi_lanes"])
+    return {"threads": win["threads"], "lds_doubles": win["lds_doubles"], "panel": win["panel"],
+            "jacobi_lanes": win["jacobi_lanes"], "seconds": win["seconds"], "table": table}
 
 
 def ising_gradient(tn: TensorNetwork, nedges: int, n: int):

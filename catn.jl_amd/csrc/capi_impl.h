@@ -64,6 +64,9 @@ class Executor {
   int threads = 256;
   // LDS budget per workgroup in doubles; 0 = lds_doubles_for_batch(batch)
   int lds_override = 0;
+  // tile parameters: Householder panel width, global Jacobi lanes per pair.
+  // The reverse sweep replays the forward's factorisations with the same ones.
+  int qb = kQBDefault, jac_lanes = kJacLanesDefault;
   int lds_for(int nb) const {
     if (lds_override > 0) return lds_override;
     const int a = lds_doubles_for_batch(nb), lim = lds_limit();
@@ -988,7 +991,8 @@ int catn_set_threads(catn_ctx* ctx, int32_t threads) {
   return 0;
 }
 
-int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles) {
+int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles, int32_t panel,
+                    int32_t jacobi_lanes) {
   if (threads != 64 && threads != 128 && threads != 256)
     return catn::fail(-2, "catn_set_launch: threads must be 64, 128 or 256");
   const int lim = ctx->ex->lds_limit();
@@ -996,15 +1000,27 @@ int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles) {
     return catn::fail(-2, "catn_set_launch: lds_doubles must be 0 (automatic) or in [" +
                               std::to_string(catn::kLdsMinDoubles) + ", " + std::to_string(lim) +
                               "] on this device");
+  if (panel != 0 && panel != 8 && panel != 16 && panel != 32)
+    return catn::fail(-2, "catn_set_launch: panel must be 0 (default 16), 8, 16 or 32");
+  if (jacobi_lanes != 0 && jacobi_lanes != 8 && jacobi_lanes != 16 && jacobi_lanes != 32 &&
+      jacobi_lanes != 64)
+    return catn::fail(-2, "catn_set_launch: jacobi_lanes must be 0 (default 16), 8, 16, 32 or 64");
+  if (jacobi_lanes > threads)
+    return catn::fail(-2, "catn_set_launch: jacobi_lanes exceeds the lanes per workgroup");
   ctx->ex->threads = threads;
   ctx->ex->lds_override = lds_doubles;
+  ctx->ex->qb = panel ? panel : catn::kQBDefault;
+  ctx->ex->jac_lanes = jacobi_lanes ? jacobi_lanes : catn::kJacLanesDefault;
   return 0;
 }
 
-int catn_get_launch(catn_ctx* ctx, int32_t* threads, int32_t* lds_doubles, int32_t* lds_limit) {
+int catn_get_launch(catn_ctx* ctx, int32_t* threads, int32_t* lds_doubles, int32_t* lds_limit,
+                    int32_t* panel, int32_t* jacobi_lanes) {
   if (threads) *threads = ctx->ex->threads;
   if (lds_doubles) *lds_doubles = ctx->ex->lds_override;
   if (lds_limit) *lds_limit = ctx->ex->lds_limit();
+  if (panel) *panel = ctx->ex->qb;
+  if (jacobi_lanes) *jacobi_lanes = ctx->ex->jac_lanes;
   return 0;
 }
 

@@ -22,7 +22,8 @@ constexpr int kThreads = 256;
 constexpr int kStaticLdsBytes = 256;
 
 __global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, const Op* ops,
-                                                             const int* ranges, int lds_cap) {
+                                                             const int* ranges, int lds_cap,
+                                                             int qb, int jac_lanes) {
   __shared__ double red[16];
   __shared__ int flag;
   WG g;
@@ -31,6 +32,8 @@ __global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, 
   g.red = red;
   g.flag = &flag;
   g.lds_cap = lds_cap;
+  g.qb = qb;
+  g.jac_lanes = jac_lanes;
   const int b = blockIdx.x;
   InstState* st = states + b;
   g.sub = st->sub_ticks;
@@ -38,7 +41,8 @@ __global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, 
 }
 
 // Reverse sweep: workgroup b walks the tape of instance b backwards.
-__global__ __launch_bounds__(kThreads, 4) void exec_backward(InstState* states, int lds_cap) {
+__global__ __launch_bounds__(kThreads, 4) void exec_backward(InstState* states, int lds_cap, int qb,
+                                                             int jac_lanes) {
   __shared__ double red[16];
   __shared__ int flag;
   WG g;
@@ -47,6 +51,8 @@ __global__ __launch_bounds__(kThreads, 4) void exec_backward(InstState* states, 
   g.red = red;
   g.flag = &flag;
   g.lds_cap = lds_cap;
+  g.qb = qb;
+  g.jac_lanes = jac_lanes;
   InstState* st = states + blockIdx.x;
   g.sub = st->sub_ticks;
   run_backward(g, st);
@@ -157,7 +163,7 @@ class HipExecutor : public Executor {
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
     hipcheck(hipGetLastError(), "before launch exec_programs");
     hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
-                       d_states_, d_ops_, d_ranges_, lds_cap);
+                       d_states_, d_ops_, d_ranges_, lds_cap, qb, jac_lanes);
     finish_launch("exec_programs");
     float ms = 0.0f;
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
@@ -178,7 +184,7 @@ class HipExecutor : public Executor {
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
     hipcheck(hipGetLastError(), "before launch exec_backward");
     hipLaunchKernelGGL(exec_backward, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
-                       d_states_, lds_cap);
+                       d_states_, lds_cap, qb, jac_lanes);
     finish_launch("exec_backward");
     float ms = 0.0f;
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");

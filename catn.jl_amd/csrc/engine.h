@@ -43,12 +43,21 @@ double* emu_lds_base();
 // ---------------------------------------------------------------------------
 // workgroup context
 // ---------------------------------------------------------------------------
+// tile parameters of a launch (set with the launch shape, picked by the
+// autotuner): Householder panel width and lanes per column pair of the
+// global-memory Jacobi
+constexpr int kQBMax = 32;            // widest panel (T factors are stored kQBMax x n)
+constexpr int kQBDefault = 16;
+constexpr int kJacLanesDefault = 16;
+
 struct WG {
   int tid, nthr;
   double* red;     // LDS, >= 16 doubles (one per wavefront)
   int* flag;       // LDS int
   int lds_cap;     // doubles available in the dynamic LDS region
   int64_t* sub;    // sub-step tick accumulators (instance state), or null
+  int qb = kQBDefault;               // Householder panel width (8, 16 or 32)
+  int jac_lanes = kJacLanesDefault;  // global Jacobi lanes per pair (8..64)
 #if defined(CATN_EMU_MT)
   // the emulator checks that every lane arrives from the same call site
   void sync(const char* file = __builtin_FILE(), int line = __builtin_LINE()) const {
@@ -567,13 +576,12 @@ struct IdxTheta {
 };
 
 // ---------------------------------------------------------------------------
-// Blocked Householder QR (LAPACK dgeqrf structure): panels of kQB columns are
+// Blocked Householder QR (LAPACK dgeqrf structure): panels of g.qb columns are
 // factorised by wg_geqr2; the trailing matrix is updated with the compact WY
 // form Q_p^T C = C - V (T^T (V^T C)) through two tiled GEMMs, so the trailing
 // matrix is streamed twice per panel instead of twice per column. The panel
-// T factors are kept (T_all, kQB x n) for wg_apply_q_blocked.
+// T factors are kept (T_all, kQBMax x n) for wg_apply_q_blocked.
 // ---------------------------------------------------------------------------
-constexpr int kQB = 16;
 
 // C (jb x N) = V^T B for jb <= 16 with the skinny tile (LDS budget permitting)
 CATN_DN void wg_gemm_skinny_tn(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
@@ -595,20 +603,20 @@ CATN_D void wg_larft(const WG& g, const double* V, int rows, int jb, const doubl
   // T(j, i) = -tau_i sum_{l=j}^{i-1} T(j, l) Y(l, i): row j of T needs only its
   // own earlier entries, so lane j builds row j in registers.
   for (int j = g.tid; j < jb; j += g.nthr) {
-    double row[kQB];
-    for (int i = 0; i < kQB; ++i) row[i] = 0.0;
+    double row[kQBMax];
+    for (int i = 0; i < kQBMax; ++i) row[i] = 0.0;
     row[j] = tau[j];
     for (int i = j + 1; i < jb; ++i) {
       double acc = 0.0;
       for (int l = j; l < i; ++l) acc += row[l] * Y[l + jb * i];
       row[i] = -tau[i] * acc;
     }
-    for (int i = 0; i < jb; ++i) T[j + kQB * i] = row[i];
+    for (int i = 0; i < jb; ++i) T[j + kQBMax * i] = row[i];
   }
   g.sync();
 }
 
-// W2 (jb x N, ld jb) = op(T) W (jb x N, ld jb); T upper triangular, ld kQB;
+// W2 (jb x N, ld jb) = op(T) W (jb x N, ld jb); T upper triangular, ld kQBMax;
 // op(T) = T^T when trans.
 CATN_D void wg_gemm_nn_t(const WG& g, int jb, int N, const double* T, const double* W, double* W2,
                          bool trans) {
@@ -616,9 +624,9 @@ CATN_D void wg_gemm_nn_t(const WG& g, int jb, int N, const double* T, const doub
     const int r = (int)(idx % jb), c = (int)(idx / jb);
     double s = 0.0;
     if (trans) {
-      for (int q = 0; q <= r; ++q) s += T[q + kQB * r] * W[q + (int64_t)jb * c];
+      for (int q = 0; q <= r; ++q) s += T[q + kQBMax * r] * W[q + (int64_t)jb * c];
     } else {
-      for (int q = r; q < jb; ++q) s += T[r + kQB * q] * W[q + (int64_t)jb * c];
+      for (int q = r; q < jb; ++q) s += T[r + kQBMax * q] * W[q + (int64_t)jb * c];
     }
     W2[idx] = s;
   }
@@ -655,17 +663,18 @@ CATN_DN void wg_geqr2_staged(const WG& g, double* A, int m, int n, int64_t lda, 
 
 CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t lda, double* tau,
                       double* Tall) {
-  if (n <= kQB) {
+  const int qb = g.qb;
+  if (n <= qb) {
     wg_geqr2_staged(g, A, m, n, lda, tau);
     return false;   // unblocked: no T factors
   }
-  double* V = sc.alloc((int64_t)m * kQB);
-  double* W = sc.alloc((int64_t)kQB * n);
-  double* W2 = sc.alloc((int64_t)kQB * n);
-  double* Y = sc.alloc(kQB * kQB);
+  double* V = sc.alloc((int64_t)m * qb);
+  double* W = sc.alloc((int64_t)qb * n);
+  double* W2 = sc.alloc((int64_t)qb * n);
+  double* Y = sc.alloc(qb * qb);
   if (sc.fail) return false;
-  for (int p = 0; p < n; p += kQB) {
-    const int jb = (n - p) < kQB ? (n - p) : kQB;
+  for (int p = 0; p < n; p += qb) {
+    const int jb = (n - p) < qb ? (n - p) : qb;
     const int rows = m - p;
     double* Ap = A + p + lda * (int64_t)p;
     int64_t t0 = now_ticks();
@@ -673,7 +682,7 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     sub_tick(g, 8, t0);
     t0 = now_ticks();
     wg_panel_v(g, Ap, rows, jb, lda, V);
-    double* T = Tall + (int64_t)kQB * p;   // kQB x jb block, ld kQB
+    double* T = Tall + (int64_t)kQBMax * p;   // jb x jb block, ld kQBMax
     wg_larft(g, V, rows, jb, tau + p, T, Y);
     const int nc = n - p - jb;
     if (nc > 0) {
@@ -700,17 +709,18 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
     return;
   }
   const int64_t save = sc.top;
-  double* V = sc.alloc((int64_t)m * kQB);
-  double* W = sc.alloc((int64_t)kQB * k);
-  double* W2 = sc.alloc((int64_t)kQB * k);
+  const int qb = g.qb;   // the panel width wg_geqrf used (same launch)
+  double* V = sc.alloc((int64_t)m * qb);
+  double* W = sc.alloc((int64_t)qb * k);
+  double* W2 = sc.alloc((int64_t)qb * k);
   if (sc.fail) return;
   const int64_t t0 = now_ticks();
-  const int last = ((n - 1) / kQB) * kQB;
-  for (int p = last; p >= 0; p -= kQB) {
-    const int jb = (n - p) < kQB ? (n - p) : kQB;
+  const int last = ((n - 1) / qb) * qb;
+  for (int p = last; p >= 0; p -= qb) {
+    const int jb = (n - p) < qb ? (n - p) : qb;
     const int rows = m - p;
     wg_panel_v(g, A + p + lda * (int64_t)p, rows, jb, lda, V);
-    const double* T = Tall + (int64_t)kQB * p;
+    const double* T = Tall + (int64_t)kQBMax * p;
     double* Xp = X + p;
     // Q_p X = X - V (T (V^T X))
     wg_gemm_skinny_tn(g, jb, k, rows, V, rows, Xp, ldx, W, jb);
@@ -842,15 +852,14 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
   return -1;
 }
 
-// Jacobi on an operand in global memory: a 16-lane group per column pair, so
-// a wavefront rotates four pairs at once (each lane holds up to kJacReg rows
-// of both columns in registers for n <= 128); 16 consecutive doubles per
-// group keep the loads coalesced.
-constexpr int kJacobiGlobalLanes = 16;
+// Jacobi on an operand in global memory: a group of g.jac_lanes lanes per
+// column pair (default 16: a wavefront rotates four pairs at once, each lane
+// holding up to kJacReg rows of both columns in registers for n <= 128);
+// consecutive lanes of a group read consecutive doubles.
 
 CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double* V,
                       int64_t ldv, int maxsweeps) {
-  const int sw = jacobi_core(g, A, m, n, lda, V, ldv, maxsweeps, -1.0, kJacobiGlobalLanes);
+  const int sw = jacobi_core(g, A, m, n, lda, V, ldv, maxsweeps, -1.0, g.jac_lanes);
   if (g.tid == 0 && g.sub && sw > 0) g.sub[11] += sw;
   return sw;
 }
@@ -1076,7 +1085,7 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
   if (use_qr) {
     tau = sc.alloc(n);
     R = sc.alloc((int64_t)n * n);
-    Tall = sc.alloc((int64_t)kQB * n);
+    Tall = sc.alloc((int64_t)kQBMax * n);
     if (sc.fail) return -1;
     blocked = wg_geqrf(g, sc, W, m, n, m, tau, Tall);
     if (sc.fail) return -1;
@@ -1367,7 +1376,7 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
     tau = sc.alloc(r);
     R = sc.alloc(rr);
     X = sc.alloc(rr);
-    Tall = sc.alloc((int64_t)kQB * r);
+    Tall = sc.alloc((int64_t)kQBMax * r);
     if (sc.fail) return false;
   }
   double fpart = 0.0;
@@ -2042,7 +2051,7 @@ CATN_DN bool wg_thin_qr(const WG& g, Scratch& sc, double* Wk, int m, int n, doub
   double* tau = sc.alloc(n);
   double* R = sc.alloc((int64_t)n * n);
   double* Q = sc.alloc((int64_t)m * n);
-  double* Tall = sc.alloc((int64_t)kQB * n);
+  double* Tall = sc.alloc((int64_t)kQBMax * n);
   if (sc.fail) return false;
   const bool blocked = wg_geqrf(g, sc, Wk, m, n, m, tau, Tall);
   if (sc.fail) return false;

@@ -156,13 +156,17 @@ int catn_bwd_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int
 /* Workgroup size of the interpreter kernel: 64, 128 or 256 lanes (default 256). */
 int catn_set_threads(catn_ctx* ctx, int32_t threads);
 /* Launch shape of the interpreter and reverse-sweep kernels: lanes per
-   workgroup (64, 128, 256) and LDS budget per workgroup in doubles (0 =
+   workgroup (64, 128, 256); LDS budget per workgroup in doubles (0 =
    automatic: 10176 for batches up to 512 networks, else 5056; otherwise
-   within [2048, lds_limit]). The autotuner (catn_amd.autotune) picks it by
-   timing a batch. No counterpart in the reference, whose CPU path has no
-   launch shape; the reference's per-network results do not depend on it. */
-int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles);
-int catn_get_launch(catn_ctx* ctx, int32_t* threads, int32_t* lds_doubles, int32_t* lds_limit);
+   within [2048, lds_limit]); Householder panel width (8, 16, 32; 0 = 16);
+   lanes per column pair of the global-memory Jacobi (8..64; 0 = 16). The
+   autotuner (catn_amd.autotune) picks it by timing a batch. No counterpart in
+   the reference, whose CPU path has no launch shape; the shape changes
+   rounding, not the algorithm or its decisions beyond rounding. */
+int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles, int32_t panel,
+                    int32_t jacobi_lanes);
+int catn_get_launch(catn_ctx* ctx, int32_t* threads, int32_t* lds_doubles, int32_t* lds_limit,
+                    int32_t* panel, int32_t* jacobi_lanes);
 /* Device memory statistics: arena high-water mark and capacity (elements). */
 int catn_net_mem(catn_net* net, int64_t* arena_used, int64_t* arena_cap);
 

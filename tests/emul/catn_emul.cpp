@@ -42,6 +42,8 @@ class CpuExecutor : public Executor {
     g.red = red;
     g.flag = &flag;
     g.lds_cap = lds_for((int)states.size());
+    g.qb = qb;
+    g.jac_lanes = jac_lanes;
     if (getenv("CATN_EMU_LDS")) g.lds_cap = atoi(getenv("CATN_EMU_LDS"));   // a device batch's budget
     for (size_t b = 0; b < states.size(); ++b) {
       const std::vector<Op>& p = *programs[b];
@@ -58,6 +60,8 @@ class CpuExecutor : public Executor {
     g.red = red;
     g.flag = &flag;
     g.lds_cap = lds_cap > 0 ? lds_cap : kLdsMaxDoubles;
+    g.qb = qb;
+    g.jac_lanes = jac_lanes;
     for (InstState* st : states) {
       g.sub = st->sub_ticks;
       catn::run_backward(g, st);

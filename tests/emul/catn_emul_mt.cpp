@@ -294,6 +294,8 @@ class MtExecutor : public Executor {
         g.red = red;
         g.flag = &flag;
         g.lds_cap = lds_cap;
+        g.qb = qb;
+        g.jac_lanes = jac_lanes;
         g.sub = st->sub_ticks;
         body(g);
       });

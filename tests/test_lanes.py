@@ -74,3 +74,34 @@ def test_lane_parallel_matches_one_thread(catn, ctxs, case):
     gtol = 1e-4 if kw.get("dtype") else 1e-9
     assert np.abs(a[2] - b[2]).max() < gtol * scale
     assert np.abs(a[3] - b[3]).max() < gtol * scale
+
+
+@pytest.mark.parametrize("tiles", [(32, 8), (8, 64)])
+def test_tile_parameters_lane_parallel(catn, ctxs, tiles):
+    """Householder panel width and global-Jacobi lanes per pair at 64 lanes
+    (one wavefront, which keeps the lane emulator fast), with an
+    LDS budget of 2048 doubles so that the blocked Householder and the
+    global-memory Jacobi run: same lnZ (1e-12 per spin) and edge sequence as
+    the one-thread build, no divergent barrier."""
+    mt, st = ctxs
+    L = 11
+    edges = X.square_edges(L)
+    n = L * L
+    r = np.random.default_rng(L)
+    w, h = r.choice([-1.0, 1.0], len(edges)), np.zeros(n)
+    kw = dict(Dmax=16, chi=64)   # SVDs up to min(m, n) = 32, 12 Jacobi sweeps
+    qb, jl = tiles
+    res = []
+    for ctx in (mt, st):
+        ctx.set_launch(64, 2048, qb, jl)
+        try:
+            tn = catn.ising_network(n, edges, w, h, 0.9, ctx=ctx, **kw)
+            lz, err, _ = catn.contraction_batch([tn])
+            res.append((lz[0], err[0], tn.edge_log()))
+        finally:
+            ctx.set_launch(256, 0, 0, 0)
+    before = _divergent()
+    assert res[0][2] == res[1][2]
+    assert abs(res[0][0] - res[1][0]) / n < 1e-12
+    assert abs(res[0][1] - res[1][1]) <= 1e-9 * max(1.0, abs(res[1][1]))
+    assert _divergent() == before
