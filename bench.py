@@ -251,14 +251,14 @@ def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
                                      for r in tuned["table"])
             + f" -> ({tuned['threads']},{tuned['lds_doubles']},{tuned['panel']},"
               f"{tuned['jacobi_lanes']})")
-    plan0 = 0.0
+    plan0 = run0 = 0.0
     for step in range(warmup + steps):
         nets = pinned_batch(catn, ctx, n, edges, w, h)
         _prefetch(nets)
         timed = step >= warmup
         if step == warmup:
             ctx.kernel_stats(reset=True)
-            plan0 = ctx.host_stats()[0]
+            plan0, run0 = ctx.host_stats()
         t0 = time.perf_counter()
         lz, _, _ = catn.contraction_batch(nets)
         g = gradient_from_lnz(lz, n, edges)
@@ -277,9 +277,13 @@ def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
                 json.dump(op_profile(nets), f, indent=1)
         del nets
     launches, ksec = ctx.kernel_stats()
-    plan = (ctx.host_stats()[0] - plan0) / max(steps, 1)
+    p1, r1 = ctx.host_stats()
+    plan = (p1 - plan0) / max(steps, 1)
+    # executor time outside the kernels: state and program copies, launch and
+    # synchronisation, per step
+    copies = (r1 - run0 - ksec) / max(steps, 1)
     return dict(times=times, lz=res[0], grad=res[1], launches=launches, seconds=ksec,
-                flops=flops, bytes=bytes_, tuned=tuned, plan_s=plan)
+                flops=flops, bytes=bytes_, tuned=tuned, plan_s=plan, copies_s=copies)
 
 
 TRAFFIC_FILE = "exec_programs_sg100_plain_traffic.json"
@@ -461,6 +465,7 @@ def main():
         out_cfg["plain_solves_per_s"] = pv
         out_cfg["plain_kernel"] = {"launches": plain["launches"], "seconds": plain["seconds"]}
         out_cfg["plain_host_planning_s_per_step"] = plain["plan_s"]
+        out_cfg["plain_host_copies_and_launch_s_per_step"] = plain["copies_s"]
         t, l, _, qb, jl = ctx.launch()
         out_cfg["launch_shape"] = {"threads": t, "lds_doubles": l or "automatic", "panel": qb,
                                    "jacobi_lanes": jl}

@@ -578,12 +578,24 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     parallel_for(na, [&](int k) {
       if (share[k] < 0) cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
     });
+    // followers take the leader's planner state without its program (they run
+    // the leader's uploaded program); the leaders' programs are set aside
+    // while the copies are made and put back afterwards
+    std::vector<std::vector<Op>> held(na);
+    std::vector<char> leads(na, 0);
+    for (int k = 0; k < na; ++k)
+      if (share[k] >= 0 && !leads[share[k]]) {
+        leads[share[k]] = 1;
+        held[share[k]].swap(active[share[k]]->pl.prog);
+      }
     parallel_for(na, [&](int k) {
       if (share[k] >= 0) {
         active[k]->pl = active[share[k]]->pl;
         cut[k] = cut[share[k]];
       }
     });
+    for (int k = 0; k < na; ++k)
+      if (leads[k]) held[k].swap(active[k]->pl.prog);
     ex->plan_seconds += wall() - t0;
     if (!ex->cut_reverse_enabled())
       for (size_t k = 0; k < active.size(); ++k)
