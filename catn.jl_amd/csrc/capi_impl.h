@@ -549,71 +549,96 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     if (exec_of(n) != ex) throw Error(-2, "contraction: networks of another context");
     n->ensure_device();
     n->reset_acc();
-    n->pl.start();
   }
+  parallel_for((int)nets.size(), [&](int k) { nets[k]->pl.start(); });
+  // Planning is a deterministic function of the planner state and of the cut
+  // results applied between segments. Networks whose states are equal at the
+  // start (pinned networks of one ising instance differ only in site values)
+  // follow one owner: only the owner plans, the followers run its uploaded
+  // program, and a follower takes a copy of the owner's planner only when its
+  // cut result parts from the owner's, or once at the end.
   std::vector<catn_net*> active = nets;
-  std::vector<int> need(nets.size());
-  while (!active.empty()) {
-    std::vector<catn_net*> still;
-    std::vector<int> cut(active.size());
+  const int n0 = (int)active.size();
+  std::vector<int> att(n0, -1);   // index in `active` of the owner followed, or -1
+  {
     const double t0 = wall();
-    // Plan each group of networks with equal planner states once (pinned
-    // networks of one ising instance differ only in site values): the first
-    // member plans, the others take a copy of its planner after the step.
-    const int na = (int)active.size();
-    std::vector<uint64_t> fp(na);
-    std::vector<int> share(na, -1);
-    parallel_for(na, [&](int k) { fp[k] = active[k]->pl.fingerprint(); });
-    {
-      std::unordered_map<uint64_t, int> first;
-      for (int k = 0; k < na; ++k) {
-        auto it = first.find(fp[k]);
-        if (it == first.end()) first.emplace(fp[k], k);
-        else share[k] = it->second;
-      }
+    std::vector<uint64_t> fp(n0);
+    parallel_for(n0, [&](int k) { fp[k] = active[k]->pl.fingerprint(); });
+    std::unordered_map<uint64_t, int> first;
+    for (int k = 0; k < n0; ++k) {
+      auto it = first.find(fp[k]);
+      if (it == first.end()) first.emplace(fp[k], k);
+      else att[k] = it->second;
     }
-    parallel_for(na, [&](int k) {   // equal fingerprints must be equal states
-      if (share[k] >= 0 && !active[k]->pl.same_state(active[share[k]]->pl)) share[k] = -1;
+    parallel_for(n0, [&](int k) {   // equal fingerprints must be equal states
+      if (att[k] >= 0 && !active[k]->pl.same_state(active[att[k]]->pl)) att[k] = -1;
     });
+    ex->plan_seconds += wall() - t0;
+  }
+  while (!active.empty()) {
+    const int na = (int)active.size();
+    std::vector<int> cut(na);
+    double t0 = wall();
     parallel_for(na, [&](int k) {
-      if (share[k] < 0) cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
-    });
-    // followers take the leader's planner state without its program (they run
-    // the leader's uploaded program); the leaders' programs are set aside
-    // while the copies are made and put back afterwards
-    std::vector<std::vector<Op>> held(na);
-    std::vector<char> leads(na, 0);
-    for (int k = 0; k < na; ++k)
-      if (share[k] >= 0 && !leads[share[k]]) {
-        leads[share[k]] = 1;
-        held[share[k]].swap(active[share[k]]->pl.prog);
-      }
-    parallel_for(na, [&](int k) {
-      if (share[k] >= 0) {
-        active[k]->pl = active[share[k]]->pl;
-        cut[k] = cut[share[k]];
-      }
+      if (att[k] < 0) cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
     });
     for (int k = 0; k < na; ++k)
-      if (leads[k]) held[k].swap(active[k]->pl.prog);
+      if (att[k] >= 0) cut[k] = cut[att[k]];
     ex->plan_seconds += wall() - t0;
     if (!ex->cut_reverse_enabled())
-      for (size_t k = 0; k < active.size(); ++k)
+      for (int k = 0; k < na; ++k)
         if (cut[k] && active[k]->hs.record)
           throw Error(-15, "gradient across a Dmax cut (cut_bondim_opt!) is disabled on this "
                            "device; use the pinning route (magnetization / correlation) or a "
                            "Dmax that the network never exceeds");
-    run_nets(ex, active, &share);
-    for (size_t k = 0; k < active.size(); ++k) {
-      catn_net* n = active[k];
-      if (cut[k]) {
-        n->pl.apply_cut(n->hs.last_cut);
-        still.push_back(n);
-      } else {
-        n->consumed = true;
+    run_nets(ex, active, &att);
+    t0 = wall();
+    // regroup by cut result: a follower whose kept count differs from its
+    // owner's joins (or founds) the subgroup of its count; a founder copies
+    // the owner's planner before the owner applies its own count
+    std::vector<int> own(na, -1);   // new owner index (in `active`) per network
+    std::vector<int> copy_from(na, -1);
+    {
+      std::vector<std::vector<std::pair<int, int>>> sub(na);   // per owner: (myd, founder)
+      for (int k = 0; k < na; ++k) {
+        if (att[k] < 0) { own[k] = k; continue; }
+        const int l = att[k];
+        if (!cut[k] || active[k]->hs.last_cut == active[l]->hs.last_cut) { own[k] = l; continue; }
+        int f = -1;
+        for (const auto& pr : sub[l])
+          if (pr.first == active[k]->hs.last_cut) f = pr.second;
+        if (f < 0) {
+          sub[l].push_back({active[k]->hs.last_cut, k});
+          copy_from[k] = l;
+          f = k;
+        }
+        own[k] = f;
       }
     }
+    // the new founders' planners, and those finished followers need (edge log,
+    // maxdim); programs were cleared by run_nets, so nothing large is copied
+    parallel_for(na, [&](int k) {
+      if (copy_from[k] >= 0) active[k]->pl = active[copy_from[k]]->pl;
+      else if (!cut[k] && att[k] >= 0) active[k]->pl = active[att[k]]->pl;
+    });
+    parallel_for(na, [&](int k) {
+      if (cut[k] && own[k] == k) active[k]->pl.apply_cut(active[k]->hs.last_cut);
+    });
+    std::vector<catn_net*> still;
+    std::vector<int> pos(na, -1), next_att;
+    for (int k = 0; k < na; ++k) {
+      if (cut[k]) {
+        pos[k] = (int)still.size();
+        still.push_back(active[k]);
+      } else {
+        active[k]->consumed = true;
+      }
+    }
+    for (int k = 0; k < na; ++k)
+      if (cut[k]) next_att.push_back(own[k] == k ? -1 : pos[own[k]]);
+    ex->plan_seconds += wall() - t0;
     active.swap(still);
+    att.swap(next_att);
   }
 }
 
