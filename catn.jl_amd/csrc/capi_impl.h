@@ -579,8 +579,13 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     const int na = (int)active.size();
     std::vector<int> cut(na);
     double t0 = wall();
-    parallel_for(na, [&](int k) {
-      if (att[k] < 0) cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
+    // work lists, so that few owners are planned without starting threads
+    std::vector<int> owners;
+    for (int k = 0; k < na; ++k)
+      if (att[k] < 0) owners.push_back(k);
+    parallel_for((int)owners.size(), [&](int q) {
+      const int k = owners[q];
+      cut[k] = active[k]->pl.step() == Planner::NEED_CUT ? 1 : 0;
     });
     for (int k = 0; k < na; ++k)
       if (att[k] >= 0) cut[k] = cut[att[k]];
@@ -617,12 +622,18 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     }
     // the new founders' planners, and those finished followers need (edge log,
     // maxdim); programs were cleared by run_nets, so nothing large is copied
-    parallel_for(na, [&](int k) {
-      if (copy_from[k] >= 0) active[k]->pl = active[copy_from[k]]->pl;
-      else if (!cut[k] && att[k] >= 0) active[k]->pl = active[att[k]]->pl;
+    std::vector<int> copies, applies;
+    for (int k = 0; k < na; ++k) {
+      if (copy_from[k] >= 0 || (!cut[k] && att[k] >= 0)) copies.push_back(k);
+      if (cut[k] && own[k] == k) applies.push_back(k);
+    }
+    parallel_for((int)copies.size(), [&](int q) {
+      const int k = copies[q];
+      active[k]->pl = active[copy_from[k] >= 0 ? copy_from[k] : att[k]]->pl;
     });
-    parallel_for(na, [&](int k) {
-      if (cut[k] && own[k] == k) active[k]->pl.apply_cut(active[k]->hs.last_cut);
+    parallel_for((int)applies.size(), [&](int q) {
+      const int k = applies[q];
+      active[k]->pl.apply_cut(active[k]->hs.last_cut);
     });
     std::vector<catn_net*> still;
     std::vector<int> pos(na, -1), next_att;
