@@ -917,6 +917,104 @@ CATN_D void jacobi_finish(const WG& g, const double* W, int rows, int n, int64_t
   g.sync();
 }
 
+// Tiny operands (m * n <= kTinySvd): lane 0 computes the whole thin SVD in
+// LDS while the other lanes wait at one barrier, which is far cheaper than the
+// dozen workgroup barriers of the parallel paths at this size. Same
+// round-robin rotations, per-pair test, noise threshold, sweep limit and
+// output conventions as jacobi_core + jacobi_finish; wide operands are
+// handled through their transpose as in wg_svd.
+constexpr int kTinySvd = 64;
+
+CATN_DN int wg_svd_tiny(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
+                        SvdResult& out, int* sweeps_used) {
+  const bool tr = m < n;
+  const int tm = tr ? n : m, tn = tr ? m : n;
+  double* U = sc.alloc((int64_t)tm * tn);
+  double* Ss = sc.alloc(tn);
+  double* Vs = sc.alloc((int64_t)tn * tn);
+  if (sc.fail) return -1;
+  double* W = lds_base();
+  double* V = W + tm * tn;
+  double* key = V + tn * tn;
+  if (g.tid == 0) {
+    double fro2 = 0.0;
+    for (int j = 0; j < tn; ++j)
+      for (int i = 0; i < tm; ++i) {
+        const double v = tr ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
+        W[i + tm * j] = v;
+        fro2 += v * v;
+      }
+    for (int j = 0; j < tn; ++j)
+      for (int i = 0; i < tn; ++i) V[i + tn * j] = i == j ? 1.0 : 0.0;
+    const double thr = 1e-36 * fro2;
+    const double tol = 4.0 * 2.220446049250313e-16 * sqrt((double)(tm > 1 ? tm : 1));
+    const int N = tn + (tn & 1), npairs = N / 2;
+    int sw = tn < 2 ? 0 : -1;
+    for (int sweep = 1; sweep <= 60 && sw < 0; ++sweep) {
+      int rotated = 0;
+      for (int r = 0; r < N - 1; ++r)
+        for (int k = 0; k < npairs; ++k) {
+          int p, q;
+          if (k == 0) { p = r; q = N - 1; }
+          else { p = (r + k) % (N - 1); q = (r - k + N - 1) % (N - 1); }
+          if (p > q) { const int t = p; p = q; q = t; }
+          if (q >= tn) continue;
+          double* ap = W + tm * p;
+          double* aq = W + tm * q;
+          double al = 0.0, be = 0.0, ga = 0.0;
+          for (int i = 0; i < tm; ++i) {
+            al += ap[i] * ap[i]; be += aq[i] * aq[i]; ga += ap[i] * aq[i];
+          }
+          if (al <= thr || be <= thr) continue;
+          if (fabs(ga) <= tol * sqrt(al) * sqrt(be)) continue;
+          rotated = 1;
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+          for (int i = 0; i < tm; ++i) {
+            const double x = ap[i], y = aq[i];
+            ap[i] = c * x - sn * y;
+            aq[i] = sn * x + c * y;
+          }
+          double* vp = V + tn * p;
+          double* vq = V + tn * q;
+          for (int i = 0; i < tn; ++i) {
+            const double x = vp[i], y = vq[i];
+            vp[i] = c * x - sn * y;
+            vq[i] = sn * x + c * y;
+          }
+        }
+      if (!rotated) sw = sweep;
+    }
+    // norms, descending order stable in the column index, U = W / s (a zero
+    // column keeps the unit vector e_r), V's columns in the same order
+    for (int j = 0; j < tn; ++j) {
+      double ss = 0.0;
+      for (int i = 0; i < tm; ++i) ss += W[i + tm * j] * W[i + tm * j];
+      key[j] = sqrt(ss);
+    }
+    for (int j = 0; j < tn; ++j) {
+      const double sj = key[j];
+      int rr = 0;
+      for (int i = 0; i < tn; ++i) rr += (key[i] > sj) || (key[i] == sj && i < j);
+      Ss[rr] = sj;
+      for (int i = 0; i < tm; ++i)
+        U[i + (int64_t)tm * rr] = sj > 0.0 ? W[i + tm * j] / sj : (i == rr ? 1.0 : 0.0);
+      for (int i = 0; i < tn; ++i) Vs[i + (int64_t)tn * rr] = V[i + tn * j];
+    }
+    *g.flag = sw;
+  }
+  g.sync();
+  const int sw = *g.flag;
+  g.sync();   // every lane has read the flag before it is reused
+  if (sweeps_used) *sweeps_used = sw;
+  out.r = tn;
+  out.S = Ss;
+  out.U = tr ? Vs : U;   // m x r (ld m)
+  out.V = tr ? U : Vs;   // n x r (ld n)
+  return sw < 0 ? -1 : tn;
+}
+
 // LDS-resident path: the whole m x n operand, V and the norms live in LDS.
 CATN_DN int wg_svd_lds(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
                        bool transpose_in, SvdResult& out, int* sweeps_used) {
@@ -1154,6 +1252,7 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
 // A (m x n, lda) -> SVD of A with U (m x r), V (n x r), r = min(m,n).
 CATN_DN int wg_svd(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
                    SvdResult& out, int* sweeps_used) {
+  if ((int64_t)m * n <= kTinySvd) return wg_svd_tiny(g, sc, A, m, n, lda, out, sweeps_used);
   if (m >= n) return wg_svd_tall(g, sc, A, m, n, lda, false, out, sweeps_used);
   // A^T = U' S V'^T  ->  A = V' S U'^T
   SvdResult t;
@@ -1349,11 +1448,139 @@ CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int
   return 1.0 / sqrt(tot);
 }
 
+// wg_factor_fullrank of a tiny operand (tm * r <= kTinySvd): lane 0 runs the
+// same steps serially in LDS (Householder QR of the tall orientation, the
+// inverse bound, the certificate, Q formation and the four output cases);
+// the other lanes wait at one barrier. The scratch outputs are allocated by
+// every lane, as the bump allocator is held per lane.
+CATN_DN bool factor_fullrank_tiny(const WG& g, Scratch& sc, const double* A, int m, int n,
+                                  int64_t lda, double cutoff, bool iso_left, Factor& F,
+                                  double noise) {
+  const int r = m < n ? m : n;
+  const bool trans = m < n;
+  const int tm = trans ? n : m;
+  const bool need_q = iso_left != trans;
+  const int64_t save = sc.top;
+  double* L = sc.alloc((int64_t)m * r);
+  double* Rt = sc.alloc((int64_t)r * n);
+  double* Qs = need_q && !(iso_left && !trans) ? sc.alloc((int64_t)tm * r) : nullptr;
+  if (sc.fail) return false;
+  double* W = lds_base();          // tm x r
+  double* R = W + tm * r;          // r x r
+  double* X = R + r * r;           // r x r
+  double* tau = X + r * r;         // r
+  if (g.tid == 0) {
+    double fro2 = 0.0;
+    for (int j = 0; j < r; ++j)
+      for (int i = 0; i < tm; ++i) {
+        const double v = trans ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
+        W[i + tm * j] = v;
+        fro2 += v * v;
+      }
+    for (int j = 0; j < r; ++j) {   // dgeqr2: H = I - tau v v^T, v(0) = 1
+      double* x = W + j + tm * j;
+      double ss = 0.0;
+      for (int i = 1; i < tm - j; ++i) ss += x[i] * x[i];
+      double t = 0.0;
+      if (ss > 0.0) {
+        const double alpha = x[0];
+        const double nrm = sqrt(alpha * alpha + ss);
+        const double beta = alpha >= 0.0 ? -nrm : nrm;
+        t = (beta - alpha) / beta;
+        const double scal = 1.0 / (alpha - beta);
+        for (int i = 1; i < tm - j; ++i) x[i] *= scal;
+        x[0] = beta;
+        for (int c = j + 1; c < r; ++c) {
+          double* y = W + j + tm * c;
+          double w = y[0];
+          for (int i = 1; i < tm - j; ++i) w += x[i] * y[i];
+          w *= t;
+          y[0] -= w;
+          for (int i = 1; i < tm - j; ++i) y[i] -= w * x[i];
+        }
+      }
+      tau[j] = t;
+    }
+    for (int j = 0; j < r; ++j)
+      for (int i = 0; i < r; ++i) R[i + r * j] = i <= j ? W[i + tm * j] : 0.0;
+    // 1 / ||R^{-1}||_F by back substitution from the bottom row
+    double tot = 0.0;
+    bool sing = false;
+    for (int i = r - 1; i >= 0 && !sing; --i) {
+      const double rii = R[i + r * i];
+      if (rii == 0.0) { sing = true; break; }
+      const double inv = 1.0 / rii;
+      for (int j = i; j < r; ++j) {
+        double xv;
+        if (j == i) {
+          xv = inv;
+        } else {
+          double acc = 0.0;
+          for (int k = i + 1; k <= j; ++k) acc += R[i + r * k] * X[k + r * j];
+          xv = -acc * inv;
+        }
+        X[i + r * j] = xv;
+        tot += xv * xv;
+      }
+    }
+    const double lb = (sing || !(tot < INFINITY) || tot <= 0.0) ? 0.0 : 1.0 / sqrt(tot);
+    const double fro = sqrt(fro2);
+    const double thr = fmax(fmax(100.0 * cutoff, 1e-12 * fro), noise * fro);
+    const bool ok = lb > thr;
+    if (ok) {
+      double* Q = need_q ? ((iso_left && !trans) ? L : Qs) : nullptr;
+      if (need_q) {   // Q = H_0 ... H_{r-1} [I_r; 0]
+        for (int c = 0; c < r; ++c)
+          for (int i = 0; i < tm; ++i) Q[i + (int64_t)tm * c] = i == c ? 1.0 : 0.0;
+        for (int j = r - 1; j >= 0; --j) {
+          const double t = tau[j];
+          if (t == 0.0) continue;
+          const double* v = W + j + tm * j;
+          for (int c = 0; c < r; ++c) {
+            double* y = Q + j + (int64_t)tm * c;
+            double w = y[0];
+            for (int i = 1; i < tm - j; ++i) w += v[i] * y[i];
+            w *= t;
+            y[0] -= w;
+            for (int i = 1; i < tm - j; ++i) y[i] -= w * v[i];
+          }
+        }
+      }
+      if (iso_left && !trans) {          // L = Q (formed in place), Rt = R
+        for (int q = 0; q < r * r; ++q) Rt[q] = R[q];
+      } else if (iso_left && trans) {    // L = I_m, Rt = A
+        for (int q = 0; q < m * m; ++q) L[q] = (q % m == q / m) ? 1.0 : 0.0;
+        for (int q = 0; q < m * n; ++q) Rt[q] = A[(q % m) + (int64_t)(q / m) * lda];
+      } else if (!iso_left && !trans) {  // L = A, Rt = I_n
+        for (int q = 0; q < m * n; ++q) L[q] = A[(q % m) + (int64_t)(q / m) * lda];
+        for (int q = 0; q < n * n; ++q) Rt[q] = (q % n == q / n) ? 1.0 : 0.0;
+      } else {                           // L = R'^T (m x m), Rt = Q'^T (m x n)
+        for (int q = 0; q < m * m; ++q) L[q] = R[(q / m) + r * (q % m)];
+        for (int q = 0; q < m * n; ++q) Rt[q] = Q[(q / m) + (int64_t)tm * (q % m)];
+      }
+    }
+    *g.flag = ok ? 1 : 0;
+  }
+  g.sync();
+  const bool ok = *g.flag != 0;
+  g.sync();   // every lane has read the flag before it is reused
+  if (!ok) {
+    sc.top = save;
+    return false;
+  }
+  F.r = r;
+  F.L = L;
+  F.Rt = Rt;
+  return true;
+}
+
 CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m, int n,
                                 int64_t lda, double cutoff, int maxdim, bool iso_left,
                                 Factor& F, double noise = 0.0) {
   const int r = m < n ? m : n;
   if (r <= 0 || r > maxdim) return false;
+  if ((int64_t)(m > n ? m : n) * r <= kTinySvd)
+    return factor_fullrank_tiny(g, sc, A, m, n, lda, cutoff, iso_left, F, noise);
   const int64_t save = sc.top;
   // tall orientation T (tm x r): A itself when m >= n, else A^T
   const bool trans = m < n;
