@@ -918,12 +918,13 @@ CATN_D void jacobi_finish(const WG& g, const double* W, int rows, int n, int64_t
 }
 
 // Tiny operands (m * n <= kTinySvd): lane 0 computes the whole thin SVD in
-// LDS while the other lanes wait at one barrier, which is far cheaper than the
-// dozen workgroup barriers of the parallel paths at this size. Same
-// round-robin rotations, per-pair test, noise threshold, sweep limit and
-// output conventions as jacobi_core + jacobi_finish; wide operands are
-// handled through their transpose as in wg_svd.
-constexpr int kTinySvd = 64;
+// LDS while the other lanes wait at one barrier instead of the dozen
+// workgroup barriers of the parallel paths. Measured on the n=100 spin glass:
+// a small gain at 16 elements; at 64 the serial LDS work was slower than the
+// barriers. Same round-robin rotations, per-pair test, noise threshold, sweep
+// limit and output conventions as jacobi_core + jacobi_finish; wide operands
+// are handled through their transpose as in wg_svd.
+constexpr int kTinySvd = 16;
 
 CATN_DN int wg_svd_tiny(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
                         SvdResult& out, int* sweeps_used) {
