@@ -1,0 +1,132 @@
+"""Double-layer (bra-ket) engine, src/braket.jl.
+
+The oracle's restatement (oracle/braket_oracle.py) is pinned first, by the
+checks of test/test_braket.jl: braket_node rebuilds E_i = sum_p T_i (x)
+conj(T_i) (:19-92), a paired-edge eat! of two sites gives <psi|psi> (:96-121),
+and contraction! of chains and a star in exact mode matches the dense
+<psi|psi> (:246-305, with exact_norm of :8-16). Real and complex tensors.
+The device path (real tensors) is then compared with the pinned oracle and
+with the dense norm, on the CPU emulator and, marked gpu, on the MI355X.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import braket_oracle as B
+from oracle import catn_oracle as O
+
+
+def _rand(rng, shape, cplx):
+    a = rng.standard_normal(shape)
+    if cplx:
+        a = a + 1j * rng.standard_normal(shape)
+    return a
+
+
+def _E(T, phys):
+    """Dense E = sum_p T (x) conj(T), axes (ket virtuals..., bra virtuals...)."""
+    nd = T.ndim
+    virt = [a for a in range(nd) if a != phys]
+    letters = "abcdefgh"
+    ket = "".join(letters[a] if a != phys else "p" for a in range(nd))
+    bra = "".join(letters[a].upper() if a != phys else "p" for a in range(nd))
+    out = "".join(letters[a] for a in virt) + "".join(letters[a].upper() for a in virt)
+    return np.einsum(f"{ket},{bra}->{out}", T, T.conj())
+
+
+# ----------------------------- oracle pins ----------------------------------
+
+@pytest.mark.parametrize("shape,phys,cplx", [((3, 2, 4), 1, True), ((3, 2, 4), 1, False),
+                                             ((3, 2), 1, True), ((2, 3, 2, 2), 1, False)])
+def test_oracle_braket_node_rebuilds_double_tensor(shape, phys, cplx):
+    rng = np.random.default_rng(len(shape) * 7 + phys)
+    T = _rand(rng, shape, cplx)
+    deg = T.ndim - 1
+    node = B.braket_node(T, list(range(10, 10 + deg)), phys + 1, chi=10_000)
+    assert node.layer == [True] * deg + [False] * deg
+    assert node.neighbor == list(range(10, 10 + deg)) * 2
+    assert node.cano == len(node.mps)
+    raw = O.mps2raw(node)
+    assert np.allclose(raw, _E(T, phys), rtol=1e-10, atol=1e-12)
+    for t in node.mps[:-1]:                      # left-canonical
+        M = t.reshape(-1, t.shape[2], order="F")
+        assert np.allclose(M.conj().T @ M, np.eye(M.shape[1]), atol=1e-10)
+
+
+@pytest.mark.parametrize("cplx", [True, False])
+def test_oracle_paired_edge_eat_two_sites(cplx):
+    rng = np.random.default_rng(3)
+    T1, T2 = _rand(rng, (2, 4), cplx), _rand(rng, (4, 2), cplx)
+    n1 = B.braket_node(T1, [2], 1, chi=10_000)
+    n2 = B.braket_node(T2, [1], 2, chi=10_000)
+    lg, _, ph = B.eat(n1, n2, 2, 1)
+    ref = B.exact_norm([T1, T2], [["p1", "a"], ["a", "p2"]])
+    assert abs(math.exp(lg) * ph - ref) <= 1e-10 * ref
+    assert ref > 0
+
+
+def _chain(rng, n, D, d, cplx):
+    ts, ixs = [], []
+    for k in range(n):
+        if k == 0:
+            ts.append(_rand(rng, (d, D), cplx)); ixs.append([f"p{k}", f"b{k}"])
+        elif k == n - 1:
+            ts.append(_rand(rng, (D, d), cplx)); ixs.append([f"b{k - 1}", f"p{k}"])
+        else:
+            ts.append(_rand(rng, (D, d, D), cplx)); ixs.append([f"b{k - 1}", f"p{k}", f"b{k}"])
+    return ts, ixs
+
+
+def _star(rng, cplx):
+    ts = [_rand(rng, (2, 2, 2, 2), cplx), _rand(rng, (2, 2), cplx), _rand(rng, (2, 2), cplx),
+          _rand(rng, (2, 2), cplx)]
+    ixs = [["a", "b", "c", "pc"], ["a", "p1"], ["b", "p2"], ["c", "p3"]]
+    return ts, ixs
+
+
+CASES = ["chain3_real", "chain4_cplx", "chain5_cplx", "star_real", "star_cplx", "chain6_D3"]
+
+
+def _case(name):
+    rng = np.random.default_rng(101 + CASES.index(name))
+    if name == "chain3_real":
+        return _chain(rng, 3, 2, 2, False)
+    if name == "chain4_cplx":
+        return _chain(rng, 4, 3, 2, True)
+    if name == "chain5_cplx":
+        return _chain(rng, 5, 3, 2, True)
+    if name == "star_real":
+        return _star(rng, False)
+    if name == "star_cplx":
+        return _star(rng, True)
+    return _chain(rng, 6, 3, 2, False)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("svdopt", [True, False])
+def test_oracle_norm_matches_exact(name, svdopt):
+    ts, ixs = _case(name)
+    ref = B.exact_norm(ts, ixs)
+    bk = B.braket_network(ts, ixs, chi=10_000, svdopt=svdopt)
+    lnz, err, psi = B.contraction(bk)
+    assert abs(math.exp(lnz) * psi - ref) <= 1e-10 * ref
+    assert err < 1e-12
+
+
+def test_oracle_rejects_cycles_and_bad_labels():
+    rng = np.random.default_rng(0)
+    ts = [rng.standard_normal((2, 2, 2)) for _ in range(3)]
+    with pytest.raises(ValueError):
+        B.braket_network(ts, [["a", "b", "p1"], ["b", "c", "p2"], ["c", "a", "p3"]])
+    with pytest.raises(ValueError):
+        B.braket_network([rng.standard_normal((2, 2))] * 2, [["a", "b"], ["a", "b"]])
+
+
+def test_oracle_plants_a_mistake():
+    """A dropped conj on the bra chain (the bra built from T, not conj(T))
+    must fail the pins on complex tensors."""
+    rng = np.random.default_rng(5)
+    T = _rand(rng, (3, 2, 4), True)
+    node = B.braket_node(T.conj(), [1, 2], 2, chi=10_000)   # ket of conj(T): bra of T
+    assert not np.allclose(O.mps2raw(node), _E(T, 1), atol=1e-6)
