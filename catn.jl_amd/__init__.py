@@ -94,6 +94,7 @@ _SIGS = {
     "catn_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
     "catn_host_stats": (C.c_int, [_P, _DP, _DP]),
     "catn_set_threads": (C.c_int, [_P, C.c_int32]),
+    "catn_braket_network": (C.c_int, [_P, C.c_int32, _I32P, _I64P, _DP, _I64P, _P, _P]),
     "catn_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "catn_get_launch": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P]),
     "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
@@ -408,6 +409,42 @@ def tensor_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
 
 
 TensorNetwork_ = tensor_network
+
+
+def braket_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
+                   ctx: Optional[Context] = None, chi: int = 64, **kw) -> TensorNetwork:
+    """braket_network(tensors, ixs; chi, cutoff, norm_method, svdopt, swapopt)
+    (src/braket.jl:369-496) for real tensors: the double-layer network of
+    <psi|psi>. contraction() of the result returns (ln <psi|psi>, error, psi).
+    Tensors are taken with Julia (column-major) index order."""
+    ctx = ctx or default_context()
+    lab_ids = {}
+    labels, ndims, dims, data = [], [], [], []
+    for t, ix in zip(tensors, ixs):
+        t = np.asarray(t)
+        if np.iscomplexobj(t):
+            raise CatnError("braket_network: complex tensors are not supported on the device")
+        t = t.astype(np.float64)
+        if t.ndim != len(ix):
+            raise CatnError("tensor rank does not match its label list")
+        ndims.append(t.ndim)
+        dims.extend(t.shape)
+        data.append(t.ravel(order="F"))
+        for l in ix:
+            if l not in lab_ids:
+                lab_ids[l] = len(lab_ids)
+            labels.append(lab_ids[l])
+    nd = _i32(ndims)
+    dm = np.ascontiguousarray(np.asarray(dims, dtype=np.int64))
+    lb = np.ascontiguousarray(np.asarray(labels, dtype=np.int64))
+    dt = _f64(np.concatenate(data) if data else np.zeros(0))
+    kw.setdefault("Dmax", -1)
+    p = _params(ctx, chi=chi, **kw)
+    out = _P()
+    ctx._check(ctx.lib.catn_braket_network(ctx.h, len(tensors), nd.ctypes.data_as(_I32P),
+                                           dm.ctypes.data_as(_I64P), _dptr(dt),
+                                           lb.ctypes.data_as(_I64P), C.byref(p), C.byref(out)))
+    return TensorNetwork(ctx, out)
 
 
 def contraction(tn: TensorNetwork):

@@ -813,6 +813,136 @@ int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
   CATN_CATCH
 }
 
+// braket_network(tensors, ixs; ...) (src/braket.jl:369-496) for real tensors:
+// one BraKetNode per tensor (braket_node, :75-185, as device split ops), the
+// paired edges in the order their labels first appear.
+int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
+                        const int64_t* dims, const double* data, const int64_t* labels,
+                        const catn_params* p, catn_net** out) {
+  CATN_TRY
+  std::unique_ptr<catn_net> net(new catn_net);
+  net->attach(ctx);
+  apply_params(net.get(), p);
+  std::map<int64_t, std::vector<int>> owners;   // label -> tensor ids (1-based)
+  std::vector<int64_t> first_seen;
+  size_t li = 0;
+  for (int t = 0; t < ntensors; ++t) {
+    for (int k = 0; k < ndims[t]; ++k) {
+      const int64_t l = labels[li + k];
+      if (!owners.count(l)) first_seen.push_back(l);
+      owners[l].push_back(t + 1);
+    }
+    li += ndims[t];
+  }
+  for (auto& kv : owners)
+    if (kv.second.size() > 2)
+      throw catn::Error(-2, "braket_network: label " + std::to_string(kv.first) + " appears " +
+                                std::to_string(kv.second.size()) + " times (expected 1 or 2)");
+  // acyclic virtual-bond graph only (:402-420)
+  std::vector<int> parent(ntensors + 1);
+  for (int t = 0; t <= ntensors; ++t) parent[t] = t;
+  auto root = [&](int x) {
+    while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; }
+    return x;
+  };
+  std::vector<std::pair<int, int>> edges;
+  for (int64_t l : first_seen) {
+    const auto& o = owners[l];
+    if (o.size() != 2) continue;
+    const int a = root(o[0]), b = root(o[1]);
+    if (a == b)
+      throw catn::Error(-2, "braket_network: virtual-bond graph contains a cycle - cyclic graphs "
+                            "are not supported");
+    parent[a] = b;
+    edges.push_back({o[0], o[1]});
+  }
+  catn::Planner& pl = net->pl;
+  size_t di = 0, dd = 0;
+  li = 0;
+  for (int t = 0; t < ntensors; ++t) {
+    const int nd = ndims[t];
+    int phys_pos = -1, nphys = 0;
+    std::vector<int> kn, virt;
+    for (int k = 0; k < nd; ++k) {
+      const auto& o = owners[labels[li + k]];
+      if (o.size() == 1) { phys_pos = k; ++nphys; }
+      else { virt.push_back(k); kn.push_back(o[0] == t + 1 ? o[1] : o[0]); }
+    }
+    if (nphys != 1)
+      throw catn::Error(-2, "braket_network: tensor " + std::to_string(t + 1) + " has " +
+                                std::to_string(nphys) + " physical (open) labels, expected 1");
+    const int deg = nd - 1;
+    if (deg < 1) throw catn::Error(-2, "braket_node: degree-0 site (no virtual legs) is not supported.");
+    const int64_t* tdims = dims + di;
+    int64_t total = 1;
+    for (int k = 0; k < nd; ++k) total *= tdims[k];
+    // column-major permutation: out axes = perm, out[i_out] = in[i_in]
+    auto permuted = [&](const std::vector<int>& perm) {
+      std::vector<double> outv((size_t)total);
+      std::vector<int64_t> stride(nd, 1);
+      for (int k = 1; k < nd; ++k) stride[k] = stride[k - 1] * tdims[k - 1];
+      std::vector<int64_t> idx(nd, 0);
+      for (int64_t q = 0; q < total; ++q) {
+        int64_t src = 0;
+        for (int k = 0; k < nd; ++k) src += idx[k] * stride[perm[k]];
+        outv[(size_t)q] = data[dd + (size_t)src];
+        for (int k = 0; k < nd; ++k) {   // next output multi-index
+          if (++idx[k] < tdims[perm[k]]) break;
+          idx[k] = 0;
+        }
+      }
+      return outv;
+    };
+    std::vector<int> pk = virt, pb;
+    pk.push_back(phys_pos);
+    pb.push_back(phys_pos);
+    pb.insert(pb.end(), virt.begin(), virt.end());
+    const std::vector<double> tk = permuted(pk);   // (D1, ..., D_deg, d)
+    const std::vector<double> tb = permuted(pb);   // (d, D1, ..., D_deg): conj of a real tensor
+    const int d = (int)tdims[phys_pos];
+    catn::PNode node;
+    // ket chain (:116-130): splits over D1..D_{deg-1}; the remainder becomes the
+    // last ket site (chi, D_deg, d)
+    const int krem = net->new_slot(1, (int)total, 1, tk.data());
+    for (int k = 0; k < deg - 1; ++k) {
+      const int s2 = net->new_slot(0, 0, 0, nullptr);
+      pl.emit(catn::OP_RAW_SPLIT, krem, s2, (int)tdims[virt[k]], pl.p.chi, 0, 0, pl.p.cutoff);
+      node.slot.push_back(s2);
+    }
+    pl.emit(catn::OP_RESHAPE, krem, 0, 0, (int)tdims[virt[deg - 1]]);
+    node.slot.push_back(krem);
+    // bra chain (:155-172): the first bra site's left bond is d
+    const int brem = net->new_slot(d, (int)(total / d), 1, tb.data());
+    for (int k = 0; k < deg - 1; ++k) {
+      const int s2 = net->new_slot(0, 0, 0, nullptr);
+      pl.emit(catn::OP_RAW_SPLIT, brem, s2, (int)tdims[virt[k]], pl.p.chi, 0, 0, pl.p.cutoff);
+      node.slot.push_back(s2);
+    }
+    node.slot.push_back(brem);
+    for (int side = 0; side < 2; ++side)
+      for (int k = 0; k < deg; ++k) {
+        node.nb.push_back(kn[k]);
+        node.phys.push_back((int)tdims[virt[k]]);
+        node.layer.push_back(side == 0 ? 1 : 0);
+      }
+    node.cano = 1;
+    pl.nodes[t + 1] = node;
+    pl.left_canonical(pl.nodes[t + 1]);   // :183
+    di += nd;
+    dd += (size_t)total;
+    li += nd;
+  }
+  pl.braket = true;
+  pl.bk_edges = edges;
+  pl.maxdim_intermediate = 0;
+  net->n = ntensors;
+  net->beta = 1.0;
+  if (!pl.prog.empty()) catn::run_one(net.get());
+  *out = net.release();
+  return 0;
+  CATN_CATCH
+}
+
 int catn_net_create(catn_ctx* ctx, const catn_params* p, catn_net** out) {
   CATN_TRY
   std::unique_ptr<catn_net> net(new catn_net);

@@ -2138,6 +2138,16 @@ CATN_DN void op_permute321(Exec& E, const Op& op) {
   g.sync();
 }
 
+// a column-major (dl, d, dr) block is also (dl, D, d * dr / D), D = op.d: descriptor
+// only (no tape record: the data, and so its adjoint, stay at the same offset)
+CATN_DN void op_reshape(Exec& E, const Op& op) {
+  const SiteDesc a = E.site(op.a);
+  const int64_t dd = (int64_t)a.d * a.dr;
+  if (op.d <= 0 || dd % op.d != 0) { E.status = STATUS_SHAPE; return; }
+  E.set_site(op.a, a.off, a.dl, op.d, (int)(dd / op.d));
+  E.g.sync();
+}
+
 CATN_DN void op_eat_a(Exec& E, const Op& op) {
   const WG& g = E.g;
   SiteDesc a = E.site(op.a), b = E.site(op.b);
@@ -2600,7 +2610,8 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
       const int nin = (op.code == OP_EAT_B || op.code == OP_EAT_C) ? 3
                       : (op.code == OP_PERMUTE321 || op.code == OP_NORMALIZE ||
                          op.code == OP_RAW_SPLIT || op.code == OP_SCALAR_LOGNORM ||
-                         op.code == OP_TSVD) ? 1 : 2;
+                         op.code == OP_TSVD) ? 1
+                      : op.code == OP_RESHAPE ? 0 : 2;
       for (int q = 0; q < nin && E.status == STATUS_OK; ++q) {
         const SiteDesc d = E.op_desc[q];
         if (E.op_slot[q] >= 0) E.convert_in(d.off, (int64_t)d.dl * d.d * d.dr);
@@ -2627,6 +2638,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
       case OP_RAW_SPLIT: op_raw_split(E, op); break;
       case OP_SCALAR_LOGNORM: op_scalar_lognorm(E, op); break;
       case OP_TSVD: op_tsvd(E, op); break;
+      case OP_RESHAPE: op_reshape(E, op); break;
       default: break;
     }
     if (E.sc.fail && E.status == STATUS_OK) E.status = STATUS_SCRATCH_FULL;

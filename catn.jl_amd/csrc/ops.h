@@ -61,6 +61,10 @@ enum OpCode : int32_t {
   // result e (src/linalg_utils.jl:15-28); d = maxdim. With g = 1 the
   // randomized SVD rsvd(A, d, h, i) instead (src/linalg_utils.jl:44-58).
   OP_TSVD = 15,
+  // reinterpret site a (dl, d, dr) as (dl, D, d * dr / D), D = d field, without moving data:
+  // the last ket site of braket_node, (chi, D_deg * d_phys, 1) ->
+  // (chi, D_deg, d_phys) (src/braket.jl:130)
+  OP_RESHAPE = 16,
 };
 
 struct Op {

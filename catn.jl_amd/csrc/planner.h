@@ -49,10 +49,11 @@ struct PNode {
   std::vector<int> slot;   // site slot per MPS position
   std::vector<int> nb;     // neighbour id per MPS position (<0: open leg)
   std::vector<int> phys;   // physical dim per MPS position
+  std::vector<char> layer; // BraKetNode only: 1 ket leg, 0 bra leg (src/braket.jl:23)
   int cano = 0;            // 1-based canonical centre (reference convention)
   int order() const { return (int)slot.size(); }
   bool operator==(const PNode& o) const {
-    return cano == o.cano && slot == o.slot && nb == o.nb && phys == o.phys;
+    return cano == o.cano && slot == o.slot && nb == o.nb && phys == o.phys && layer == o.layer;
   }
 };
 
@@ -146,6 +147,7 @@ class Planner {
     std::swap(n.phys[lo - 1], n.phys[hi - 1]);
     n.cano = j;
     std::swap(n.nb[i - 1], n.nb[j - 1]);
+    if (!n.layer.empty()) std::swap(n.layer[i - 1], n.layer[j - 1]);   // src/braket.jl:561
   }
 
   void move(PNode& n, int a, int b) {   // src/mps_node.jl:186-199
@@ -413,8 +415,111 @@ class Planner {
   enum StepResult { NEED_CUT = 1, DONE = 2 };
 
   void start() {
-    if ((p.select == 0 || p.select == 1) && edge_count.empty()) select_edge_init();
+    if (!braket && (p.select == 0 || p.select == 1) && edge_count.empty()) select_edge_init();
     state_ = ST_EDGE;
+  }
+
+  // ---- src/braket.jl: the double-layer network ----------------------------
+  bool braket = false;
+  std::vector<std::pair<int, int>> bk_edges;   // paired edges in contraction order
+
+  static int find_leg(const PNode& n, int j, bool isket) {   // :287-294
+    for (int k = 0; k < n.order(); ++k)
+      if (n.nb[k] == j && (n.layer[k] != 0) == isket) return k + 1;
+    return 0;
+  }
+
+  // fuse MPS positions q, q+1 (1-based): ein"ijk,kab->ijab" reshaped (dl, d1*d2, dr2)
+  void fuse_pair(PNode& n, int q) {
+    emit(OP_MERGE_FUSE, n.slot[q - 1], n.slot[q]);
+    n.phys[q - 1] *= n.phys[q];
+    n.slot.erase(n.slot.begin() + q);
+    n.phys.erase(n.phys.begin() + q);
+    n.nb.erase(n.nb.begin() + q);
+    n.layer.erase(n.layer.begin() + q);
+    if (n.cano > q) n.cano -= 1;
+  }
+
+  // eat! of the paired (ket, bra) legs of a bond (:638-798). After the two
+  // pairs are fused, the four cases are the MPS eat! folds: A = case (a),
+  // C = case (b), D = case (c) without moving j's centre to its head.
+  void braket_eat(PNode& ni, PNode& nj, int j_id, int i_id) {
+    const int n_i = ni.order(), n_j = nj.order();
+    const int bi = find_leg(ni, j_id, false);
+    if (bi <= 0) throw PlanError("eat!: bra leg to neighbor " + std::to_string(j_id) + " not found in node_i");
+    move(ni, bi, n_i);
+    const int ki = find_leg(ni, j_id, true);
+    if (ki <= 0) throw PlanError("eat!: ket leg to neighbor " + std::to_string(j_id) + " not found in node_i");
+    move(ni, ki, n_i - 1);
+    cano_to(ni, n_i);
+    const int kj = find_leg(nj, i_id, true);
+    if (kj <= 0) throw PlanError("eat!: ket leg to neighbor " + std::to_string(i_id) + " not found in node_j");
+    move(nj, kj, 1);
+    const int bj = find_leg(nj, i_id, false);
+    if (bj <= 0) throw PlanError("eat!: bra leg to neighbor " + std::to_string(i_id) + " not found in node_j");
+    move(nj, bj, 2);
+    fuse_pair(ni, n_i - 1);   // W_i (dl, Dk Db, 1)
+    fuse_pair(nj, 1);         // W_j (1, Dk Db, dr)
+    if (n_i == 2 && n_j == 2) {                  // case A: scalar, phase z / |z|
+      emit(OP_EAT_A, ni.slot[0], nj.slot[0], 0, 0, 0, 0, p.cutoff);
+      ni.slot.clear(); ni.phys.clear(); ni.nb.clear(); ni.layer.clear();
+      return;
+    }
+    if (n_i == 2) throw PlanError("eat!: case B (order(i) < order(j)) is not reached by contraction!");
+    const int L = ni.order();
+    if (n_j == 2) {                              // case C: fold, normalise the folded site
+      emit(OP_EAT_B, ni.slot[L - 2], ni.slot[L - 1], nj.slot[0]);
+      emit(OP_NORMALIZE, ni.slot[L - 2], 0, 0, p.norm_method, 0, 0, p.cutoff);
+      ni.slot.pop_back(); ni.phys.pop_back(); ni.nb.pop_back(); ni.layer.pop_back();
+      ni.cano = ni.order();
+      return;
+    }
+    emit(OP_EAT_C, ni.slot[L - 2], ni.slot[L - 1], nj.slot[0]);   // case D
+    ni.slot.pop_back(); ni.phys.pop_back(); ni.nb.pop_back(); ni.layer.pop_back();
+    ni.cano = ni.order();
+    ni.slot.insert(ni.slot.end(), nj.slot.begin() + 1, nj.slot.end());
+    ni.phys.insert(ni.phys.end(), nj.phys.begin() + 1, nj.phys.end());
+    ni.nb.insert(ni.nb.end(), nj.nb.begin() + 1, nj.nb.end());
+    ni.layer.insert(ni.layer.end(), nj.layer.begin() + 1, nj.layer.end());
+    cano_to(ni, ni.order());
+    emit(OP_NORMALIZE, ni.slot[ni.cano - 1], 0, 0, p.norm_method, 0, 0, p.cutoff);
+  }
+
+  // contraction!(bk) (:960-1078): paired edges in order, compress after
+  // every eat!, then braket_lognorm (:915-950, ascending node id)
+  StepResult braket_step() {
+    while (!bk_edges.empty()) {
+      int i = bk_edges[0].first, j = bk_edges[0].second;
+      if (node(j).order() > node(i).order()) std::swap(i, j);
+      edge_log.push_back({i, j});
+      acc_err = true;
+      braket_eat(node(i), node(j), j, i);
+      PNode& ni = node(i);
+      compress_node(ni, p.svdopt);
+      for (int d : ni.phys) maxdim_intermediate = std::max(maxdim_intermediate, d);
+      std::vector<std::pair<int, int>> rest;
+      for (const auto& e : bk_edges) {
+        if ((e.first == i && e.second == j) || (e.first == j && e.second == i)) continue;
+        if (e.first == j || e.second == j) {   // re-point j's bonds to i (:1020-1071)
+          const int other = e.first == j ? e.second : e.first;
+          for (int& v : node(other).nb)
+            if (v == j) v = i;
+          rest.push_back(e.first == j ? std::make_pair(i, other) : std::make_pair(other, i));
+        } else {
+          rest.push_back(e);
+        }
+      }
+      bk_edges.swap(rest);
+      nodes.erase(j);
+    }
+    for (auto& kv : nodes) {
+      PNode& n = kv.second;
+      if (n.order() == 0) continue;
+      cano_to(n, n.order());
+      emit(OP_NORMALIZE, n.slot[n.cano - 1], 0, 0, p.norm_method, 0, 0, p.cutoff);
+    }
+    state_ = ST_DONE;
+    return DONE;
   }
 
   // Planning is a deterministic function of the planner state (and of the
@@ -427,6 +532,8 @@ class Planner {
     m.add((uint64_t)p.norm_method); m.add((uint64_t)state_); m.add((uint64_t)cur_i_);
     m.add((uint64_t)cur_j_); m.add((uint64_t)dup_pos_); m.add(dups_);
     m.add((uint64_t)nresults); m.add((uint64_t)(int64_t)maxdim_intermediate);
+    m.add((uint64_t)braket); m.add(bk_edges.size());
+    for (const auto& e : bk_edges) { m.add((uint64_t)e.first); m.add((uint64_t)e.second); }
     m.add(nodes.size());
     for (const auto& kv : nodes) {
       m.add((uint64_t)kv.first); m.add((uint64_t)kv.second.cano);
@@ -445,10 +552,12 @@ class Planner {
            cut_pending_ == o.cut_pending_ && nresults == o.nresults &&
            maxdim_intermediate == o.maxdim_intermediate && acc_err == o.acc_err &&
            std::equal(pending_cut_, pending_cut_ + 5, o.pending_cut_) && nodes == o.nodes &&
-           edge_count == o.edge_count && edge_log == o.edge_log && prog.size() == o.prog.size();
+           edge_count == o.edge_count && edge_log == o.edge_log && prog.size() == o.prog.size() &&
+           braket == o.braket && bk_edges == o.bk_edges;
   }
 
   StepResult step() {
+    if (braket) return state_ == ST_DONE ? DONE : braket_step();
     for (;;) {
       if (state_ == ST_DUPS) {
         if (cut_pending_) {

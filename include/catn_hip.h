@@ -75,6 +75,15 @@ int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
                         const int64_t* dims, const double* data, const int64_t* labels,
                         const catn_params* p, catn_net** out);
 
+/* braket_network(tensors, ixs; chi, cutoff, norm_method, svdopt, swapopt)
+ * (src/braket.jl:369-496): the double-layer network <psi|psi> of a ket state
+ * given like catn_tensor_network (real tensors; each has exactly one open,
+ * physical label; the virtual-bond graph must be acyclic). catn_contraction
+ * then runs contraction!(bk) (src/braket.jl:960-1078): lnZ = ln <psi|psi>. */
+int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
+                        const int64_t* dims, const double* data, const int64_t* labels,
+                        const catn_params* p, catn_net** out);
+
 /* Empty network holding hand-built MPS nodes (for the primitive API below). */
 int catn_net_create(catn_ctx* ctx, const catn_params* p, catn_net** out);
 /* MPSNode(mps, neighbor, cano, ...) (src/mps_node.jl:17-21): nsites sites,

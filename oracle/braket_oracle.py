@@ -175,6 +175,7 @@ class BraKetNetwork:
     svdopt: bool
     swapopt: bool
     maxdim_intermediate: int = 0
+    edge_log: List[tuple] = field(default_factory=list)   # (i, j) eaten, in order
 
 
 def braket_network(tensors, ixs, chi=64, cutoff=1e-15, norm_method=1, svdopt=True,
@@ -246,6 +247,7 @@ def contraction(bk):
         ni, nj = bk.tensors[i], bk.tensors[j]
         if len(nj.mps) > len(ni.mps):
             i, j, ni, nj = j, i, nj, ni
+        bk.edge_log.append((i, j))
         lg, e, ph = eat(ni, nj, j, i)
         err += e
         psi *= ph

@@ -130,3 +130,77 @@ def test_oracle_plants_a_mistake():
     T = _rand(rng, (3, 2, 4), True)
     node = B.braket_node(T.conj(), [1, 2], 2, chi=10_000)   # ket of conj(T): bra of T
     assert not np.allclose(O.mps2raw(node), _E(T, 1), atol=1e-6)
+
+
+# ----------------------------- device path ----------------------------------
+
+@pytest.fixture(scope="module", params=["emu", pytest.param("gpu", marks=pytest.mark.gpu)])
+def env(request, catn):
+    if request.param == "emu":
+        return catn, request.getfixturevalue("emu_ctx")
+    return catn, request.getfixturevalue("gpu_ctx")
+
+
+def _tree(rng, n, D, d):
+    """Random tree: node k > 1 hangs off a random earlier node."""
+    nb = {k: [] for k in range(1, n + 1)}
+    labels = {k: [] for k in range(1, n + 1)}
+    for k in range(2, n + 1):
+        par = int(rng.integers(1, k))
+        lab = f"e{par}_{k}"
+        labels[par].append(lab)
+        labels[k].append(lab)
+    ts, ixs = [], []
+    for k in range(1, n + 1):
+        ix = labels[k][:]
+        ix.insert(int(rng.integers(0, len(ix) + 1)), f"p{k}")
+        ts.append(rng.standard_normal([d if l.startswith("p") else D for l in ix]))
+        ixs.append(ix)
+    return ts, ixs
+
+
+@pytest.mark.parametrize("name", ["chain3_real", "star_real", "chain6_D3"])
+@pytest.mark.parametrize("svdopt", [True, False])
+def test_device_norm_matches_exact(env, name, svdopt):
+    catn, ctx = env
+    ts, ixs = _case(name)
+    ref = B.exact_norm(ts, ixs)
+    tn = catn.braket_network(ts, ixs, ctx=ctx, chi=10_000, svdopt=svdopt)
+    lnz, err, psi = catn.contraction(tn)
+    assert abs(math.exp(lnz) * psi - ref) <= 1e-10 * ref
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("n,D,chi,seed", [(8, 3, 10_000, 1), (10, 3, 6, 2), (12, 4, 8, 3),
+                                          (9, 2, 3, 4)])
+def test_device_matches_oracle_on_random_trees(env, n, D, chi, seed):
+    """Exact and truncated (chi below the exact bond) random trees: the same
+    edge sequence, maxdim_intermediate, lnZ (1e-12 relative) and discarded
+    weight (1e-8 relative) as the oracle."""
+    catn, ctx = env
+    ts, ixs = _tree(np.random.default_rng(seed), n, D, 2)
+    bk = B.braket_network(ts, ixs, chi=chi)
+    lo, eo, po = B.contraction(bk)
+    tn = catn.braket_network(ts, ixs, ctx=ctx, chi=chi)
+    lnz, err, psi = catn.contraction(tn)
+    assert [tuple(e) for e in tn.edge_log()] == bk.edge_log
+    assert tn.maxdim_intermediate == bk.maxdim_intermediate
+    assert abs(lnz - lo) <= 1e-12 * max(1.0, abs(lo))
+    assert abs(err - eo) <= 1e-8 * max(1e-12, eo) + 1e-14
+    assert psi == po == 1.0
+    if chi >= 10_000:
+        assert abs(lnz - math.log(B.exact_norm(ts, ixs))) < 1e-10
+
+
+def test_device_rejects_cycles_and_complex(env):
+    catn, ctx = env
+    rng = np.random.default_rng(0)
+    ts = [rng.standard_normal((2, 2, 2)) for _ in range(3)]
+    with pytest.raises(catn.CatnError):
+        catn.braket_network(ts, [["a", "b", "p1"], ["b", "c", "p2"], ["c", "a", "p3"]], ctx=ctx)
+    with pytest.raises(catn.CatnError):
+        catn.braket_network([rng.standard_normal((2, 2)) + 1j, rng.standard_normal((2, 2))],
+                            [["p1", "a"], ["a", "p2"]], ctx=ctx)
+    with pytest.raises(catn.CatnError):
+        catn.braket_network([rng.standard_normal((2, 2)), rng.standard_normal((2, 2))],
+                            [["a", "b"], ["a", "b"]], ctx=ctx)
