@@ -233,18 +233,24 @@ def op_profile(nets):
                                     "max": float(per_net.max())}}
 
 
-def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
-    n, edges, w, h = inst
+def run_plain(catn, ctx, insts, steps, warmup, profile_path="", tune=True):
+    """The plain form on len(insts) solves per step: the pinned networks of
+    every instance in one contraction_batch (value = solves per second)."""
+    n, edges, w, h = insts[0]
+    per = 1 + 2 * n + 4 * len(edges)
     times, res = [], None
     flops = bytes_ = 0.0
     tuned = None
+
+    def build():
+        nets = []
+        for (nk, ek, wk, hk) in insts:
+            nets += pinned_batch(catn, ctx, nk, ek, wk, hk)
+        _prefetch(nets)
+        return nets
     if tune:
-        # launch shape (lanes, LDS per workgroup) picked by timing this solve's
+        # launch shape (lanes, LDS per workgroup) picked by timing this step's
         # batch once per shape, before the warmup and the timed steps
-        def build():
-            nets = pinned_batch(catn, ctx, n, edges, w, h)
-            _prefetch(nets)
-            return nets
         tuned = catn.autotune(ctx, build)
         log("autotune: " + ", ".join(f"({r['threads']},{r['lds_doubles']},{r['panel']},"
                                      f"{r['jacobi_lanes']}) {r['seconds']:.3f} s"
@@ -253,15 +259,16 @@ def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
               f"{tuned['jacobi_lanes']})")
     plan0 = run0 = 0.0
     for step in range(warmup + steps):
-        nets = pinned_batch(catn, ctx, n, edges, w, h)
-        _prefetch(nets)
+        nets = build()
         timed = step >= warmup
         if step == warmup:
             ctx.kernel_stats(reset=True)
             plan0, run0 = ctx.host_stats()
         t0 = time.perf_counter()
         lz, _, _ = catn.contraction_batch(nets)
-        g = gradient_from_lnz(lz, n, edges)
+        grads = [gradient_from_lnz(lz[k * per:(k + 1) * per], ik[0], ik[1])
+                 for k, ik in enumerate(insts)]
+        g = grads[0]
         t1 = time.perf_counter()
         if timed:
             times.append(t1 - t0)
@@ -271,7 +278,7 @@ def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
                 bytes_ += b
             res = (lz.copy(), g)
         log(f"plain step {step} {'timed' if timed else 'warmup'}: {t1 - t0:.3f} s "
-            f"(801 pinned networks), lnZ/N={lz[0] / n:.12f}")
+            f"({len(insts)} solve(s), {len(nets)} pinned networks), lnZ/N={lz[0] / n:.12f}")
         if profile_path and step == warmup + steps - 1:
             with open(profile_path, "w") as f:
                 json.dump(op_profile(nets), f, indent=1)
@@ -283,7 +290,8 @@ def run_plain(catn, ctx, inst, steps, warmup, profile_path="", tune=True):
     # synchronisation, per step
     copies = (r1 - run0 - ksec) / max(steps, 1)
     return dict(times=times, lz=res[0], grad=res[1], launches=launches, seconds=ksec,
-                flops=flops, bytes=bytes_, tuned=tuned, plan_s=plan, copies_s=copies)
+                flops=flops, bytes=bytes_, tuned=tuned, plan_s=plan, copies_s=copies,
+                solves=len(insts))
 
 
 TRAFFIC_FILE = "exec_programs_sg100_plain_traffic.json"
@@ -296,7 +304,7 @@ def _pmc_rows(path, counter):
         return [r for r in csv.DictReader(f) if r["Counter_Name"] == counter]
 
 
-def summarize_pmc(paths, copy_mib, out_path):
+def summarize_pmc(paths, copy_mib, out_path, solves=2):
     """HBM traffic per exec_programs launch from rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE runs (KiB per dispatch), each scaled by what the same counter
     reads for a plain device copy of copy_mib MiB (the largest dispatch of the
@@ -312,8 +320,9 @@ def summarize_pmc(paths, copy_mib, out_path):
     n = len(fetch)
     per_launch = 1024.0 * (sf * sum(fetch) + sw * sum(write)) / max(n, 1)
     summary = {
-        "kernel": KERNEL_KEY, "workload": "plain form, spin glass n=100, Dmax=32, chi=256, FP64, "
-                                          "one solve (801 pinned networks)",
+        "kernel": KERNEL_KEY, "workload": f"plain form, spin glass n=100, Dmax=32, chi=256, FP64, "
+                                          f"one step of {solves} solve(s) ({801 * solves} pinned "
+                                          f"networks)",
         "dispatches": n,
         "fetch_kib_per_dispatch": [round(x, 1) for x in fetch],
         "write_kib_per_dispatch": [round(x, 1) for x in write],
@@ -344,6 +353,9 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="instances per chosen-form step")
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both")
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--plain-batch", type=int, default=2,
+                    help="solves per plain-form step (their pinned networks in one batch; 2 "
+                         "fills the GPU best: 1602 networks over 1024 workgroup slots)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cut-free-batch", type=int, default=128,
                     help="instances per chosen-form step of the cut-free comparison (0: skip it)")
@@ -364,7 +376,8 @@ def main():
                     help="traffic summary read into roofline.traffic (written by --summarize-pmc)")
     args = ap.parse_args()
     if args.summarize_pmc:
-        summarize_pmc(args.summarize_pmc, args.copy_kernel_mib or 1024, args.traffic)
+        summarize_pmc(args.summarize_pmc, args.copy_kernel_mib or 1024, args.traffic,
+                      args.plain_batch)
         return
     if args.copy_kernel_mib:
         import torch
@@ -393,7 +406,9 @@ def main():
     out_cfg = {
         "workload": f"spin glass n=100 random 3-regular (150 edges), w=+-1, h~N(0,0.1^2), "
                     f"beta=1.0, Dmax=32, chi=256, FP64; one solve = lnZ + 250 gradient entries; "
-                    f"chosen form: {args.batch} seeded instances per batch, forward + reverse sweep",
+                    f"chosen form: {args.batch} seeded instances per batch, forward + reverse sweep; "
+                    f"plain form: {args.plain_batch} seeded instance(s) per batch, 801 pinned "
+                    f"contractions each",
         "batch": args.batch,
     }
     chosen = plain = None
@@ -410,11 +425,11 @@ def main():
             if args.form == "chosen":
                 args.form = "plain"
         if refused and plain is None:
-            plain = run_plain(catn, ctx, instances[0], args.steps, min(args.warmup, 1),
-                              args.op_profile, not args.no_autotune)
+            plain = run_plain(catn, ctx, instances[:args.plain_batch], args.steps,
+                              min(args.warmup, 1), args.op_profile, not args.no_autotune)
     if args.form in ("both", "plain") and plain is None:
-        plain = run_plain(catn, ctx, instances[0], args.plain_steps, min(args.warmup, 1),
-                          args.op_profile, not args.no_autotune)
+        plain = run_plain(catn, ctx, instances[:args.plain_batch], args.plain_steps,
+                          min(args.warmup, 1), args.op_profile, not args.no_autotune)
 
     if refused and args.form == "both" and args.cut_free_batch > 0:
         out_cfg["cut_free_comparison"] = run_cut_free(catn, ctx, 2, 1, args.cut_free_batch)
@@ -453,7 +468,7 @@ def main():
             out_cfg["adjoint_vs_pinning_max_abs_dh"] = float(np.max(np.abs(g_adj[1] - g_pin[1])))
     else:
         total = sum(plain["times"])
-        value = len(plain["times"]) / total
+        value = plain["solves"] * len(plain["times"]) / total
         ms = 1e3 * total / len(plain["times"])
         form = "plain"
         fl = max(plain["launches"], 1)
@@ -461,8 +476,9 @@ def main():
         flops_per_launch = plain["flops"] / fl
         bytes_per_launch = plain["bytes"] / fl
     if plain is not None:
-        pv = len(plain["times"]) / sum(plain["times"])
+        pv = plain["solves"] * len(plain["times"]) / sum(plain["times"])
         out_cfg["plain_solves_per_s"] = pv
+        out_cfg["plain_solves_per_step"] = plain["solves"]
         out_cfg["plain_kernel"] = {"launches": plain["launches"], "seconds": plain["seconds"]}
         out_cfg["plain_host_planning_s_per_step"] = plain["plan_s"]
         out_cfg["plain_host_copies_and_launch_s_per_step"] = plain["copies_s"]
