@@ -105,3 +105,22 @@ def test_tile_parameters_lane_parallel(catn, ctxs, tiles):
     assert abs(res[0][0] - res[1][0]) / n < 1e-12
     assert abs(res[0][1] - res[1][1]) <= 1e-9 * max(1.0, abs(res[1][1]))
     assert _divergent() == before
+
+
+def test_braket_lane_parallel(catn, ctxs):
+    """The double-layer engine (construction splits, descriptor reshape,
+    paired-edge eat!, compress) at 256 lanes: same lnZ, error and edge
+    sequence as the one-thread build on a truncated random tree."""
+    from test_braket import _tree
+    mt, st = ctxs
+    ts, ixs = _tree(np.random.default_rng(4), 9, 2, 2)
+    before = _divergent()
+    res = []
+    for ctx in (mt, st):
+        tn = catn.braket_network(ts, ixs, ctx=ctx, chi=3)
+        lz, err, psi = catn.contraction(tn)
+        res.append((lz, err, tn.edge_log()))
+    assert _divergent() == before
+    assert res[0][2] == res[1][2]
+    assert abs(res[0][0] - res[1][0]) <= 1e-12 * abs(res[1][0])
+    assert abs(res[0][1] - res[1][1]) <= 1e-9 * abs(res[1][1])
