@@ -353,6 +353,9 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="instances per chosen-form step")
     ap.add_argument("--form", choices=("both", "chosen", "plain"), default="both")
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--dtype", choices=("f64", "f32x6", "f32"), default="f64",
+                    help="site storage of the plain form (f32x6: FP32 storage, 6-pass split-BF16 "
+                         "swap products)")
     ap.add_argument("--plain-batch", type=int, default=2,
                     help="solves per plain-form step (their pinned networks in one batch; 2 "
                          "fills the GPU best: 1602 networks over 1024 workgroup slots)")
@@ -375,6 +378,7 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", TRAFFIC_FILE),
                     help="traffic summary read into roofline.traffic (written by --summarize-pmc)")
     args = ap.parse_args()
+    NET_KW["dtype"] = args.dtype
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc, args.copy_kernel_mib or 1024, args.traffic,
                       args.plain_batch)
@@ -507,7 +511,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"f64": "f64", "f32x6": "f32 storage, f64 ops, bf16x6 swap GEMM",
+                  "f32": "f32 storage, f64 ops, bf16x3 swap GEMM"}[args.dtype],
         "data": "synthetic (seeded random 3-regular spin glasses)",
         "config": out_cfg,
         "roofline": {
