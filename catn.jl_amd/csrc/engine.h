@@ -766,12 +766,15 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
   // columns below 1e-18 ||A||_F are rounding noise of a rank-deficient input:
   // they cannot be made orthogonal (the column space is full) and only shrink
   // towards denormals, so they are treated as zero columns
-  double fpart = 0.0;
-  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
-    const double v = A[(idx % m) + (idx / m) * lda];
-    fpart += v * v;
+  double thr = thr_in;
+  if (thr < 0.0) {
+    double fpart = 0.0;
+    for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+      const double v = A[(idx % m) + (idx / m) * lda];
+      fpart += v * v;
+    }
+    thr = 1e-36 * wg_sum(g, fpart);
   }
-  const double thr = thr_in >= 0.0 ? thr_in : 1e-36 * wg_sum(g, fpart);
   for (int sweep = 1; sweep <= maxsweeps; ++sweep) {
     int rotated = 0;
     for (int r = 0; r < N - 1; ++r) {
@@ -862,6 +865,93 @@ CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double*
   const int sw = jacobi_core(g, A, m, n, lda, V, ldv, maxsweeps, -1.0, g.jac_lanes);
   if (g.tid == 0 && g.sub && sw > 0) g.sub[11] += sw;
   return sw;
+}
+
+// Block width of the LDS-staged Jacobi below for an m x n operand: the largest
+// b with a 2b-column slice (max(m, n) rows) plus its 2b x 2b rotation in the
+// LDS budget, or 0 when the blocking would not pay (blocks under 8 columns,
+// or more than 12 blocks: the barrier rounds grow with the block count).
+CATN_D int jacobi_block_width(const WG& g, int m, int n) {
+  const int64_t rows = m > n ? m : n;
+  int b = 0;
+  while (2 * (int64_t)(b + 1) * rows + 4 * (int64_t)(b + 1) * (b + 1) <= g.lds_cap) ++b;
+  if (b < 8 || b >= n) return 0;
+  return (n + b - 1) / b <= 12 ? b : 0;
+}
+
+// One-sided Jacobi of an operand in global memory by column blocks: the
+// columns are cut into nb blocks of b; the outer sweep walks the round-robin
+// schedule of block pairs, and each block pair (I, J) is loaded into LDS, given
+// one round-robin sweep over its |I| + |J| columns there (jacobi_core, which
+// accumulates the pair's rotations into a small matrix Q), and written back,
+// V's columns as V[:, I u J] Q. Every column pair is tested at least once per
+// outer sweep with the same threshold and cosine test as the plain sweep, so
+// an outer sweep with no rotation is the plain convergence test. A and V
+// cross HBM once per outer round (nb - 1 per sweep, nb - 1 + (nb & 1) with a
+// padding block) instead of once per column round (n - 1 per sweep).
+CATN_DN int wg_jacobi_blocked(const WG& g, double* A, int m, int n, int64_t lda, double* V,
+                              int64_t ldv, int maxsweeps, int b) {
+  for (int64_t idx = g.tid; idx < (int64_t)n * n; idx += g.nthr) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    V[i + (int64_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+  }
+  double fpart = 0.0;
+  for (int64_t idx = g.tid; idx < (int64_t)m * n; idx += g.nthr) {
+    const double v = A[(idx % m) + (idx / m) * lda];
+    fpart += v * v;
+  }
+  const double thr = 1e-36 * wg_sum(g, fpart);   // the plain sweep's noise threshold
+  const int rows = m > n ? m : n;
+  const int nb = (n + b - 1) / b, NB = nb + (nb & 1);
+  double* Xs = lds_base();                     // |I|+|J| columns of A, then of V (ld m / n)
+  double* Qs = Xs + (int64_t)rows * 2 * b;     // (|I|+|J|)^2 rotation, ld |I|+|J|
+  for (int sweep = 1; sweep <= maxsweeps; ++sweep) {
+    bool rotated = false;
+    for (int r = 0; r < NB - 1; ++r) {
+      for (int k = 0; k < NB / 2; ++k) {
+        int P, Q;
+        if (k == 0) { P = r; Q = NB - 1; }
+        else { P = (r + k) % (NB - 1); Q = (r - k + NB - 1) % (NB - 1); }
+        if (P > Q) { const int t = P; P = Q; Q = t; }
+        if (Q >= nb) continue;   // the padding block
+        const int i0 = P * b, wi = (n - i0) < b ? (n - i0) : b;
+        const int j0 = Q * b, wj = (n - j0) < b ? (n - j0) : b;
+        const int w = wi + wj;
+        for (int64_t idx = g.tid; idx < (int64_t)m * w; idx += g.nthr) {
+          const int i = (int)(idx % m), c = (int)(idx / m);
+          const int col = c < wi ? i0 + c : j0 + c - wi;
+          Xs[idx] = A[i + lda * (int64_t)col];
+        }
+        g.sync();
+        if (jacobi_core(g, Xs, m, w, m, Qs, w, 1, thr, 0) < 0) rotated = true;
+        for (int64_t idx = g.tid; idx < (int64_t)m * w; idx += g.nthr) {
+          const int i = (int)(idx % m), c = (int)(idx / m);
+          const int col = c < wi ? i0 + c : j0 + c - wi;
+          A[i + lda * (int64_t)col] = Xs[idx];
+        }
+        g.sync();
+        for (int64_t idx = g.tid; idx < (int64_t)n * w; idx += g.nthr) {
+          const int i = (int)(idx % n), c = (int)(idx / n);
+          const int col = c < wi ? i0 + c : j0 + c - wi;
+          Xs[idx] = V[i + ldv * (int64_t)col];
+        }
+        g.sync();
+        for (int64_t idx = g.tid; idx < (int64_t)n * w; idx += g.nthr) {
+          const int i = (int)(idx % n), c = (int)(idx / n);
+          const int col = c < wi ? i0 + c : j0 + c - wi;
+          double s = 0.0;
+          for (int q = 0; q < w; ++q) s += Xs[i + (int64_t)n * q] * Qs[q + (int64_t)w * c];
+          V[i + ldv * (int64_t)col] = s;
+        }
+        g.sync();
+      }
+    }
+    if (!rotated) {
+      if (g.tid == 0 && g.sub) g.sub[11] += sweep;
+      return sweep;
+    }
+  }
+  return -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1161,7 +1251,9 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
   // to R0 (n x n; B itself when square), a column-pivoted QR gives
   // R0 P = Q2 R, and the Jacobi sweeps run on R^T, whose graded columns
   // converge in a few sweeps: R^T Vj = U' S, so B = (Q1 Q2 Vj) S (P U')^T.
-  // The Jacobi runs in LDS when R and its rotations fit, else in global memory.
+  // The Jacobi runs in LDS when R and its rotations fit, else by column blocks
+  // staged in LDS (wg_jacobi_blocked), or column pair by column pair in global
+  // memory when blocking would not pay.
   const int64_t lds_direct = (int64_t)m * n + (int64_t)n * n + 2 * (int64_t)n;
   if (lds_direct <= g.lds_cap) return wg_svd_lds(g, sc, A, m, n, lda, transpose_in, out, sweeps_used);
   double* W = sc.alloc((int64_t)m * n);
@@ -1219,7 +1311,8 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
     double* key = sc.alloc(n);
     int* rk = (int*)sc.alloc(n);
     if (sc.fail) return -1;
-    sw = wg_jacobi(g, X, n, n, n, V, n, 60);
+    const int jb = jacobi_block_width(g, n, n);
+    sw = jb ? wg_jacobi_blocked(g, X, n, n, n, V, n, 60, jb) : wg_jacobi(g, X, n, n, n, V, n, 60);
     if (sweeps_used) *sweeps_used = sw;
     sub_tick(g, 1, t0);
     t0 = now_ticks();

@@ -60,6 +60,33 @@ def test_tsvd_spectrum_and_reconstruction(env, shape):
     assert np.allclose(V.T @ V, np.eye(V.shape[1]), atol=1e-12)
 
 
+@pytest.mark.parametrize("lds,shape,rank", [(5056, (100, 100), None), (5056, (400, 128), None),
+                                            (2048, (64, 40), None), (5056, (100, 100), 30),
+                                            (5056, (70, 90), None)])
+def test_tsvd_column_block_jacobi(env, lds, shape, rank):
+    """LDS budgets under which the Jacobi of the R factor runs by column blocks
+    staged in LDS: an even block count (n=100, b=18), an odd one with a padding
+    block (n=128, b=15), a ragged last block (n=40, b=14), a rank-deficient
+    operand (noise columns under the threshold) and a wide one (through its
+    transpose). Spectrum against LAPACK, reconstruction, orthogonality."""
+    catn, ctx = env
+    r = rng(7)
+    A = r.standard_normal(shape)
+    if rank:
+        A = r.standard_normal((shape[0], rank)) @ r.standard_normal((rank, shape[1]))
+    ctx.set_launch(256, lds, 0, 0)
+    try:
+        U, S, V, disc, Sfull = catn.tsvd(A, cutoff=0.0, ctx=ctx)
+    finally:
+        ctx.set_launch(256, 0, 0, 0)
+    ref = np.linalg.svd(A, compute_uv=False)
+    assert np.allclose(Sfull, ref, rtol=1e-12, atol=1e-12 * ref[0])
+    assert np.allclose(U @ np.diag(S) @ V.T, A, atol=1e-11 * ref[0])
+    k = rank or min(shape)
+    assert np.allclose(U[:, :k].T @ U[:, :k], np.eye(k), atol=1e-12)
+    assert np.allclose(V[:, :k].T @ V[:, :k], np.eye(k), atol=1e-12)
+
+
 def test_tsvd_truncation_rules(env):
     """test/test_linalg.jl:4-18 plus the keep-at-least-one rule."""
     catn, ctx = env

@@ -124,3 +124,25 @@ def test_braket_lane_parallel(catn, ctxs):
     assert res[0][2] == res[1][2]
     assert abs(res[0][0] - res[1][0]) <= 1e-12 * abs(res[1][0])
     assert abs(res[0][1] - res[1][1]) <= 1e-9 * abs(res[1][1])
+
+
+def test_column_block_jacobi_lane_parallel(catn, ctxs):
+    """The column-block Jacobi of a large SVD (LDS budget 5056 doubles, R
+    factor 100 x 100: six blocks of 18 columns staged in LDS) at 256 lanes:
+    the one-thread build's singular values and reconstruction to rounding, no
+    divergent barrier."""
+    mt, st = ctxs
+    A = np.random.default_rng(9).standard_normal((160, 100))
+    before = _divergent()
+    res = []
+    for ctx in (mt, st):
+        ctx.set_launch(256, 5056, 0, 0)
+        try:
+            res.append(catn.tsvd(A, cutoff=0.0, ctx=ctx))
+        finally:
+            ctx.set_launch(256, 0, 0, 0)
+    assert _divergent() == before
+    (U0, S0, V0, _, _), (U1, S1, V1, _, _) = res
+    assert np.allclose(S0, S1, rtol=1e-13, atol=1e-13 * S1[0])
+    assert np.allclose(U0 @ np.diag(S0) @ V0.T, A, atol=1e-11 * S1[0])
+    assert np.allclose(U0.T @ U0, np.eye(100), atol=1e-12)
