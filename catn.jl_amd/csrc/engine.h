@@ -578,20 +578,132 @@ struct IdxTheta {
 // ---------------------------------------------------------------------------
 // Blocked Householder QR (LAPACK dgeqrf structure): panels of g.qb columns are
 // factorised by wg_geqr2; the trailing matrix is updated with the compact WY
-// form Q_p^T C = C - V (T^T (V^T C)) through two tiled GEMMs, so the trailing
+// form Q_p^T C = C - V (T^T (V^T C)) through two skinny products, so the trailing
 // matrix is streamed twice per panel instead of twice per column. The panel
 // T factors are kept (T_all, kQBMax x n) for wg_apply_q_blocked.
 // ---------------------------------------------------------------------------
 
-// C (jb x N) = V^T B for jb <= 16 with the skinny tile (LDS budget permitting)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef double catn_d4 __attribute__((ext_vector_type(4)));
+
+// FP64 matrix-core forms of the two products of a compact-WY update, for
+// v_mfma_f64_16x16x4f64: a wavefront computes a 16 x 16 tile D += A B with
+// A 16 x 4 and B 4 x 16; lane l holds A(l % 16, l / 16), B(l / 16, l % 16) and
+// D(l / 16 + 4 v, l % 16) in accumulator element v (checked on gfx950). Every loop below has
+// wave-uniform bounds (the matrix instruction runs for the whole wavefront);
+// out-of-range operand entries are loaded as 0.
+
+// W (jb x N, ldw) = V^T C, V rows x jb (ldv), C rows x N (ldc): one wavefront
+// per 16 x 16 tile of W walking the rows 4 at a time, so no workgroup barrier
+// is needed inside. When the tiles are fewer than the wavefronts the rows are
+// split among them and the partial tiles are summed through LDS (the top
+// 256 doubles per wavefront of the free LDS region).
+CATN_D void wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
+                        const double* C, int64_t ldc, double* W, int64_t ldw) {
+  const int nw = g.nthr >> 6, wv = g.tid >> 6, l = g.tid & 63;
+  const int li = l & 15, lk = l >> 4;
+  const int qt = (jb + 15) >> 4, ct = (N + 15) >> 4, tasks = qt * ct;
+  const int ks = tasks >= nw ? 1 : nw / tasks;
+  const int chunk = ((rows + ks - 1) / ks + 3) & ~3;
+  double* P = lds_base() + (g.lds_cap - 256 * nw);
+  for (int t = wv; t < tasks * ks; t += nw) {
+    const int tile = t % tasks, part = t / tasks;
+    const int q0 = (tile % qt) * 16, c0 = (tile / qt) * 16;
+    const int r0 = part * chunk;
+    const int r1 = (r0 + chunk) < rows ? (r0 + chunk) : rows;
+    const int q = q0 + li, c = c0 + li;
+    const double* vp = V + ldv * (int64_t)(q < jb ? q : 0);
+    const double* cp = C + ldc * (int64_t)(c < N ? c : 0);
+    const bool qok = q < jb, cok = c < N;
+    catn_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = r0; k0 < r1; k0 += 4) {
+      const int k = k0 + lk;
+      const double a = (qok && k < r1) ? vp[k] : 0.0;
+      const double b = (cok && k < r1) ? cp[k] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    if (ks == 1) {
+      for (int v = 0; v < 4; ++v) {
+        const int qq = q0 + lk + 4 * v;
+        if (qq < jb && cok) W[qq + ldw * (int64_t)c] = acc[v];
+      }
+    } else {
+      for (int v = 0; v < 4; ++v) P[256 * t + 16 * (lk + 4 * v) + li] = acc[v];
+    }
+  }
+  if (ks > 1) {
+    g.sync();
+    for (int e = g.tid; e < tasks * 256; e += g.nthr) {
+      const int tile = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
+      const int qq = (tile % qt) * 16 + rr, c = (tile / qt) * 16 + cc;
+      double s = 0.0;
+      for (int part = 0; part < ks; ++part) s += P[256 * (part * tasks + tile) + 16 * rr + cc];
+      if (qq < jb && c < N) W[qq + ldw * (int64_t)c] = s;
+    }
+  }
+  g.sync();
+}
+
+// C (rows x N, ldc) -= V W2, V rows x jb (ldv), W2 jb x N (ldw), jb <= kQBMax:
+// one wavefront per 16 x 16 tile of C, jb / 4 matrix instructions per tile
+// accumulating onto the loaded tile with -V as the A operand.
+CATN_D void wg_c_minus_vw_mfma(const WG& g, int rows, int N, int jb, const double* V,
+                               int64_t ldv, const double* W2, int64_t ldw, double* C,
+                               int64_t ldc) {
+  constexpr int kS = kQBMax / 4;
+  const int nw = g.nthr >> 6, wv = g.tid >> 6, l = g.tid & 63;
+  const int li = l & 15, lk = l >> 4;
+  const int rt = (rows + 15) >> 4, ct = (N + 15) >> 4;
+  const int steps = (jb + 3) >> 2;
+  for (int t = wv; t < rt * ct; t += nw) {
+    const int r0 = (t % rt) * 16, c0 = (t / rt) * 16;
+    const int ra = r0 + li, c = c0 + li;
+    catn_d4 acc;
+    for (int v = 0; v < 4; ++v) {
+      const int r = r0 + lk + 4 * v;
+      acc[v] = (r < rows && c < N) ? C[r + ldc * (int64_t)c] : 0.0;
+    }
+    for (int st = 0; st < kS; ++st) {
+      if (st >= steps) break;
+      const int k = 4 * st + lk;
+      const double a = (ra < rows && k < jb) ? -V[ra + ldv * (int64_t)k] : 0.0;
+      const double b = (k < jb && c < N) ? W2[k + ldw * (int64_t)c] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    for (int v = 0; v < 4; ++v) {
+      const int r = r0 + lk + 4 * v;
+      if (r < rows && c < N) C[r + ldc * (int64_t)c] = acc[v];
+    }
+  }
+  g.sync();
+}
+#endif
+
+// C (jb x N) = V^T B: FP64 matrix cores on the device; on the host the skinny
+// LDS tile (budget permitting)
 CATN_DN void wg_gemm_skinny_tn(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
                                const double* B, int64_t ldb, double* C, int64_t ldc) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  wg_vtc_mfma(g, jb, N, rows, V, ldv, B, ldb, C, ldc);
+#else
   IdxPlain o;
   o.ldc = ldc;
   if (g.lds_cap >= 32 * (16 + 128) + 64)
     wg_gemm_tile<16, 128, 32>(g, jb, N, rows, 1.0, V, ldv, true, B, ldb, false, 0.0, C, o);
   else
     wg_gemm_tile<kTM, kTN, kTK>(g, jb, N, rows, 1.0, V, ldv, true, B, ldb, false, 0.0, C, o);
+#endif
+}
+
+// C (rows x N) -= V W2 with V rows x jb (jb <= kQBMax): FP64 matrix cores on
+// the device, the tiled GEMM on the host
+CATN_DN void wg_update_cvw(const WG& g, int rows, int N, int jb, const double* V, int64_t ldv,
+                           const double* W2, int64_t ldw, double* C, int64_t ldc) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  wg_c_minus_vw_mfma(g, rows, N, jb, V, ldv, W2, ldw, C, ldc);
+#else
+  wg_gemm(g, rows, N, jb, -1.0, V, ldv, false, W2, ldw, false, 1.0, C, ldc);
+#endif
 }
 
 // T (jb x jb, upper) of H_0..H_{jb-1} = I - V T V^T from the unit-lower V
@@ -692,7 +804,7 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
       // W2 = T^T W
       wg_gemm_nn_t(g, jb, nc, T, W, W2, true);
       // C -= V W2
-      wg_gemm(g, rows, nc, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Cp, lda);
+      wg_update_cvw(g, rows, nc, jb, V, rows, W2, jb, Cp, lda);
     }
     sub_tick(g, 9, t0);
   }
@@ -725,7 +837,7 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
     // Q_p X = X - V (T (V^T X))
     wg_gemm_skinny_tn(g, jb, k, rows, V, rows, Xp, ldx, W, jb);
     wg_gemm_nn_t(g, jb, k, T, W, W2, false);
-    wg_gemm(g, rows, k, jb, -1.0, V, rows, false, W2, jb, false, 1.0, Xp, ldx);
+    wg_update_cvw(g, rows, k, jb, V, rows, W2, jb, Xp, ldx);
   }
   sub_tick(g, 10, t0);
   sc.top = save;
