@@ -1633,6 +1633,42 @@ CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int
     }
     g.sync();
     // columns right of the block
+#if defined(__HIP_DEVICE_COMPILE__)
+    // on the FP64 matrix cores, one wavefront per 16-column tile: the tile
+    // T = R(I, i1:c) X(i1:c, tile) accumulates over K = i1 .. tile end (the
+    // diagonal stage of each block zeroed X below its diagonal, so the
+    // triangular K range is exact), then X(I, tile) = -X(I, I) T. The
+    // accumulator layout of T (lane l, element v: row l/16 + 4v, column l%16)
+    // is the B-operand layout of K-step v of the second product.
+    {
+      const int nw = g.nthr >> 6, wv = g.tid >> 6, l = g.tid & 63;
+      const int li = l & 15, lk = l >> 4;
+      for (int c0 = i1 + 16 * wv; c0 < n; c0 += 16 * nw) {
+        const int c = c0 + li;
+        const int kend = (c0 + 16) < n ? (c0 + 16) : n;
+        catn_d4 tt = {0.0, 0.0, 0.0, 0.0};
+        for (int q0 = i1; q0 < kend; q0 += 4) {
+          const int q = q0 + lk;
+          const double a = (li < bs && q < kend) ? R[(i0 + li) + (int64_t)q * ldr] : 0.0;
+          const double b = (q < kend && c < n) ? X[q + (int64_t)c * n] : 0.0;
+          tt = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, tt, 0, 0, 0);
+        }
+        catn_d4 xo = {0.0, 0.0, 0.0, 0.0};
+        for (int st = 0; st < 4; ++st) {
+          const int k = 4 * st + lk;
+          const double a = (li < bs && k < bs) ? -X[(i0 + li) + (int64_t)(i0 + k) * n] : 0.0;
+          xo = __builtin_amdgcn_mfma_f64_16x16x4f64(a, tt[st], xo, 0, 0, 0);
+        }
+        for (int v = 0; v < 4; ++v) {
+          const int r = lk + 4 * v;
+          if (r < bs && c < n) {
+            X[(i0 + r) + (int64_t)c * n] = xo[v];
+            part += xo[v] * xo[v];
+          }
+        }
+      }
+    }
+#else
     for (int c = i1 + g.tid; c < n; c += g.nthr) {
       double t[kInvB];
       for (int r = 0; r < bs; ++r) t[r] = 0.0;
@@ -1647,6 +1683,7 @@ CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int
         part += acc * acc;
       }
     }
+#endif
     g.sync();
   }
   const double tot = wg_sum(g, part);
