@@ -593,17 +593,32 @@ typedef double catn_d4 __attribute__((ext_vector_type(4)));
 // wave-uniform bounds (the matrix instruction runs for the whole wavefront);
 // out-of-range operand entries are loaded as 0.
 
-// W (jb x N, ldw) = V^T C, V rows x jb (ldv), C rows x N (ldc): one wavefront
+// entry (k, q) of a reflector panel: the stored value, or, for a unit-lower
+// panel read in place from the factored matrix, 1 on the diagonal and 0 above
+CATN_D double refl_at(const double* V, int64_t ldv, bool unit, int k, int q) {
+  if (unit && k <= q) return k == q ? 1.0 : 0.0;
+  return V[k + ldv * (int64_t)q];
+}
+
+// W (jb x N, ldw) = V^T C, V rows x jb (ldv), C rows x N (ldc), either of them
+// a unit-lower reflector panel read in place (vunit / cunit): one wavefront
 // per 16 x 16 tile of W walking the rows 4 at a time, so no workgroup barrier
 // is needed inside. When the tiles are fewer than the wavefronts the rows are
 // split among them and the partial tiles are summed through LDS (the top
 // 256 doubles per wavefront of the free LDS region).
-CATN_D void wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
-                        const double* C, int64_t ldc, double* W, int64_t ldw) {
+// With T given (jb <= 16, upper triangular, ld kQBMax) and the rows not split,
+// W = op(T) V^T C is formed in registers (op(T) = T^T when tT): the
+// accumulator layout of a tile (lane l, element v: row l/16 + 4v, column l%16)
+// is the B-operand layout of K-step v of the product with op(T). The return
+// value says whether T was applied.
+CATN_D bool wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
+                        bool vunit, const double* C, int64_t ldc, bool cunit, double* W,
+                        int64_t ldw, const double* T = nullptr, bool tT = false) {
   const int nw = g.nthr >> 6, wv = g.tid >> 6, l = g.tid & 63;
   const int li = l & 15, lk = l >> 4;
   const int qt = (jb + 15) >> 4, ct = (N + 15) >> 4, tasks = qt * ct;
   const int ks = tasks >= nw ? 1 : nw / tasks;
+  const bool fuse = T != nullptr && ks == 1 && jb <= 16;
   const int chunk = ((rows + ks - 1) / ks + 3) & ~3;
   double* P = lds_base() + (g.lds_cap - 256 * nw);
   for (int t = wv; t < tasks * ks; t += nw) {
@@ -612,15 +627,27 @@ CATN_D void wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, i
     const int r0 = part * chunk;
     const int r1 = (r0 + chunk) < rows ? (r0 + chunk) : rows;
     const int q = q0 + li, c = c0 + li;
-    const double* vp = V + ldv * (int64_t)(q < jb ? q : 0);
-    const double* cp = C + ldc * (int64_t)(c < N ? c : 0);
     const bool qok = q < jb, cok = c < N;
+    const int qs = qok ? q : 0, cs = cok ? c : 0;
     catn_d4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int k0 = r0; k0 < r1; k0 += 4) {
       const int k = k0 + lk;
-      const double a = (qok && k < r1) ? vp[k] : 0.0;
-      const double b = (cok && k < r1) ? cp[k] : 0.0;
+      const double a = (qok && k < r1) ? refl_at(V, ldv, vunit, k, qs) : 0.0;
+      const double b = (cok && k < r1) ? refl_at(C, ldc, cunit, k, cs) : 0.0;
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    if (fuse) {
+      catn_d4 o = {0.0, 0.0, 0.0, 0.0};
+      for (int st = 0; st < 4; ++st) {
+        const int kk = 4 * st + lk;
+        double a = 0.0;
+        if (li < jb && kk < jb) {
+          if (tT) a = kk <= li ? T[kk + kQBMax * li] : 0.0;   // T^T(li, kk) = T(kk, li)
+          else a = li <= kk ? T[li + kQBMax * kk] : 0.0;
+        }
+        o = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[st], o, 0, 0, 0);
+      }
+      acc = o;
     }
     if (ks == 1) {
       for (int v = 0; v < 4; ++v) {
@@ -642,14 +669,16 @@ CATN_D void wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, i
     }
   }
   g.sync();
+  return fuse;
 }
 
-// C (rows x N, ldc) -= V W2, V rows x jb (ldv), W2 jb x N (ldw), jb <= kQBMax:
-// one wavefront per 16 x 16 tile of C, jb / 4 matrix instructions per tile
-// accumulating onto the loaded tile with -V as the A operand.
+// C (rows x N, ldc) -= V W2, V rows x jb (ldv; unit-lower read in place when
+// vunit), W2 jb x N (ldw), jb <= kQBMax: one wavefront per 16 x 16 tile of C,
+// jb / 4 matrix instructions per tile accumulating onto the loaded tile with
+// -V as the A operand.
 CATN_D void wg_c_minus_vw_mfma(const WG& g, int rows, int N, int jb, const double* V,
-                               int64_t ldv, const double* W2, int64_t ldw, double* C,
-                               int64_t ldc) {
+                               int64_t ldv, bool vunit, const double* W2, int64_t ldw,
+                               double* C, int64_t ldc) {
   constexpr int kS = kQBMax / 4;
   const int nw = g.nthr >> 6, wv = g.tid >> 6, l = g.tid & 63;
   const int li = l & 15, lk = l >> 4;
@@ -666,7 +695,7 @@ CATN_D void wg_c_minus_vw_mfma(const WG& g, int rows, int N, int jb, const doubl
     for (int st = 0; st < kS; ++st) {
       if (st >= steps) break;
       const int k = 4 * st + lk;
-      const double a = (ra < rows && k < jb) ? -V[ra + ldv * (int64_t)k] : 0.0;
+      const double a = (ra < rows && k < jb) ? -refl_at(V, ldv, vunit, ra, k) : 0.0;
       const double b = (k < jb && c < N) ? W2[k + ldw * (int64_t)c] : 0.0;
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
@@ -684,7 +713,7 @@ CATN_D void wg_c_minus_vw_mfma(const WG& g, int rows, int N, int jb, const doubl
 CATN_DN void wg_gemm_skinny_tn(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
                                const double* B, int64_t ldb, double* C, int64_t ldc) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  wg_vtc_mfma(g, jb, N, rows, V, ldv, B, ldb, C, ldc);
+  wg_vtc_mfma(g, jb, N, rows, V, ldv, false, B, ldb, false, C, ldc);
 #else
   IdxPlain o;
   o.ldc = ldc;
@@ -700,18 +729,24 @@ CATN_DN void wg_gemm_skinny_tn(const WG& g, int jb, int N, int rows, const doubl
 CATN_DN void wg_update_cvw(const WG& g, int rows, int N, int jb, const double* V, int64_t ldv,
                            const double* W2, int64_t ldw, double* C, int64_t ldc) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  wg_c_minus_vw_mfma(g, rows, N, jb, V, ldv, W2, ldw, C, ldc);
+  wg_c_minus_vw_mfma(g, rows, N, jb, V, ldv, false, W2, ldw, C, ldc);
 #else
   wg_gemm(g, rows, N, jb, -1.0, V, ldv, false, W2, ldw, false, 1.0, C, ldc);
 #endif
 }
 
 // T (jb x jb, upper) of H_0..H_{jb-1} = I - V T V^T from the unit-lower V
-// (rows x jb, ld rows) and tau (LAPACK dlarft, forward columnwise).
-CATN_D void wg_larft(const WG& g, const double* V, int rows, int jb, const double* tau, double* T,
-                     double* Y) {
+// (rows x jb, ld ldv; read in place from the factored panel when vunit) and
+// tau (LAPACK dlarft, forward columnwise).
+CATN_D void wg_larft(const WG& g, const double* V, int64_t ldv, bool vunit, int rows, int jb,
+                     const double* tau, double* T, double* Y) {
   // Y = V^T V (jb x jb)
-  wg_gemm_skinny_tn(g, jb, jb, rows, V, rows, V, rows, Y, jb);
+#if defined(__HIP_DEVICE_COMPILE__)
+  wg_vtc_mfma(g, jb, jb, rows, V, ldv, vunit, V, ldv, vunit, Y, jb);
+#else
+  (void)vunit;
+  wg_gemm_skinny_tn(g, jb, jb, rows, V, ldv, V, ldv, Y, jb);
+#endif
   // T(j, i) = -tau_i sum_{l=j}^{i-1} T(j, l) Y(l, i): row j of T needs only its
   // own earlier entries, so lane j builds row j in registers.
   for (int j = g.tid; j < jb; j += g.nthr) {
@@ -743,6 +778,31 @@ CATN_D void wg_gemm_nn_t(const WG& g, int jb, int N, const double* T, const doub
     W2[idx] = s;
   }
   g.sync();
+}
+
+// C (rows x N, ldc) := (I - V op(T) V^T) C with op(T) = T^T when tT (the
+// trailing update of wg_geqrf, Q_p^T C) or T (wg_apply_q_blocked, Q_p C).
+// W, W2: jb x N scratch. On the device V is the factored panel read in place
+// (vunit) and op(T) is folded into the V^T C tiles when they are not split;
+// on the host V is the materialised unit-lower copy.
+CATN_DN void wg_wy_apply(const WG& g, int rows, int N, int jb, const double* V, int64_t ldv,
+                         bool vunit, const double* T, bool tT, double* C, int64_t ldc, double* W,
+                         double* W2) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const bool fuse = jb <= 16 && ((N + 15) >> 4) >= (g.nthr >> 6);
+  if (fuse) {
+    wg_vtc_mfma(g, jb, N, rows, V, ldv, vunit, C, ldc, false, W2, jb, T, tT);
+  } else {
+    wg_vtc_mfma(g, jb, N, rows, V, ldv, vunit, C, ldc, false, W, jb);
+    wg_gemm_nn_t(g, jb, N, T, W, W2, tT);
+  }
+  wg_c_minus_vw_mfma(g, rows, N, jb, V, ldv, vunit, W2, jb, C, ldc);
+#else
+  (void)vunit;
+  wg_gemm_skinny_tn(g, jb, N, rows, V, ldv, C, ldc, W, jb);
+  wg_gemm_nn_t(g, jb, N, T, W, W2, tT);
+  wg_update_cvw(g, rows, N, jb, V, ldv, W2, jb, C, ldc);
+#endif
 }
 
 // unit-lower copy of the panel's Householder vectors
@@ -793,19 +853,21 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     wg_geqr2_staged(g, Ap, rows, jb, lda, tau + p);
     sub_tick(g, 8, t0);
     t0 = now_ticks();
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double* Vp = Ap;   // reflectors read in place (unit lower)
+    const int64_t ldv = lda;
+    const bool vunit = true;
+#else
     wg_panel_v(g, Ap, rows, jb, lda, V);
+    const double* Vp = V;
+    const int64_t ldv = rows;
+    const bool vunit = false;
+#endif
     double* T = Tall + (int64_t)kQBMax * p;   // jb x jb block, ld kQBMax
-    wg_larft(g, V, rows, jb, tau + p, T, Y);
+    wg_larft(g, Vp, ldv, vunit, rows, jb, tau + p, T, Y);
     const int nc = n - p - jb;
-    if (nc > 0) {
-      double* Cp = Ap + lda * (int64_t)jb;
-      // W = V^T C   (jb x nc)
-      wg_gemm_skinny_tn(g, jb, nc, rows, V, rows, Cp, lda, W, jb);
-      // W2 = T^T W
-      wg_gemm_nn_t(g, jb, nc, T, W, W2, true);
-      // C -= V W2
-      wg_update_cvw(g, rows, nc, jb, V, rows, W2, jb, Cp, lda);
-    }
+    // C := Q_p^T C = C - V T^T (V^T C)
+    if (nc > 0) wg_wy_apply(g, rows, nc, jb, Vp, ldv, vunit, T, true, Ap + lda * (int64_t)jb, lda, W, W2);
     sub_tick(g, 9, t0);
   }
   return true;
@@ -831,13 +893,19 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
   for (int p = last; p >= 0; p -= qb) {
     const int jb = (n - p) < qb ? (n - p) : qb;
     const int rows = m - p;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double* Vp = A + p + lda * (int64_t)p;   // reflectors read in place (unit lower)
+    const int64_t ldv = lda;
+    const bool vunit = true;
+#else
     wg_panel_v(g, A + p + lda * (int64_t)p, rows, jb, lda, V);
+    const double* Vp = V;
+    const int64_t ldv = rows;
+    const bool vunit = false;
+#endif
     const double* T = Tall + (int64_t)kQBMax * p;
-    double* Xp = X + p;
     // Q_p X = X - V (T (V^T X))
-    wg_gemm_skinny_tn(g, jb, k, rows, V, rows, Xp, ldx, W, jb);
-    wg_gemm_nn_t(g, jb, k, T, W, W2, false);
-    wg_update_cvw(g, rows, k, jb, V, rows, W2, jb, Xp, ldx);
+    wg_wy_apply(g, rows, k, jb, Vp, ldv, vunit, T, false, X + p, ldx, W, W2);
   }
   sub_tick(g, 10, t0);
   sc.top = save;
