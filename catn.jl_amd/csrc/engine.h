@@ -92,6 +92,26 @@ CATN_D double wg_sum(const WG& g, double v) {
 #endif
 }
 
+// wg_sum with one barrier: the wavefront sums go to half `parity` of the
+// reduction slots (nthr <= 512). The caller alternates the parity and keeps a
+// workgroup barrier between the reads of one half and its next write.
+CATN_D double wg_sum_db(const WG& g, double v, int parity) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = g.tid >> 6, l = g.tid & 63;
+  double* r = g.red + 8 * (parity & 1);
+  if (l == 0) r[w] = v;
+  g.sync();
+  double s = 0.0;
+  const int nw = (g.nthr + 63) >> 6;
+  for (int i = 0; i < nw; ++i) s += r[i];
+  return s;
+#else
+  (void)parity;
+  return wg_sum(g, v);
+#endif
+}
+
 CATN_D double wg_max(const WG& g, double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -221,7 +241,10 @@ CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double*
       for (int i = 1 + g.tid; i < m - pend_col; i += g.nthr) xp[i] *= pend_scal;
       if (g.tid == 0) xp[0] = pend_beta;
     }
-    const double s = wg_sum(g, part);   // barriers: pending scaling now visible
+    // one barrier: the end-of-column barrier separates this column's reads of
+    // the reduction slots from the next writes; column 0 writes the half that
+    // no other reduction uses, so a caller's last wg_sum needs no barrier after
+    const double s = wg_sum_db(g, part, (j + 1) & 1);
     const double alpha = x[0];
     double t = 0.0, beta = alpha, scal = 0.0;
     if (s > 0.0) {
