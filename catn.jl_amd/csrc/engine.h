@@ -286,8 +286,12 @@ CATN_DN void wg_geqr2(const WG& g, double* A, int m, int n, int64_t lda, double*
 
 // U (m x k) := Q * U where Q = H_0 ... H_{n-1} from wg_geqr2 (stored in A);
 // U is given as its top n x k block followed by zeros (rows n..m-1 set here).
+// identity_in: U is [I; 0] with k = n (forming Q): H_j leaves the unit
+// columns c < j alone (their nonzero is above row j), so only the columns
+// c >= j are updated (LAPACK dorg2r).
 CATN_DN void wg_apply_q(const WG& g, const double* A, int m, int n, int64_t lda,
-                        const double* tau, double* U, int k, int64_t ldu) {
+                        const double* tau, double* U, int k, int64_t ldu,
+                        bool identity_in = false) {
   for (int64_t idx = g.tid; idx < (int64_t)(m - n) * k; idx += g.nthr) {
     const int i = n + (int)(idx % (m - n)), c = (int)(idx / (m - n));
     U[i + (int64_t)c * ldu] = 0.0;
@@ -300,7 +304,7 @@ CATN_DN void wg_apply_q(const WG& g, const double* A, int m, int n, int64_t lda,
     const double t = tau[j];
     if (t != 0.0) {
       const double* v = A + j + (int64_t)j * lda;   // v[0] implicit 1
-      for (int c = grp; c < k; c += ng) {
+      for (int c = (identity_in ? j : 0) + grp; c < k; c += ng) {
         double* y = U + j + (int64_t)c * ldu;
         double w = 0.0;
         for (int i = lane; i < m - j; i += L) w += (i == 0 ? 1.0 : v[i]) * y[i];
@@ -900,9 +904,9 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
 // wg_geqr2 (Tall null); X's rows n..m-1 must be set by the caller.
 CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m, int n,
                                 int64_t lda, const double* tau, const double* Tall, double* X,
-                                int k, int64_t ldx) {
+                                int k, int64_t ldx, bool identity_in = false) {
   if (!Tall) {
-    wg_apply_q(g, A, m, n, lda, tau, X, k, ldx);
+    wg_apply_q(g, A, m, n, lda, tau, X, k, ldx, identity_in);
     return;
   }
   const int64_t save = sc.top;
@@ -927,8 +931,13 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
     const bool vunit = false;
 #endif
     const double* T = Tall + (int64_t)kQBMax * p;
-    // Q_p X = X - V (T (V^T X))
-    wg_wy_apply(g, rows, k, jb, Vp, ldv, vunit, T, false, X + p, ldx, W, W2);
+    // Q_p X = X - V (T (V^T X)); from [I; 0] the columns left of the panel
+    // are still unit vectors above its rows, so only X(p:, p:) changes
+    // (LAPACK dorgqr)
+    if (identity_in)
+      wg_wy_apply(g, rows, k - p, jb, Vp, ldv, vunit, T, false, X + p + ldx * (int64_t)p, ldx, W, W2);
+    else
+      wg_wy_apply(g, rows, k, jb, Vp, ldv, vunit, T, false, X + p, ldx, W, W2);
   }
   sub_tick(g, 10, t0);
   sc.top = save;
@@ -1983,8 +1992,8 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
       Q[idx] = (i == j) ? 1.0 : 0.0;
     }
     g.sync();
-    if (in_lds) wg_apply_q(g, W, tm, r, tm, tau, Q, r, tm);
-    else wg_apply_q_blocked(g, sc, W, tm, r, tm, tau, blocked ? Tall : nullptr, Q, r, tm);
+    if (in_lds) wg_apply_q(g, W, tm, r, tm, tau, Q, r, tm, true);
+    else wg_apply_q_blocked(g, sc, W, tm, r, tm, tau, blocked ? Tall : nullptr, Q, r, tm, true);
     sub_tick(g, 6, t0);
   }
   if (iso_left && !trans) {          // A = Q R: L = Q (m x n, formed in place), Rt = R (n x n)
@@ -2635,7 +2644,7 @@ CATN_DN bool wg_thin_qr(const WG& g, Scratch& sc, double* Wk, int m, int n, doub
     Q[idx] = (i == j) ? 1.0 : 0.0;
   }
   g.sync();
-  wg_apply_q_blocked(g, sc, Wk, m, n, m, tau, blocked ? Tall : nullptr, Q, n, m);
+  wg_apply_q_blocked(g, sc, Wk, m, n, m, tau, blocked ? Tall : nullptr, Q, n, m, true);
   *Qo = Q;
   *Ro = R;
   return true;
