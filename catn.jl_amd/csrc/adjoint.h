@@ -642,13 +642,8 @@ CATN_DN void bwd_cut(const BwdCtx& c, const TapeRec& rc) {
   double* Mg = sc.alloc((int64_t)ri_rows * rj_rows);
   if (sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
   const int ldri = fl ? d : mi, ldrj = fr ? d : mj;
-  for (int64_t idx = g.tid; idx < (int64_t)ri_rows * rj_rows; idx += g.nthr) {
-    const int i = (int)(idx % ri_rows), j = (int)(idx / ri_rows);
-    double s = 0.0;
-    for (int q = 0; q < d; ++q) s += Ri[i + (int64_t)ldri * q] * Rj[j + (int64_t)ldrj * q];
-    Mg[idx] = s;
-  }
-  g.sync();
+  // the forward op_cut's product, same call
+  wg_gemm_any(g, ri_rows, rj_rows, d, 1.0, Ri, ldri, false, Rj, ldrj, true, 0.0, Mg, ri_rows);
   SvdResult S;
   if (wg_svd(g, sc, Mg, ri_rows, rj_rows, ri_rows, S, nullptr) < 0 || k > S.r) {
     E.status = sc.fail ? STATUS_SCRATCH_FULL : STATUS_REPLAY;

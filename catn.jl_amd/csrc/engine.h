@@ -2685,13 +2685,9 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
   double* Mg = E.sc.alloc((int64_t)ri_rows * rj_rows);
   if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
   const int ldri = fl ? d : mi, ldrj = fr ? d : mj;
-  for (int64_t idx = g.tid; idx < (int64_t)ri_rows * rj_rows; idx += g.nthr) {
-    const int i = (int)(idx % ri_rows), j = (int)(idx / ri_rows);
-    double s = 0.0;
-    for (int q = 0; q < d; ++q) s += Ri[i + (int64_t)ldri * q] * Rj[j + (int64_t)ldrj * q];
-    Mg[idx] = s;
-  }
-  g.sync();
+  // tiled (LDS-staged) when large enough, else one lane per entry; both sum
+  // q = 0 .. d-1 in order
+  wg_gemm_any(g, ri_rows, rj_rows, d, 1.0, Ri, ldri, false, Rj, ldrj, true, 0.0, Mg, ri_rows);
   SvdResult R;
   int sw = 0;
   if (wg_svd(g, E.sc, Mg, ri_rows, rj_rows, ri_rows, R, &sw) < 0) {
