@@ -2373,15 +2373,9 @@ CATN_DN void op_cano_l(Exec& E, const Op& op) {
     const int64_t row = idx / k;
     na[idx] = sp.L[row + (int64_t)m * p];
   }
-  // new b[u, p] = sum_c b[u, c] R[p, c]
-  const double* bp = E.ptr(b);
-  for (int64_t idx = g.tid; idx < rb * k; idx += g.nthr) {
-    const int64_t u = idx % rb;
-    const int p = (int)(idx / rb);
-    double s = 0.0;
-    for (int c = 0; c < n; ++c) s += bp[u + rb * c] * sp.R[p + (int64_t)k * c];
-    nbp[idx] = s;
-  }
+  // new b[u, p] = sum_c b[u, c] R[p, c]: (rb x n) (n x k)^T, summed over c in
+  // order (tiled when large enough)
+  wg_gemm_any(g, (int)rb, k, n, 1.0, E.ptr(b), rb, false, sp.R, k, true, 0.0, nbp, rb);
   E.work(2.0 * (double)rb * n * k, (double)m * n + (double)rb * n + (double)k * m + (double)rb * k);
   E.rec_begin(OP_CANO_L); E.rec_in(a); E.rec_in(b);
   E.rec_out(offa, k, d, dr); E.rec_out(offb, b.dl, b.d, k);
@@ -2790,13 +2784,7 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
     // mat = Rl * Rr^T   (rl x rr)
     M = E.sc.alloc((int64_t)rl * rr);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    for (int64_t idx = g.tid; idx < (int64_t)rl * rr; idx += g.nthr) {
-      const int i = (int)(idx % rl), j = (int)(idx / rl);
-      double s = 0.0;
-      for (int q = 0; q < dd; ++q) s += Rl[i + (int64_t)rl * q] * Rr[j + (int64_t)rr * q];
-      M[idx] = s;
-    }
-    g.sync();
+    wg_gemm_any(g, rl, rr, dd, 1.0, Rl, rl, false, Rr, rr, true, 0.0, M, rl);
     E.work((fl ? 2.0 * qr_flops(m, dd) : 0.0) + (fr ? 2.0 * qr_flops(n, dd) : 0.0) +
                2.0 * rl * rr * (double)dd, 0.0);
   }
@@ -2814,13 +2802,7 @@ CATN_DN void op_compress(Exec& E, const Op& op, bool opt) {
   if (fr) {
     Gr = E.sc.alloc((int64_t)k * n);
     if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
-    for (int64_t idx = g.tid; idx < (int64_t)k * n; idx += g.nthr) {
-      const int p = (int)(idx % k);
-      const int c = (int)(idx / k);
-      double s = 0.0;
-      for (int q = 0; q < dd; ++q) s += sp.R[p + (int64_t)k * q] * Qr[c + (int64_t)n * q];
-      Gr[idx] = s;
-    }
+    wg_gemm_any(g, k, n, dd, 1.0, sp.R, k, false, Qr, n, true, 0.0, Gr, k);
   }
   g.sync();
   const int64_t offa = E.alloc((int64_t)m * k);
