@@ -433,11 +433,60 @@ CATN_D void wg_gemm_tile(const WG& g, int M, int N, int K, double alpha, const d
   g.sync();
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef double catn_d4 __attribute__((ext_vector_type(4)));
+
+// The same product on the FP64 matrix cores (v_mfma_f64_16x16x4f64; lane l
+// holds A(l % 16, l / 16), B(l / 16, l % 16) and D(l / 16 + 4 v, l % 16) in
+// accumulator element v, checked on gfx950): one wavefront per 16 x 16 output
+// tile, K in steps of 4 with operands loaded straight from memory, so there is
+// no LDS staging and no workgroup barrier inside. Loop bounds are
+// wave-uniform; out-of-range operand entries are loaded as 0.
+template <class Out>
+CATN_D void wg_gemm_mfma(const WG& g, int M, int N, int K, double alpha, const double* A,
+                         int64_t lda, bool tA, const double* B, int64_t ldb, bool tB, double beta,
+                         double* C, Out out) {
+  const int nw = g.nthr >> 6, wv = g.tid >> 6, l = g.tid & 63;
+  const int li = l & 15, lk = l >> 4;
+  const int rt = (M + 15) >> 4, ct = (N + 15) >> 4;
+  for (int t = wv; t < rt * ct; t += nw) {
+    const int r0 = (t % rt) * 16, c0 = (t / rt) * 16;
+    const int ra = r0 + li, cb = c0 + li;
+    const bool rok = ra < M, cok = cb < N;
+    const double* ap = tA ? A + lda * (int64_t)(rok ? ra : 0) : A + (rok ? ra : 0);
+    const double* bp = tB ? B + (cok ? cb : 0) : B + ldb * (int64_t)(cok ? cb : 0);
+    const int64_t sa = tA ? 1 : lda, sb = tB ? ldb : 1;   // strides along k
+    catn_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + lk;
+      const double a = (rok && k < K) ? ap[sa * k] : 0.0;
+      const double b = (cok && k < K) ? bp[sb * k] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    for (int v = 0; v < 4; ++v) {
+      const int r = r0 + lk + 4 * v;
+      if (r < M && cok) {
+        const int64_t o = out(r, cb);
+        C[o] = beta == 0.0 ? alpha * acc[v] : alpha * acc[v] + beta * C[o];
+      }
+    }
+  }
+  g.sync();
+}
+#endif
+
+// general product: FP64 matrix cores on the device, the LDS-tiled FMA GEMM on
+// the host
 template <class Out>
 CATN_D void wg_gemm_t(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
                       bool tA, const double* B, int64_t ldb, bool tB, double beta, double* C,
                       Out out) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  wg_gemm_mfma(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, out);
+#else
   wg_gemm_tile<kTM, kTN, kTK>(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, out);
+#endif
 }
 
 CATN_DN void wg_gemm(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
@@ -611,8 +660,6 @@ struct IdxTheta {
 // ---------------------------------------------------------------------------
 
 #if defined(__HIP_DEVICE_COMPILE__)
-typedef double catn_d4 __attribute__((ext_vector_type(4)));
-
 // FP64 matrix-core forms of the two products of a compact-WY update, for
 // v_mfma_f64_16x16x4f64: a wavefront computes a 16 x 16 tile D += A B with
 // A 16 x 4 and B 4 x 16; lane l holds A(l % 16, l / 16), B(l / 16, l % 16) and
