@@ -154,8 +154,25 @@ CATN_D int64_t now_ticks() {
 #endif
 }
 
+// add to a profiling counter without waiting for it: on the device a global
+// atomic whose result is unused (no read-modify-write latency on the lane)
+CATN_D void count_add(int64_t* p, int64_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicAdd((unsigned long long*)p, (unsigned long long)v);
+#else
+  *p += v;
+#endif
+}
+CATN_D void count_add(int32_t* p, int32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicAdd((int*)p, (int)v);
+#else
+  *p += v;
+#endif
+}
+
 CATN_D void sub_tick(const WG& g, int k, int64_t t0) {
-  if (g.tid == 0 && g.sub) g.sub[k] += now_ticks() - t0;
+  if (g.tid == 0 && g.sub) count_add(g.sub + k, now_ticks() - t0);
 }
 
 CATN_D int size_bucket(int r) {
@@ -1122,7 +1139,7 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
 CATN_DN int wg_jacobi(const WG& g, double* A, int m, int n, int64_t lda, double* V,
                       int64_t ldv, int maxsweeps) {
   const int sw = jacobi_core(g, A, m, n, lda, V, ldv, maxsweeps, -1.0, g.jac_lanes);
-  if (g.tid == 0 && g.sub && sw > 0) g.sub[11] += sw;
+  if (g.tid == 0 && g.sub && sw > 0) count_add(g.sub + 11, sw);
   return sw;
 }
 
@@ -1206,7 +1223,7 @@ CATN_DN int wg_jacobi_blocked(const WG& g, double* A, int m, int n, int64_t lda,
       }
     }
     if (!rotated) {
-      if (g.tid == 0 && g.sub) g.sub[11] += sweep;
+      if (g.tid == 0 && g.sub) count_add(g.sub + 11, sweep);
       return sweep;
     }
   }
@@ -1594,7 +1611,7 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
     wg_apply_q_blocked(g, sc, W, m, n, m, tau, blocked ? Tall : nullptr, U, n, m);
   }
   sub_tick(g, 2, t0);
-  if (g.tid == 0 && g.sub && sw > 0) g.sub[3] += sw;
+  if (g.tid == 0 && g.sub && sw > 0) count_add(g.sub + 3, sw);
   out.r = n;
   out.U = U;
   out.S = Ss;
@@ -2297,7 +2314,6 @@ CATN_DN bool wg_split(Exec& E, const double* A, int m, int n, int64_t lda, bool 
     const double tm = m > n ? m : n;
     E.work(qr_flops(tm, r0) + r0 * (double)r0 * r0 / 3.0 + (iso_left != (m < n) ? qr_flops(tm, r0) : 0.0),
            0.0);
-    if (g.tid == 0 && g.sub) g.sub[3] += 0;   // (QR path: no Jacobi sweeps)
     return true;
   }
   if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return false; }
@@ -3016,11 +3032,11 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
     if (g.tid == 0) {
       const int64_t dt = now_ticks() - t0;
       const int c = op.code & 15;
-      st->op_ticks[c] += dt;
-      st->op_count[c] += 1;
+      count_add(st->op_ticks + c, dt);
+      count_add(st->op_count + c, 1);
       if (E.bucket >= 0) {
-        st->svd_ticks[E.bucket] += dt;
-        st->svd_count[E.bucket] += 1;
+        count_add(st->svd_ticks + E.bucket, dt);
+        count_add(st->svd_count + E.bucket, 1);
       }
     }
   }
