@@ -35,6 +35,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libcatn_hip.so")
 
 __all__ = [
     "Context", "TensorNetwork", "MPSNet", "ising_network", "tensor_network", "contraction",
+    "result_tensor",
     "contraction_batch", "free_energy", "magnetization", "correlation", "tsvd",
     "default_context", "LIB_PATH", "CatnError", "gradient_batch", "ising_gradient",
     "lnz_and_grad", "lnz_and_grad_batch", "rsvd",
@@ -110,6 +111,7 @@ _SIGS = {
     "catn_cut_bondim": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "catn_node_get": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32]),
     "catn_node_site": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_int64]),
+    "catn_result_tensor": (C.c_int, [_P, _DP, C.c_int64, _I32P, _I32P, _I32P, C.c_int32]),
     "catn_tsvd": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_double, C.c_int32, _I32P, _DP,
                             _DP, _DP, _DP]),
     "catn_rsvd": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_int32, C.c_int32, C.c_int32,
@@ -405,7 +407,14 @@ def tensor_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
     ctx._check(ctx.lib.catn_tensor_network(ctx.h, len(tensors), nd.ctypes.data_as(_I32P),
                                            dm.ctypes.data_as(_I64P), _dptr(dt),
                                            lb.ctypes.data_as(_I64P), C.byref(p), C.byref(out)))
-    return TensorNetwork(ctx, out)
+    tn = TensorNetwork(ctx, out)
+    # open labels in order of first appearance: the device's sentinels -1, -2, ...
+    count = {}
+    for ix in ixs:
+        for l in ix:
+            count[l] = count.get(l, 0) + 1
+    tn.open_labels = [l for l in lab_ids if count[l] == 1]
+    return tn
 
 
 TensorNetwork_ = tensor_network
@@ -453,6 +462,36 @@ def contraction(tn: TensorNetwork):
     tn.ctx._check(tn.ctx.lib.catn_contraction(tn.h, C.byref(lz), C.byref(err), C.byref(psi)))
     tn.lnZ, tn.psi = lz.value, psi.value
     return lz.value, err.value, psi.value
+
+
+def result_tensor(tn: TensorNetwork):
+    """result_tensor(tn) (src/tensor_network.jl:504-523) after contraction(tn):
+    (array, labels). array is the dense tensor of the one surviving open-leg
+    node (sites fused on the device), its legs in that node's MPS order, a
+    permutation of the open labels given in `labels`; the full open result is
+    array * exp(lnZ) * psi. A closed network gives a 0-d array holding 1.0 and
+    no labels. More than one open-leg node raises CatnError, as the reference
+    does. The network is consumed."""
+    kmax = 64
+    dims = np.zeros(kmax, dtype=np.int32)
+    labs = np.zeros(kmax, dtype=np.int32)
+    rank = C.c_int32()
+    # a query call (no output buffer) gives the rank, dims and labels
+    tn.ctx._check(tn.ctx.lib.catn_result_tensor(tn.h, None, 0, C.byref(rank),
+                                                dims.ctypes.data_as(_I32P),
+                                                labs.ctypes.data_as(_I32P), kmax))
+    if rank.value == 0:
+        return np.array(1.0), []
+    out = np.zeros(int(np.prod(dims[: rank.value].astype(np.int64))))
+    tn.ctx._check(tn.ctx.lib.catn_result_tensor(tn.h, _dptr(out), out.size, C.byref(rank),
+                                                dims.ctypes.data_as(_I32P),
+                                                labs.ctypes.data_as(_I32P), kmax))
+    r = rank.value
+    if r == 0:
+        return np.array(1.0), []
+    names = getattr(tn, "open_labels", None)
+    labels = [names[-int(v) - 1] if names is not None and v < 0 else int(v) for v in labs[:r]]
+    return out.reshape(tuple(int(x) for x in dims[:r]), order="F"), labels
 
 
 def contraction_batch(tns: Sequence[TensorNetwork]):

@@ -1323,6 +1323,62 @@ int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t ca
   CATN_CATCH
 }
 
+int catn_result_tensor(catn_net* net, double* out, int64_t cap, int32_t* rank, int32_t* dims,
+                       int32_t* labels, int32_t dims_cap) {
+  CATN_TRY
+  // open-leg nodes: those with a site whose physical dimension is not 1
+  // (src/tensor_network.jl:518-519), in ascending id
+  std::vector<int> open;
+  for (auto& kv : net->pl.nodes) {
+    bool all1 = true;
+    for (int d : kv.second.phys)
+      if (d != 1) { all1 = false; break; }
+    if (!all1) open.push_back(kv.first);
+  }
+  if (open.empty()) {   // closed network: 0-d one() (:520)
+    if (rank) *rank = 0;
+    if (!out) return 0;
+    if (cap < 1) throw catn::Error(-2, "result_tensor: output buffer too small");
+    out[0] = 1.0;
+    return 0;
+  }
+  if (open.size() != 1)   // :521
+    throw catn::Error(-2, "result_tensor: expected one surviving open-leg node, got " +
+                              std::to_string(open.size()) +
+                              " (disconnected/multi-open networks are unsupported)");
+  catn::PNode& nd = net->pl.node(open[0]);
+  const int r = nd.order();
+  if (r > dims_cap) throw catn::Error(-2, "result_tensor: dims buffer too small");
+  int64_t total = 1;
+  for (int k = 0; k < r; ++k) {
+    if (dims) dims[k] = nd.phys[k];
+    if (labels) labels[k] = nd.nb[k];
+    total *= nd.phys[k];
+  }
+  if (rank) *rank = r;
+  if (!out) return 0;   // query: rank, dims and labels only
+  if (total > cap) throw catn::Error(-2, "result_tensor: output buffer too small");
+  net->ensure_device();
+  if (r > 1) {   // mps2raw on the device: the sites fused into one
+    const double lz = net->hs.lnZ, er = net->hs.err, ps = net->hs.psi, sg = net->hs.sign;
+    net->pl.fuse_all(nd);
+    catn::run_one(net);
+    net->hs.lnZ = lz; net->hs.err = er; net->hs.psi = ps; net->hs.sign = sg;
+  }
+  const catn::SiteDesc d = net->read_desc(nd.slot[0]);
+  const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
+  if (cnt != total) throw catn::Error(-3, "result_tensor: fused site has the wrong size");
+  if (net->f32) {
+    std::vector<float> fb(cnt);
+    catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off, sizeof(float) * cnt);
+    for (int64_t q = 0; q < cnt; ++q) out[q] = fb[q];
+  } else {
+    catn::exec_of(net)->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
+  }
+  return 0;
+  CATN_CATCH
+}
+
 int catn_tsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, double cutoff, int32_t maxdim,
               int32_t* k, double* U, double* S, double* V, double* discarded) {
   CATN_TRY

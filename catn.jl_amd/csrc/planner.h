@@ -258,6 +258,22 @@ class Planner {
     cano_to(n, idx1);
   }
 
+  // result_tensor's mps2raw (src/tensor_network.jl:517-523, src/mps_node.jl
+  // mps2raw): the node's sites fused left to right into one site
+  // (1, d1*d2*...*dn, 1), first leg fastest; its legs' neighbour entries go with
+  // the fused site, so the caller reads them first
+  void fuse_all(PNode& n) {
+    while (n.order() > 1) {
+      emit(OP_MERGE_FUSE, n.slot[0], n.slot[1]);
+      n.phys[0] *= n.phys[1];
+      n.slot.erase(n.slot.begin() + 1);
+      n.phys.erase(n.phys.begin() + 1);
+      n.nb.erase(n.nb.begin() + 1);
+      if (!n.layer.empty()) n.layer.erase(n.layer.begin() + 1);
+    }
+    n.cano = 1;
+  }
+
   // compress! / compress_opt! (src/mps_node.jl:356-447)
   void compress_node(PNode& n, bool opt) {
     if (n.order() == 0) return;

@@ -199,6 +199,19 @@ int catn_node_get(catn_net* net, int32_t id, int32_t* order, int32_t* cano, int3
                   int32_t* dims, int32_t cap);
 /* Download site k (1-based) data (dl*d*dr doubles). */
 int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t cap);
+/* result_tensor(tn) (src/tensor_network.jl:504-523), after catn_contraction:
+ * the dense tensor of the one surviving node with an open leg (a site whose
+ * physical dimension is not 1), its legs in that node's MPS order, column-major
+ * with the first leg fastest; the sites are fused on the device. rank, dims[k]
+ * and labels[k] (the leg's neighbour entry: the negative sentinel that
+ * catn_tensor_network gave the open label, -1 for the first open label in
+ * order of appearance, -2 for the next, ...) describe it; dims_cap bounds
+ * dims and labels. The full open result is out * exp(lnZ) * psi. A closed
+ * network gives rank 0 and out[0] = 1. More than one open-leg node is an
+ * error, as in the reference. out = NULL only fills rank, dims and labels;
+ * otherwise the surviving node is consumed. */
+int catn_result_tensor(catn_net* net, double* out, int64_t cap, int32_t* rank, int32_t* dims,
+                       int32_t* labels, int32_t dims_cap);
 
 /* rsvd(A, k, oversample, power; rng) (src/linalg_utils.jl:44-58) on the device:
  * Gaussian sketch of p = min(n, oversample*k) columns drawn by a counter-based
