@@ -53,7 +53,7 @@ class _Params(C.Structure):
         ("svdopt", C.c_int32), ("swapopt", C.c_int32), ("compress", C.c_int32),
         ("cut_bond", C.c_int32), ("seed", C.c_int32), ("arena_elems", C.c_int64),
         ("scratch_elems", C.c_int64), ("rsvd_both", C.c_int32), ("rsvd_any", C.c_int32),
-        ("precision", C.c_int32),
+        ("precision", C.c_int32), ("loops", C.c_int32),
     ]
 
 
@@ -222,7 +222,8 @@ def default_context() -> Context:
 
 def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse=True,
             svdopt=True, swapopt=True, compress=False, cut_bond=False, seed=1,
-            arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000, dtype="f64"):
+            arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000, dtype="f64",
+            loops=False):
     p = _Params()
     ctx.lib.catn_params_default(C.byref(p))
     p.Dmax, p.chi, p.cutoff = int(Dmax), int(chi), float(cutoff)
@@ -236,6 +237,7 @@ def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse
         raise CatnError(f"dtype must be 'f64' (Float64), 'f32' (Float32 storage, split-BF16 "
                         f"3-pass theta) or 'f32x6' (6-pass theta), got {dtype!r}")
     p.precision = levels[dtype]
+    p.loops = int(bool(loops))
     return p
 
 
@@ -425,7 +427,10 @@ def braket_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
     """braket_network(tensors, ixs; chi, cutoff, norm_method, svdopt, swapopt)
     (src/braket.jl:369-496) for real tensors: the double-layer network of
     <psi|psi>. contraction() of the result returns (ln <psi|psi>, error, psi).
-    Tensors are taken with Julia (column-major) index order."""
+    Tensors are taken with Julia (column-major) index order. loops=True accepts
+    graphs with cycles, which the reference rejects: when an eat closes a
+    cycle, the doubled bonds to the shared neighbour are merged (ket and bra
+    legs separately, the older bond as the fast index)."""
     ctx = ctx or default_context()
     lab_ids = {}
     labels, ndims, dims, data = [], [], [], []

@@ -693,7 +693,7 @@ void catn_params_default(catn_params* p) {
   p->Dmax = 32; p->chi = 32; p->cutoff = 1e-15; p->norm_method = 1; p->select = 1;
   p->reverse = 1; p->svdopt = 1; p->swapopt = 1; p->compress = 0; p->cut_bond = 0;
   p->seed = 1; p->arena_elems = 0; p->scratch_elems = 0;
-  p->rsvd_both = 7000; p->rsvd_any = 20000; p->precision = 0;
+  p->rsvd_both = 7000; p->rsvd_any = 20000; p->precision = 0; p->loops = 0;
 }
 
 const char* catn_last_error(void) { return catn::g_last_error.c_str(); }
@@ -730,6 +730,7 @@ static void apply_params(catn_net* net, const catn_params* p) {
   q.seed = p->seed;
   q.rsvd_both = p->rsvd_both;
   q.rsvd_any = p->rsvd_any;
+  q.loops = p->loops != 0;
   if (p->precision < 0 || p->precision > 2)
     throw catn::Error(-2, "precision must be 0 (FP64), 1 (FP32, split-BF16 3-pass) or 2 (FP32, 6-pass)");
   net->f32 = p->precision != 0;
@@ -838,7 +839,9 @@ int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
     if (kv.second.size() > 2)
       throw catn::Error(-2, "braket_network: label " + std::to_string(kv.first) + " appears " +
                                 std::to_string(kv.second.size()) + " times (expected 1 or 2)");
-  // acyclic virtual-bond graph only (:402-420)
+  // acyclic virtual-bond graph only (:402-420), unless loops are enabled;
+  // two bonds between one pair of tensors are rejected either way
+  std::set<std::pair<int, int>> pairs;
   std::vector<int> parent(ntensors + 1);
   for (int t = 0; t <= ntensors; ++t) parent[t] = t;
   auto root = [&](int x) {
@@ -849,10 +852,15 @@ int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
   for (int64_t l : first_seen) {
     const auto& o = owners[l];
     if (o.size() != 2) continue;
+    const std::pair<int, int> pr = {std::min(o[0], o[1]), std::max(o[0], o[1])};
+    if (pairs.count(pr))
+      throw catn::Error(-2, "braket_network: tensors " + std::to_string(pr.first) + " and " +
+                                std::to_string(pr.second) + " share more than one bond");
+    pairs.insert(pr);
     const int a = root(o[0]), b = root(o[1]);
-    if (a == b)
+    if (a == b && !net->pl.p.loops)
       throw catn::Error(-2, "braket_network: virtual-bond graph contains a cycle - cyclic graphs "
-                            "are not supported");
+                            "are not supported (loops=1 enables them)");
     parent[a] = b;
     edges.push_back({o[0], o[1]});
   }

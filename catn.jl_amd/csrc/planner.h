@@ -43,6 +43,9 @@ struct Params {
   // rsvd_both, or either above rsvd_any
   int rsvd_both = 7000;
   int rsvd_any = 20000;
+  // double-layer networks on graphs with cycles (no reference counterpart: the
+  // reference rejects them, src/braket.jl:405-420)
+  bool loops = false;
 };
 
 struct PNode {
@@ -62,7 +65,7 @@ inline bool same_params(const Params& a, const Params& b) {
          a.norm_method == b.norm_method && a.select == b.select && a.reverse == b.reverse &&
          a.svdopt == b.svdopt && a.swapopt == b.swapopt && a.compress == b.compress &&
          a.cut_bond == b.cut_bond && a.seed == b.seed && a.rsvd_both == b.rsvd_both &&
-         a.rsvd_any == b.rsvd_any;
+         a.rsvd_any == b.rsvd_any && a.loops == b.loops;
 }
 
 // 64-bit mixing of a sequence of integers (splitmix64 finaliser per word)
@@ -503,6 +506,25 @@ class Planner {
 
   // contraction!(bk) (:960-1078): paired edges in order, compress after
   // every eat!, then braket_lognorm (:915-950, ascending node id)
+  // Loopy double layer (no reference counterpart): merge! (src/mps_node.jl:
+  // 595-622) of the two legs of one layer (ket or bra) that point to `target`,
+  // the layer list following the moves. cross: the leg that carried the bond
+  // to `target` before the eat comes second, and is moved in front, so that
+  // it is the fast index of the fused leg on both ends of the bond.
+  void braket_merge(PNode& n, int target, bool isket, bool cross) {
+    std::vector<int> pos;
+    for (int k = 0; k < n.order(); ++k)
+      if (n.nb[k] == target && (n.layer[k] != 0) == isket) pos.push_back(k + 1);
+    if (pos.size() != 2)
+      throw PlanError("braket merge: expected two " + std::string(isket ? "ket" : "bra") +
+                      " legs to node " + std::to_string(target));
+    const int idx1 = pos[0], idx2 = pos[1];
+    move(n, idx2, cross ? idx1 : idx1 + 1);
+    cano_to(n, idx1);
+    fuse_pair(n, idx1);
+    cano_to(n, idx1);
+  }
+
   StepResult braket_step() {
     while (!bk_edges.empty()) {
       int i = bk_edges[0].first, j = bk_edges[0].second;
@@ -514,19 +536,46 @@ class Planner {
       compress_node(ni, p.svdopt);
       for (int d : ni.phys) maxdim_intermediate = std::max(maxdim_intermediate, d);
       std::vector<std::pair<int, int>> rest;
+      // neighbours of j already bonded to i (a cycle closes): their legs to
+      // j, and i's legs to them, are merged below (loops only)
+      std::vector<std::pair<int, std::pair<bool, bool>>> dups;   // k, (cross ket, cross bra)
       for (const auto& e : bk_edges) {
         if ((e.first == i && e.second == j) || (e.first == j && e.second == i)) continue;
         if (e.first == j || e.second == j) {   // re-point j's bonds to i (:1020-1071)
           const int other = e.first == j ? e.second : e.first;
-          for (int& v : node(other).nb)
+          PNode& nk = node(other);
+          const int ki = find_leg(nk, i, true);
+          if (ki > 0) {
+            if (!p.loops)
+              throw PlanError("braket_network: a cycle closes at node " + std::to_string(other));
+            dups.push_back({other, {ki > find_leg(nk, j, true),
+                                    find_leg(nk, i, false) > find_leg(nk, j, false)}});
+          }
+          for (int& v : nk.nb)
             if (v == j) v = i;
-          rest.push_back(e.first == j ? std::make_pair(i, other) : std::make_pair(other, i));
+          const auto ne = e.first == j ? std::make_pair(i, other) : std::make_pair(other, i);
+          bool seen = false;   // the bond (i, other) is already listed when a cycle closes
+          for (const auto& r : rest)
+            if ((r.first == i && r.second == other) || (r.first == other && r.second == i)) seen = true;
+          for (const auto& r : bk_edges)
+            if ((r.first == i && r.second == other) || (r.first == other && r.second == i)) seen = true;
+          if (!seen) rest.push_back(ne);
         } else {
           rest.push_back(e);
         }
       }
       bk_edges.swap(rest);
       nodes.erase(j);
+      for (const auto& d : dups) {
+        PNode& nk = node(d.first);
+        braket_merge(nk, i, true, d.second.first);
+        braket_merge(nk, i, false, d.second.second);
+        PNode& ni2 = node(i);
+        braket_merge(ni2, d.first, true, false);
+        braket_merge(ni2, d.first, false, false);
+        for (int dd : nk.phys) maxdim_intermediate = std::max(maxdim_intermediate, dd);
+        for (int dd : ni2.phys) maxdim_intermediate = std::max(maxdim_intermediate, dd);
+      }
     }
     for (auto& kv : nodes) {
       PNode& n = kv.second;

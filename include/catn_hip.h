@@ -47,6 +47,13 @@ typedef struct catn_params {
    * products accumulated in FP32. 2: the same with a three-term BF16 split
    * (6 passes, FP32-accurate products). Log-norms accumulate in FP64. */
   int32_t precision;
+  /* catn_braket_network only: 1 accepts virtual-bond graphs with cycles (the
+   * reference rejects them, src/braket.jl:405-420). When an eat! closes a
+   * cycle, the two bonds to the shared neighbour are merged on both nodes, ket
+   * and bra legs separately, the older bond as the fast index (merge!,
+   * src/mps_node.jl:595-622). Two bonds between one pair of tensors are still
+   * rejected. */
+  int32_t loops;
 } catn_params;
 
 /* Fill `p` with the reference defaults (Dmax=32, chi=32, cutoff=1e-15, ...). */

@@ -165,6 +165,26 @@ def eat(node_i, node_j, j_id, i_id):
     return math.log(nrm), err, one
 
 
+def merge_pair(node, target, isket, cross):
+    """Loopy extension, no reference counterpart (src/braket.jl:405-420 rejects
+    cycles): merge! (src/mps_node.jl:595-622) of the two legs of one layer that
+    point to `target`, the layer list following the moves; cross moves the
+    second leg in front of the first, so that the bond older than the eat is
+    the fast index of the fused leg on both of its ends."""
+    pos = [k + 1 for k in range(len(node.mps)) if node.neighbor[k] == target and node.layer[k] == isket]
+    if len(pos) != 2:
+        raise ValueError("merge: expected two legs of one layer to the shared neighbour")
+    idx1, idx2 = pos
+    err = move(node, idx2, idx1 if cross else idx1 + 1)
+    del node.neighbor[idx1], node.layer[idx1]        # equal entries: keep one
+    O.cano_to(node, idx1)
+    a, b = node.mps[idx1 - 1], node.mps[idx1]
+    node.mps[idx1 - 1] = _r(np.einsum("ijk,kab->ijab", a, b), a.shape[0], -1, b.shape[2])
+    del node.mps[idx1]
+    O.cano_to(node, idx1)
+    return err
+
+
 @dataclass
 class BraKetNetwork:
     tensors: Dict[int, BraKetNode]
@@ -176,11 +196,14 @@ class BraKetNetwork:
     swapopt: bool
     maxdim_intermediate: int = 0
     edge_log: List[tuple] = field(default_factory=list)   # (i, j) eaten, in order
+    loops: bool = False
 
 
 def braket_network(tensors, ixs, chi=64, cutoff=1e-15, norm_method=1, svdopt=True,
-                   swapopt=True):
-    """src/braket.jl:369-496 (edge order: first appearance of the label)."""
+                   swapopt=True, loops=False):
+    """src/braket.jl:369-496 (edge order: first appearance of the label).
+    loops=True accepts cycles (no reference counterpart); two bonds between
+    one pair of tensors are rejected either way."""
     n = len(tensors)
     count, owners, order_ = {}, {}, []
     for i in range(n):
@@ -202,11 +225,15 @@ def braket_network(tensors, ixs, chi=64, cutoff=1e-15, norm_method=1, svdopt=Tru
             parent[x] = parent[parent[x]]
             x = parent[x]
         return x
+    pairs = set()
     for l in order_:
         if count[l] == 2:
             a, b = owners[l]
+            if (min(a, b), max(a, b)) in pairs:
+                raise ValueError("braket_network: two bonds between one pair of tensors")
+            pairs.add((min(a, b), max(a, b)))
             ra, rb = root(a), root(b)
-            if ra == rb:
+            if ra == rb and not loops:
                 raise ValueError("braket_network: virtual-bond graph contains a cycle")
             parent[ra] = rb
     nodes = {}
@@ -220,7 +247,7 @@ def braket_network(tensors, ixs, chi=64, cutoff=1e-15, norm_method=1, svdopt=Tru
                 kn.append(b if a == i + 1 else a)
         nodes[i + 1] = braket_node(tensors[i], kn, pp, chi, cutoff, norm_method, svdopt, swapopt)
     edges = [tuple(owners[l]) for l in order_ if count[l] == 2]
-    return BraKetNetwork(nodes, edges, chi, cutoff, norm_method, svdopt, swapopt)
+    return BraKetNetwork(nodes, edges, chi, cutoff, norm_method, svdopt, swapopt, loops=loops)
 
 
 def braket_lognorm(bk):
@@ -257,18 +284,34 @@ def contraction(bk):
             bk.maxdim_intermediate = max(bk.maxdim_intermediate, t.shape[1])
         edges = [x for x in edges if not (set(x) == {i, j})]
         # re-point j's edges (and the neighbour entries of j's other
-        # neighbours) to i (src/braket.jl:1020-1071)
-        new = []
+        # neighbours) to i (src/braket.jl:1020-1071); with loops, a neighbour
+        # already bonded to i closes a cycle: its bond is listed once and the
+        # doubled legs are merged on both ends below
+        new, dups = [], []
         for (a, b) in edges:
             if j in (a, b):
                 other = b if a == j else a
                 on = bk.tensors[other]
+                if find_leg(on, i, True) > 0:
+                    if not bk.loops:
+                        raise ValueError("braket contraction: a cycle closes")
+                    dups.append((other, find_leg(on, i, True) > find_leg(on, j, True),
+                                 find_leg(on, i, False) > find_leg(on, j, False)))
                 on.neighbor = [i if v == j else v for v in on.neighbor]
-                new.append((i, other) if a == j else (other, i))
+                if not any({i, other} == set(r) for r in edges):
+                    new.append((i, other) if a == j else (other, i))
             else:
                 new.append((a, b))
         edges = new
         del bk.tensors[j]
+        for (k, cross_ket, cross_bra) in dups:
+            nk = bk.tensors[k]
+            err += merge_pair(nk, i, True, cross_ket)
+            err += merge_pair(nk, i, False, cross_bra)
+            err += merge_pair(ni, k, True, False)
+            err += merge_pair(ni, k, False, False)
+            for t in nk.mps + ni.mps:
+                bk.maxdim_intermediate = max(bk.maxdim_intermediate, t.shape[1])
     ln, sg = braket_lognorm(bk)
     return lnz + ln, err, psi * sg
 

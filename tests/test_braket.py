@@ -204,3 +204,113 @@ def test_device_rejects_cycles_and_complex(env):
     with pytest.raises(catn.CatnError):
         catn.braket_network([rng.standard_normal((2, 2)), rng.standard_normal((2, 2))],
                             [["a", "b"], ["a", "b"]], ctx=ctx)
+
+
+# ------------------------- loopy double layer --------------------------------
+# No reference counterpart (src/braket.jl:405-420 rejects cycles): with
+# loops=True an eat! that closes a cycle leaves doubled bonds between the eater
+# and a shared neighbour, merged ket and bra separately as merge! does
+# (src/mps_node.jl:595-622). The oracle's extension is pinned in exact mode by
+# the dense <psi|psi> on lattices and on a triangle whose third tensor lists
+# the bond to the eaten node first (the merge with cross = True there).
+
+def _lattice(rng, Lx, Ly, D, d, cplx=False):
+    ts, ixs = [], []
+    for x in range(Lx):
+        for y in range(Ly):
+            ix = []
+            if x > 0:
+                ix.append(f"v{x - 1}_{y}")
+            if y > 0:
+                ix.append(f"h{x}_{y - 1}")
+            ix.append(f"p{x}_{y}")
+            if y < Ly - 1:
+                ix.append(f"h{x}_{y}")
+            if x < Lx - 1:
+                ix.append(f"v{x}_{y}")
+            ts.append(_rand(rng, [d if l.startswith("p") else D for l in ix], cplx))
+            ixs.append(ix)
+    return ts, ixs
+
+
+def _triangle(rng, D):
+    ts = [rng.standard_normal((D, 2, D)), rng.standard_normal((D, D, 2)),
+          rng.standard_normal((D, D, 2))]
+    return ts, [["a", "p1", "b"], ["a", "c", "p2"], ["c", "b", "p3"]]
+
+
+LOOPY = ["triangle_D3", "square_D2", "square_D3", "lattice2x3_D2", "lattice3x3_D2"]
+
+
+def _loopy(name, cplx=False):
+    rng = np.random.default_rng(200 + LOOPY.index(name))
+    if name == "triangle_D3":
+        return _triangle(rng, 3)
+    Lx, Ly, D = {"square_D2": (2, 2, 2), "square_D3": (2, 2, 3), "lattice2x3_D2": (2, 3, 2),
+                 "lattice3x3_D2": (3, 3, 2)}[name]
+    return _lattice(rng, Lx, Ly, D, 2, cplx)
+
+
+@pytest.mark.parametrize("name", LOOPY)
+@pytest.mark.parametrize("cplx", [False, True])
+def test_oracle_loopy_norm_matches_exact(name, cplx):
+    ts, ixs = _loopy(name, cplx and name != "triangle_D3")
+    ref = B.exact_norm(ts, ixs)
+    bk = B.braket_network(ts, ixs, chi=10_000, loops=True)
+    lnz, err, psi = B.contraction(bk)
+    assert abs(math.exp(lnz) * psi - ref) <= 1e-10 * ref
+    assert err < 1e-12
+
+
+def test_oracle_loopy_plants_a_mistake(monkeypatch):
+    """Fusing the doubled legs in opposite orders on the two ends of the bond
+    (the cross flag ignored on the shared neighbour) must break the triangle."""
+    ts, ixs = _loopy("triangle_D3")
+    ref = B.exact_norm(ts, ixs)
+    orig, calls = B.merge_pair, [0]
+
+    def one_end(node, target, isket, cross):
+        calls[0] += 1
+        return orig(node, target, isket, (not cross) if (calls[0] - 1) % 4 < 2 else cross)
+    monkeypatch.setattr(B, "merge_pair", one_end)
+    bk = B.braket_network(ts, ixs, chi=10_000, loops=True)
+    lnz, _, psi = B.contraction(bk)
+    assert abs(math.exp(lnz) * psi - ref) > 1e-3 * ref
+
+
+@pytest.mark.parametrize("name", LOOPY)
+@pytest.mark.parametrize("chi", [10_000, 3])
+def test_device_loopy_matches_oracle(env, name, chi):
+    """Device against the oracle: edge sequence, maxdim_intermediate, lnZ
+    (1e-12 relative), discarded weight; in exact mode also the dense norm."""
+    catn, ctx = env
+    ts, ixs = _loopy(name)
+    bk = B.braket_network(ts, ixs, chi=chi, loops=True)
+    lo, eo, po = B.contraction(bk)
+    tn = catn.braket_network(ts, ixs, ctx=ctx, chi=chi, loops=True)
+    lnz, err, psi = catn.contraction(tn)
+    assert [tuple(e) for e in tn.edge_log()] == bk.edge_log
+    assert tn.maxdim_intermediate == bk.maxdim_intermediate
+    assert abs(lnz - lo) <= 1e-12 * max(1.0, abs(lo))
+    assert abs(err - eo) <= 1e-8 * max(1e-12, eo) + 1e-14
+    assert psi == po == 1.0
+    if chi >= 10_000:
+        assert abs(lnz - math.log(B.exact_norm(ts, ixs))) < 1e-10
+
+
+def test_loops_flag_and_double_bonds(env):
+    """Without loops=True a cycle is rejected (as in the reference); two bonds
+    between one pair of tensors are rejected either way."""
+    catn, ctx = env
+    ts, ixs = _loopy("square_D2")
+    with pytest.raises(catn.CatnError):
+        catn.braket_network(ts, ixs, ctx=ctx, chi=64)
+    with pytest.raises(ValueError):
+        B.braket_network(ts, ixs, chi=64)
+    rng = np.random.default_rng(1)
+    t2 = [rng.standard_normal((2, 2, 2)), rng.standard_normal((2, 2, 2))]
+    ix2 = [["a", "b", "p1"], ["a", "b", "p2"]]
+    with pytest.raises(catn.CatnError):
+        catn.braket_network(t2, ix2, ctx=ctx, chi=64, loops=True)
+    with pytest.raises(ValueError):
+        B.braket_network(t2, ix2, chi=64, loops=True)

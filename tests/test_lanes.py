@@ -146,3 +146,22 @@ def test_column_block_jacobi_lane_parallel(catn, ctxs):
     assert np.allclose(S0, S1, rtol=1e-13, atol=1e-13 * S1[0])
     assert np.allclose(U0 @ np.diag(S0) @ V0.T, A, atol=1e-11 * S1[0])
     assert np.allclose(U0.T @ U0, np.eye(100), atol=1e-12)
+
+
+def test_loopy_braket_lane_parallel(catn, ctxs):
+    """The loopy double layer (doubled-bond merges on a 2x3 lattice, truncated
+    at chi = 3) at 256 lanes: the one-thread build's lnZ, error and edge
+    sequence, no divergent barrier."""
+    from test_braket import _loopy
+    mt, st = ctxs
+    ts, ixs = _loopy("lattice2x3_D2")
+    before = _divergent()
+    res = []
+    for ctx in (mt, st):
+        tn = catn.braket_network(ts, ixs, ctx=ctx, chi=3, loops=True)
+        lz, err, psi = catn.contraction(tn)
+        res.append((lz, err, tn.edge_log()))
+    assert _divergent() == before
+    assert res[0][2] == res[1][2]
+    assert abs(res[0][0] - res[1][0]) <= 1e-12 * abs(res[1][0])
+    assert abs(res[0][1] - res[1][1]) <= 1e-9 * max(abs(res[1][1]), 1e-12)
