@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cut-free-batch", type=int, default=128,
                     help="instances per chosen-form step of the cut-free comparison (0: skip it)")
+    ap.add_argument("--cut-free-only", action="store_true",
+                    help="run only the cut-free chosen-vs-plain comparison and print its JSON")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the automatic launch shape instead of timing the candidates")
     ap.add_argument("--op-profile", default="",
@@ -404,6 +406,10 @@ def main():
     from oracle import catn_oracle as O
     catn = G.load_package()
     ctx = catn.Context(0)
+    if args.cut_free_only:
+        print(json.dumps(run_cut_free(catn, ctx, args.steps, args.warmup, args.cut_free_batch)),
+              flush=True)
+        return
     instances = [Cf.spin_glass_3reg(N_SPINS, seed=args.seed + k) for k in range(args.batch)]
     n, edges, w, h = instances[0]
 
