@@ -185,7 +185,9 @@ CATN_DN bool split_replay(Exec& E, const double* A, int m, int n, int64_t lda, b
   t.r = m < n ? m : n;
   if (path == 1) {
     const int maxd = rule == 1 ? (1 << 30) : maxdim;
-    if (!wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, t.F, E.noise_floor()) ||
+    // the recorded step was certified on this same input: no second bound
+    if (!wg_factor_fullrank(g, E.sc, A, m, n, lda, cutoff, maxd, iso_left, t.F, E.noise_floor(),
+                            true) ||
         t.F.r != k) {
       E.status = E.sc.fail ? STATUS_SCRATCH_FULL : STATUS_REPLAY;
       return false;

@@ -1981,9 +1981,12 @@ CATN_DN bool factor_fullrank_tiny(const WG& g, Scratch& sc, const double* A, int
   return true;
 }
 
+// assume_certified: the caller knows the step was certified (the reverse
+// sweep's replay of a recorded certified step on the recorded input, which
+// reproduces the same R), so the inverse bound is not formed again.
 CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m, int n,
                                 int64_t lda, double cutoff, int maxdim, bool iso_left,
-                                Factor& F, double noise = 0.0) {
+                                Factor& F, double noise = 0.0, bool assume_certified = false) {
   const int r = m < n ? m : n;
   if (r <= 0 || r > maxdim) return false;
   if ((int64_t)(m > n ? m : n) * r <= kTinySvd)
@@ -2034,7 +2037,7 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
     R[idx] = (i <= j) ? W[i + (int64_t)j * tm] : 0.0;
   }
   g.sync();
-  const double lb = wg_inv_fro_bound(g, R, r, r, X);
+  const double lb = assume_certified ? INFINITY : wg_inv_fro_bound(g, R, r, r, X);
   sub_tick(g, 5, t0);
   t0 = now_ticks();
   const double thr = fmax(fmax(100.0 * cutoff, 1e-12 * fro), noise * fro);
