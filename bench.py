@@ -16,7 +16,7 @@ Networks are built and uploaded to HBM before the timed region; the timed
 region is the batched contraction (host planner + device kernels), the reverse
 sweep and the host chain rule to d lnZ/d(w, h), until results are on the host.
 
-Prints one JSON line. `--gpus N` with N > 1 exits: this build targets one GPU.
+Prints one JSON line. `--gpus N` with N > 1 exits: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -396,7 +396,7 @@ def main():
         print(json.dumps({"copy_kernel_bytes_per_launch": nbytes}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+        print(json.dumps({"error": "the library runs on one GPU; --gpus > 1 is not supported "
                                    "(instances and pinned contractions are independent units "
                                    "that would shard across GPUs with one final all-gather)"}))
         sys.exit(2)
