@@ -110,3 +110,15 @@ def test_two_open_components_are_rejected(env):
     catn.contraction(tn)
     with pytest.raises(catn.CatnError):
         catn.result_tensor(tn)
+
+
+def test_fp32_storage_result_tensor(env):
+    """The readout of an FP32-storage network (FP32 arena, six-pass BF16
+    theta): the dense contraction to FP32 accuracy."""
+    catn, ctx = env
+    rng = np.random.default_rng(12)
+    ts = [rng.standard_normal((2, 3)), rng.standard_normal((3, 2, 3)), rng.standard_normal((3, 2))]
+    ixs = [["p1", "a"], ["a", "p2", "b"], ["b", "p3"]]
+    got, labels, _ = _open_value(catn, ctx, ts, ixs, Dmax=-1, chi=10_000, dtype="f32x6")
+    ref = _dense(ts, ixs, labels)
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
