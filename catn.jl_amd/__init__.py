@@ -430,7 +430,8 @@ def braket_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
     Tensors are taken with Julia (column-major) index order. loops=True accepts
     graphs with cycles, which the reference rejects: when an eat closes a
     cycle, the doubled bonds to the shared neighbour are merged (ket and bra
-    legs separately, the older bond as the fast index)."""
+    legs separately, the older bond as the fast index), and with Dmax > 0 a
+    merged bond wider than Dmax is cut as cut_bondim_opt! does."""
     ctx = ctx or default_context()
     lab_ids = {}
     labels, ndims, dims, data = [], [], [], []

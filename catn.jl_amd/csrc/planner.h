@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <set>
 #include <stdexcept>
@@ -421,6 +422,22 @@ class Planner {
     return r;
   }
 
+  // cut_bondim_opt! (src/tensor_network.jl:399-477) of an explicit leg pair:
+  // leg idx_i of node i and leg idx_k of node k carry the same bond (the
+  // double layer's merged ket or bra bond, which find_neighbor cannot tell
+  // apart); returns the result index that receives `myd`.
+  int emit_cut_legs(int i, int idx_i, int k, int idx_k) {
+    PNode& ni = node(i);
+    PNode& nk = node(k);
+    cano_to(ni, idx_i);
+    cano_to(nk, idx_k);
+    const int r = nresults++;
+    emit(OP_CUT, ni.slot[idx_i - 1], nk.slot[idx_k - 1], 0, p.Dmax, r, 1, p.cutoff);
+    pending_cut_[0] = i; pending_cut_[1] = idx_i; pending_cut_[2] = k;
+    pending_cut_[3] = idx_k; pending_cut_[4] = r;
+    return r;
+  }
+
   // apply the device's kept count to both physical legs of the cut bond
   void apply_cut(int myd) {
     node(pending_cut_[0]).phys[pending_cut_[1] - 1] = myd;
@@ -526,7 +543,20 @@ class Planner {
   }
 
   StepResult braket_step() {
-    while (!bk_edges.empty()) {
+    for (;;) {
+      // pending cuts of merged bonds (loops with Dmax > 0): one per segment
+      while (!bk_cuts_.empty()) {
+        const std::array<int, 3> c = bk_cuts_.front();
+        bk_cuts_.erase(bk_cuts_.begin());
+        const int ai = find_leg(node(c[0]), c[1], c[2] != 0);
+        const int ak = find_leg(node(c[1]), c[0], c[2] != 0);
+        if (ai > 0 && ak > 0 && node(c[0]).phys[ai - 1] > p.Dmax) {
+          acc_err = true;
+          emit_cut_legs(c[0], ai, c[1], ak);
+          return NEED_CUT;
+        }
+      }
+      if (bk_edges.empty()) break;
       int i = bk_edges[0].first, j = bk_edges[0].second;
       if (node(j).order() > node(i).order()) std::swap(i, j);
       edge_log.push_back({i, j});
@@ -575,6 +605,10 @@ class Planner {
         braket_merge(ni2, d.first, false, false);
         for (int dd : nk.phys) maxdim_intermediate = std::max(maxdim_intermediate, dd);
         for (int dd : ni2.phys) maxdim_intermediate = std::max(maxdim_intermediate, dd);
+        if (p.Dmax > 0) {   // the merged ket bond, then the bra bond
+          bk_cuts_.push_back({i, d.first, 1});
+          bk_cuts_.push_back({i, d.first, 0});
+        }
       }
     }
     for (auto& kv : nodes) {
@@ -597,7 +631,8 @@ class Planner {
     m.add((uint64_t)p.norm_method); m.add((uint64_t)state_); m.add((uint64_t)cur_i_);
     m.add((uint64_t)cur_j_); m.add((uint64_t)dup_pos_); m.add(dups_);
     m.add((uint64_t)nresults); m.add((uint64_t)(int64_t)maxdim_intermediate);
-    m.add((uint64_t)braket); m.add(bk_edges.size());
+    m.add((uint64_t)braket); m.add(bk_edges.size()); m.add(bk_cuts_.size());
+    for (const auto& c : bk_cuts_) { m.add((uint64_t)c[0]); m.add((uint64_t)c[1]); m.add((uint64_t)c[2]); }
     for (const auto& e : bk_edges) { m.add((uint64_t)e.first); m.add((uint64_t)e.second); }
     m.add(nodes.size());
     for (const auto& kv : nodes) {
@@ -618,7 +653,7 @@ class Planner {
            maxdim_intermediate == o.maxdim_intermediate && acc_err == o.acc_err &&
            std::equal(pending_cut_, pending_cut_ + 5, o.pending_cut_) && nodes == o.nodes &&
            edge_count == o.edge_count && edge_log == o.edge_log && prog.size() == o.prog.size() &&
-           braket == o.braket && bk_edges == o.bk_edges;
+           braket == o.braket && bk_edges == o.bk_edges && bk_cuts_ == o.bk_cuts_;
   }
 
   StepResult step() {
@@ -685,6 +720,8 @@ class Planner {
   int dup_pos_ = 0;
   bool cut_pending_ = false;
   int pending_cut_[5] = {0, 0, 0, 0, 0};
+  // loopy double layer: merged bonds (node, neighbour, ket) still to be cut
+  std::vector<std::array<int, 3>> bk_cuts_;
 
   static int cld2(int a) { return (a + 1) / 2; }   // cld(a, 2) for a >= 0
 

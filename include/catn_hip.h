@@ -52,7 +52,8 @@ typedef struct catn_params {
    * cycle, the two bonds to the shared neighbour are merged on both nodes, ket
    * and bra legs separately, the older bond as the fast index (merge!,
    * src/mps_node.jl:595-622). Two bonds between one pair of tensors are still
-   * rejected. */
+   * rejected. With Dmax > 0, a merged ket or bra bond wider than Dmax is then
+   * cut as cut_bondim_opt! does (the reference's double layer has no Dmax). */
   int32_t loops;
 } catn_params;
 

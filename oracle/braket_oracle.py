@@ -197,10 +197,11 @@ class BraKetNetwork:
     maxdim_intermediate: int = 0
     edge_log: List[tuple] = field(default_factory=list)   # (i, j) eaten, in order
     loops: bool = False
+    Dmax: int = -1     # loops only: cut of merged bonds wider than Dmax
 
 
 def braket_network(tensors, ixs, chi=64, cutoff=1e-15, norm_method=1, svdopt=True,
-                   swapopt=True, loops=False):
+                   swapopt=True, loops=False, Dmax=-1):
     """src/braket.jl:369-496 (edge order: first appearance of the label).
     loops=True accepts cycles (no reference counterpart); two bonds between
     one pair of tensors are rejected either way."""
@@ -247,7 +248,8 @@ def braket_network(tensors, ixs, chi=64, cutoff=1e-15, norm_method=1, svdopt=Tru
                 kn.append(b if a == i + 1 else a)
         nodes[i + 1] = braket_node(tensors[i], kn, pp, chi, cutoff, norm_method, svdopt, swapopt)
     edges = [tuple(owners[l]) for l in order_ if count[l] == 2]
-    return BraKetNetwork(nodes, edges, chi, cutoff, norm_method, svdopt, swapopt, loops=loops)
+    return BraKetNetwork(nodes, edges, chi, cutoff, norm_method, svdopt, swapopt, loops=loops,
+                         Dmax=Dmax)
 
 
 def braket_lognorm(bk):
@@ -312,6 +314,16 @@ def contraction(bk):
             err += merge_pair(ni, k, False, False)
             for t in nk.mps + ni.mps:
                 bk.maxdim_intermediate = max(bk.maxdim_intermediate, t.shape[1])
+        # Dmax (no reference counterpart): the merged ket bond, then the bra
+        # bond, cut as cut_bondim_opt! (src/tensor_network.jl:399-477) when
+        # wider than Dmax
+        if bk.Dmax > 0:
+            for (k, _, _) in dups:
+                for isket in (True, False):
+                    nk = bk.tensors[k]
+                    ai, ak = find_leg(ni, k, isket), find_leg(nk, i, isket)
+                    if ai > 0 and ak > 0 and ni.mps[ai - 1].shape[1] > bk.Dmax:
+                        err += O.cut_legs(ni, ai, nk, ak, True, bk.cutoff, bk.Dmax)
     ln, sg = braket_lognorm(bk)
     return lnz + ln, err, psi * sg
 

@@ -700,6 +700,13 @@ def _cut_common(tn, i, idx_j_in_i, opt):
     idx_i_in_j = find_neighbor(node_j, i)
     if idx_i_in_j == 0:
         raise ValueError(f"cut_bondim: node {i} not found as neighbor of node {j}")
+    return cut_legs(node_i, idx_j_in_i, node_j, idx_i_in_j, opt, tn.cutoff, tn.Dmax)
+
+
+def cut_legs(node_i, idx_j_in_i, node_j, idx_i_in_j, opt, cutoff, Dmax):
+    """The cut of src/tensor_network.jl:342-388 / :399-477 on an explicit pair
+    of legs that carry one bond (leg idx_j_in_i of node_i, leg idx_i_in_j of
+    node_j)."""
     if opt:
         cano_to(node_i, idx_j_in_i)
         cano_to(node_j, idx_i_in_j)
@@ -720,13 +727,13 @@ def _cut_common(tn, i, idx_j_in_i, opt):
     else:
         rj = matj
     merged = ri @ rj.T
-    U, S, V, disc = tsvd(merged, cutoff=tn.cutoff, kind="cut")
-    myd = min(len(S), tn.Dmax)
+    U, S, V, disc = tsvd(merged, cutoff=cutoff, kind="cut")
+    myd = min(len(S), Dmax)
     if myd == 0:
         myd = 1
     extra = float(np.sum(S[myd:])) if myd < len(S) else 0.0
     err = disc + extra
-    _log("cut_keep", merged.shape, myd, S, tn.cutoff)
+    _log("cut_keep", merged.shape, myd, S, cutoff)
     S, U, V = S[:myd], U[:, :myd], V[:, :myd]
     sq = np.sqrt(S)
     mi = U * sq[None, :]

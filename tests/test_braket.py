@@ -317,3 +317,33 @@ def test_loops_flag_and_double_bonds(env):
         catn.braket_network(t2, ix2, ctx=ctx, chi=64, loops=True)
     with pytest.raises(ValueError):
         B.braket_network(t2, ix2, chi=64, loops=True)
+
+
+@pytest.mark.parametrize("name,Dmax,chi", [("lattice3x3_D2", 3, 10_000), ("lattice3x3_D2", 2, 8),
+                                           ("lattice2x3_D2", 3, 10_000), ("lattice3x3_D3", 5, 10_000)])
+def test_device_loopy_dmax_matches_oracle(env, name, Dmax, chi):
+    """Dmax on the loopy double layer (no reference counterpart): a merged ket
+    or bra bond wider than Dmax is cut as cut_bondim_opt! does. Device against
+    the oracle's restatement: edge sequence, maxdim_intermediate, lnZ
+    (1e-12 relative), discarded weight, phase."""
+    catn, ctx = env
+    ts, ixs = _loopy(name)
+    bk = B.braket_network(ts, ixs, chi=chi, loops=True, Dmax=Dmax)
+    lo, eo, po = B.contraction(bk)
+    assert eo > 1e-10                      # the cut did truncate
+    tn = catn.braket_network(ts, ixs, ctx=ctx, chi=chi, loops=True, Dmax=Dmax)
+    lnz, err, psi = catn.contraction(tn)
+    assert [tuple(e) for e in tn.edge_log()] == bk.edge_log
+    assert tn.maxdim_intermediate == bk.maxdim_intermediate
+    assert abs(lnz - lo) <= 1e-12 * max(1.0, abs(lo))
+    assert abs(err - eo) <= 1e-8 * max(1e-12, eo) + 1e-14
+    assert psi == po
+
+
+def test_loopy_dmax_above_every_bond_changes_nothing(env):
+    catn, ctx = env
+    ts, ixs = _loopy("lattice3x3_D2")
+    a = catn.contraction(catn.braket_network(ts, ixs, ctx=ctx, chi=10_000, loops=True))
+    b = catn.contraction(catn.braket_network(ts, ixs, ctx=ctx, chi=10_000, loops=True, Dmax=64))
+    assert a == b
+    assert abs(a[0] - math.log(B.exact_norm(ts, ixs))) < 1e-10
