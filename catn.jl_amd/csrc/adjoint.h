@@ -768,6 +768,7 @@ CATN_DN int run_backward(const WG& g, InstState* st) {
   c.adj = st->adj;
   int64_t shwm = 0;
   for (int ti = st->tape_len - 1; ti >= 0 && E.status == STATUS_OK; --ti) {
+    CATN_EMU_OP_BEGIN(g, st);
     const TapeRec rc = st->tape[ti];
     const int64_t t0 = now_ticks();
     // replay with the forward launch's LDS budget: the same kernels then take

@@ -27,6 +27,18 @@
 #define CATN_DN
 #endif
 
+// CATN_DEV selects the device code paths: FP64/BF16 matrix-core products,
+// wavefront shuffles and the one-barrier reductions. They are compiled for
+// gfx950 and, with CATN_EMU_DEV, by the test-only wavefront emulator
+// (tests/emul/catn_emul_mt.cpp), which supplies the three wavefront
+// primitives below on host threads so that the device's own operand and
+// accumulator index arithmetic runs under the host sanitizers.
+#if defined(__HIP_DEVICE_COMPILE__) || defined(CATN_EMU_DEV)
+#define CATN_DEV 1
+#else
+#define CATN_DEV 0
+#endif
+
 namespace catn {
 
 #if defined(CATN_EMU_MT)
@@ -38,6 +50,48 @@ double emu_wg_sum(double v);
 double emu_wg_max(double v);
 double emu_grp_sum(double v, int L);
 double* emu_lds_base();
+// start of every op and reverse step: with CATN_EMU_REPOISON set, the
+// scratch used so far and the LDS are refilled with a NaN pattern, so a read
+// of either before this op wrote it shows up in the results
+struct InstState;
+struct WG;
+void emu_op_begin(const WG& g, InstState* st);
+#define CATN_EMU_OP_BEGIN(g, st) emu_op_begin(g, st)
+#else
+#define CATN_EMU_OP_BEGIN(g, st) ((void)0)
+#endif
+
+// wavefront primitives of the device paths:
+//   shfl_xor(v, o)            value of v in lane (this lane ^ o), o < 64;
+//   mfma_f64_16x16x4(a, b, c) v_mfma_f64_16x16x4f64: lane l holds A(l % 16,
+//                             l / 16), B(l / 16, l % 16) and D = A B + C at
+//                             (l / 16 + 4 v, l % 16) in element v;
+//   mfma_bf16_16x16x16(a, b, c) v_mfma_f32_16x16x16bf16_1k: lane l holds
+//                             A(l % 16, 4 (l / 16) + v), B(4 (l / 16) + v,
+//                             l % 16) and D at (4 (l / 16) + v, l % 16).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef double catn_d4 __attribute__((ext_vector_type(4)));
+typedef short catn_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float catn_f32x4 __attribute__((ext_vector_type(4)));
+CATN_D double shfl_xor(double v, int o) { return __shfl_xor(v, o, 64); }
+CATN_D catn_d4 mfma_f64_16x16x4(double a, double b, catn_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+CATN_D catn_f32x4 mfma_bf16_16x16x16(catn_bf16x4 a, catn_bf16x4 b, catn_f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+#elif defined(CATN_EMU_DEV)
+typedef double catn_d4 __attribute__((vector_size(32)));
+typedef short catn_bf16x4 __attribute__((vector_size(8)));
+typedef float catn_f32x4 __attribute__((vector_size(16)));
+double emu_shfl_xor(double v, int o);
+catn_d4 emu_mfma_f64(double a, double b, catn_d4 c);
+catn_f32x4 emu_mfma_bf16(catn_bf16x4 a, catn_bf16x4 b, catn_f32x4 c);
+inline double shfl_xor(double v, int o) { return emu_shfl_xor(v, o); }
+inline catn_d4 mfma_f64_16x16x4(double a, double b, catn_d4 c) { return emu_mfma_f64(a, b, c); }
+inline catn_f32x4 mfma_bf16_16x16x16(catn_bf16x4 a, catn_bf16x4 b, catn_f32x4 c) {
+  return emu_mfma_bf16(a, b, c);
+}
 #endif
 
 // ---------------------------------------------------------------------------
@@ -73,8 +127,8 @@ struct WG {
 };
 
 CATN_D double wg_sum(const WG& g, double v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#if CATN_DEV
+  for (int o = 32; o > 0; o >>= 1) v += shfl_xor(v, o);
   const int w = g.tid >> 6, l = g.tid & 63;
   g.sync();
   if (l == 0) g.red[w] = v;
@@ -96,8 +150,8 @@ CATN_D double wg_sum(const WG& g, double v) {
 // reduction slots (nthr <= 512). The caller alternates the parity and keeps a
 // workgroup barrier between the reads of one half and its next write.
 CATN_D double wg_sum_db(const WG& g, double v, int parity) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#if CATN_DEV
+  for (int o = 32; o > 0; o >>= 1) v += shfl_xor(v, o);
   const int w = g.tid >> 6, l = g.tid & 63;
   double* r = g.red + 8 * (parity & 1);
   if (l == 0) r[w] = v;
@@ -113,8 +167,8 @@ CATN_D double wg_sum_db(const WG& g, double v, int parity) {
 }
 
 CATN_D double wg_max(const WG& g, double v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+#if CATN_DEV
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, shfl_xor(v, o));
   const int w = g.tid >> 6, l = g.tid & 63;
   g.sync();
   if (l == 0) g.red[w] = v;
@@ -136,8 +190,8 @@ CATN_D double wg_max(const WG& g, double v) {
 // butterfly gives every lane the bitwise-identical sum (a+b == b+a), so
 // decisions taken on it are uniform across the group.
 CATN_D double grp_sum(double v, int L) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  for (int o = L >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+#if CATN_DEV
+  for (int o = L >> 1; o > 0; o >>= 1) v += shfl_xor(v, o);
 #elif defined(CATN_EMU_MT)
   v = emu_grp_sum(v, L);
 #else
@@ -450,9 +504,7 @@ CATN_D void wg_gemm_tile(const WG& g, int M, int N, int K, double alpha, const d
   g.sync();
 }
 
-#if defined(__HIP_DEVICE_COMPILE__)
-typedef double catn_d4 __attribute__((ext_vector_type(4)));
-
+#if CATN_DEV
 // The same product on the FP64 matrix cores (v_mfma_f64_16x16x4f64; lane l
 // holds A(l % 16, l / 16), B(l / 16, l % 16) and D(l / 16 + 4 v, l % 16) in
 // accumulator element v, checked on gfx950): one wavefront per 16 x 16 output
@@ -479,7 +531,7 @@ CATN_D void wg_gemm_mfma(const WG& g, int M, int N, int K, double alpha, const d
       const int k = k0 + lk;
       const double a = (rok && k < K) ? ap[sa * k] : 0.0;
       const double b = (cok && k < K) ? bp[sb * k] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      acc = mfma_f64_16x16x4(a, b, acc);
     }
     for (int v = 0; v < 4; ++v) {
       const int r = r0 + lk + 4 * v;
@@ -499,7 +551,7 @@ template <class Out>
 CATN_D void wg_gemm_t(const WG& g, int M, int N, int K, double alpha, const double* A, int64_t lda,
                       bool tA, const double* B, int64_t ldb, bool tB, double beta, double* C,
                       Out out) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
   wg_gemm_mfma(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, out);
 #else
   wg_gemm_tile<kTM, kTN, kTK>(g, M, N, K, alpha, A, lda, tA, B, ldb, tB, beta, C, out);
@@ -600,16 +652,11 @@ struct Bf16Passes<3> {
   CATN_D static int b(int p) { return (0x012010 >> (4 * p)) & 15; }
 };
 
-#if defined(__HIP_DEVICE_COMPILE__)
-typedef short catn_bf16x4 __attribute__((ext_vector_type(4)));
-typedef float catn_f32x4 __attribute__((ext_vector_type(4)));
-#endif
-
 template <int NT, class Out>
 CATN_DN void wg_gemm_bf16(const WG& g, int M, int N, int K, const float* A, int64_t lda,
                           const float* B, int64_t ldb, double* C, Out out) {
   typedef Bf16Passes<NT> P;
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
   // 16 x 16 x 16 MFMA operand layout: lane l holds A(l % 16, 4 (l / 16) + v),
   // B(4 (l / 16) + v, l % 16) and result C(4 (l / 16) + v, l % 16), v = 0..3
   const int nw = g.nthr >> 6, w = g.tid >> 6, l = g.tid & 63;
@@ -634,7 +681,7 @@ CATN_DN void wg_gemm_bf16(const WG& g, int M, int N, int K, const float* A, int6
         }
       }
       for (int p = 0; p < P::n; ++p)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(at[P::a(p)], bt[P::b(p)], acc, 0, 0, 0);
+        acc = mfma_bf16_16x16x16(at[P::a(p)], bt[P::b(p)], acc);
     }
     for (int v = 0; v < 4; ++v) {
       const int i = i0 + lk + v, j = j0 + lr;
@@ -676,7 +723,7 @@ struct IdxTheta {
 // T factors are kept (T_all, kQBMax x n) for wg_apply_q_blocked.
 // ---------------------------------------------------------------------------
 
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
 // FP64 matrix-core forms of the two products of a compact-WY update, for
 // v_mfma_f64_16x16x4f64: a wavefront computes a 16 x 16 tile D += A B with
 // A 16 x 4 and B 4 x 16; lane l holds A(l % 16, l / 16), B(l / 16, l % 16) and
@@ -725,7 +772,7 @@ CATN_D bool wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, i
       const int k = k0 + lk;
       const double a = (qok && k < r1) ? refl_at(V, ldv, vunit, k, qs) : 0.0;
       const double b = (cok && k < r1) ? refl_at(C, ldc, cunit, k, cs) : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      acc = mfma_f64_16x16x4(a, b, acc);
     }
     if (fuse) {
       catn_d4 o = {0.0, 0.0, 0.0, 0.0};
@@ -736,7 +783,7 @@ CATN_D bool wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, i
           if (tT) a = kk <= li ? T[kk + kQBMax * li] : 0.0;   // T^T(li, kk) = T(kk, li)
           else a = li <= kk ? T[li + kQBMax * kk] : 0.0;
         }
-        o = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[st], o, 0, 0, 0);
+        o = mfma_f64_16x16x4(a, acc[st], o);
       }
       acc = o;
     }
@@ -788,7 +835,7 @@ CATN_D void wg_c_minus_vw_mfma(const WG& g, int rows, int N, int jb, const doubl
       const int k = 4 * st + lk;
       const double a = (ra < rows && k < jb) ? -refl_at(V, ldv, vunit, ra, k) : 0.0;
       const double b = (k < jb && c < N) ? W2[k + ldw * (int64_t)c] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      acc = mfma_f64_16x16x4(a, b, acc);
     }
     for (int v = 0; v < 4; ++v) {
       const int r = r0 + lk + 4 * v;
@@ -803,7 +850,7 @@ CATN_D void wg_c_minus_vw_mfma(const WG& g, int rows, int N, int jb, const doubl
 // LDS tile (budget permitting)
 CATN_DN void wg_gemm_skinny_tn(const WG& g, int jb, int N, int rows, const double* V, int64_t ldv,
                                const double* B, int64_t ldb, double* C, int64_t ldc) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
   wg_vtc_mfma(g, jb, N, rows, V, ldv, false, B, ldb, false, C, ldc);
 #else
   IdxPlain o;
@@ -819,7 +866,7 @@ CATN_DN void wg_gemm_skinny_tn(const WG& g, int jb, int N, int rows, const doubl
 // the device, the tiled GEMM on the host
 CATN_DN void wg_update_cvw(const WG& g, int rows, int N, int jb, const double* V, int64_t ldv,
                            const double* W2, int64_t ldw, double* C, int64_t ldc) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
   wg_c_minus_vw_mfma(g, rows, N, jb, V, ldv, false, W2, ldw, C, ldc);
 #else
   wg_gemm(g, rows, N, jb, -1.0, V, ldv, false, W2, ldw, false, 1.0, C, ldc);
@@ -832,7 +879,7 @@ CATN_DN void wg_update_cvw(const WG& g, int rows, int N, int jb, const double* V
 CATN_D void wg_larft(const WG& g, const double* V, int64_t ldv, bool vunit, int rows, int jb,
                      const double* tau, double* T, double* Y) {
   // Y = V^T V (jb x jb)
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
   wg_vtc_mfma(g, jb, jb, rows, V, ldv, vunit, V, ldv, vunit, Y, jb);
 #else
   (void)vunit;
@@ -879,7 +926,7 @@ CATN_D void wg_gemm_nn_t(const WG& g, int jb, int N, const double* T, const doub
 CATN_DN void wg_wy_apply(const WG& g, int rows, int N, int jb, const double* V, int64_t ldv,
                          bool vunit, const double* T, bool tT, double* C, int64_t ldc, double* W,
                          double* W2) {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
   const bool fuse = jb <= 16 && ((N + 15) >> 4) >= (g.nthr >> 6);
   if (fuse) {
     wg_vtc_mfma(g, jb, N, rows, V, ldv, vunit, C, ldc, false, W2, jb, T, tT);
@@ -931,7 +978,9 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     wg_geqr2_staged(g, A, m, n, lda, tau);
     return false;   // unblocked: no T factors
   }
-  double* V = sc.alloc((int64_t)m * qb);
+#if !CATN_DEV
+  double* V = sc.alloc((int64_t)m * qb);   // unit-lower panel copy (device: read in place)
+#endif
   double* W = sc.alloc((int64_t)qb * n);
   double* W2 = sc.alloc((int64_t)qb * n);
   double* Y = sc.alloc(qb * qb);
@@ -944,7 +993,7 @@ CATN_DN bool wg_geqrf(const WG& g, Scratch& sc, double* A, int m, int n, int64_t
     wg_geqr2_staged(g, Ap, rows, jb, lda, tau + p);
     sub_tick(g, 8, t0);
     t0 = now_ticks();
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
     const double* Vp = Ap;   // reflectors read in place (unit lower)
     const int64_t ldv = lda;
     const bool vunit = true;
@@ -975,7 +1024,9 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
   }
   const int64_t save = sc.top;
   const int qb = g.qb;   // the panel width wg_geqrf used (same launch)
-  double* V = sc.alloc((int64_t)m * qb);
+#if !CATN_DEV
+  double* V = sc.alloc((int64_t)m * qb);   // unit-lower panel copy (device: read in place)
+#endif
   double* W = sc.alloc((int64_t)qb * k);
   double* W2 = sc.alloc((int64_t)qb * k);
   if (sc.fail) return;
@@ -984,7 +1035,7 @@ CATN_DN void wg_apply_q_blocked(const WG& g, Scratch& sc, const double* A, int m
   for (int p = last; p >= 0; p -= qb) {
     const int jb = (n - p) < qb ? (n - p) : qb;
     const int rows = m - p;
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
     const double* Vp = A + p + lda * (int64_t)p;   // reflectors read in place (unit lower)
     const int64_t ldv = lda;
     const bool vunit = true;
@@ -1797,7 +1848,7 @@ CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int
     }
     g.sync();
     // columns right of the block
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CATN_DEV
     // on the FP64 matrix cores, one wavefront per 16-column tile: the tile
     // T = R(I, i1:c) X(i1:c, tile) accumulates over K = i1 .. tile end (the
     // diagonal stage of each block zeroed X below its diagonal, so the
@@ -1815,13 +1866,13 @@ CATN_DN double wg_inv_fro_bound_blocked(const WG& g, const double* R, int n, int
           const int q = q0 + lk;
           const double a = (li < bs && q < kend) ? R[(i0 + li) + (int64_t)q * ldr] : 0.0;
           const double b = (q < kend && c < n) ? X[q + (int64_t)c * n] : 0.0;
-          tt = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, tt, 0, 0, 0);
+          tt = mfma_f64_16x16x4(a, b, tt);
         }
         catn_d4 xo = {0.0, 0.0, 0.0, 0.0};
         for (int st = 0; st < 4; ++st) {
           const int k = 4 * st + lk;
           const double a = (li < bs && k < bs) ? -X[(i0 + li) + (int64_t)(i0 + k) * n] : 0.0;
-          xo = __builtin_amdgcn_mfma_f64_16x16x4f64(a, tt[st], xo, 0, 0, 0);
+          xo = mfma_f64_16x16x4(a, tt[st], xo);
         }
         for (int v = 0; v < 4; ++v) {
           const int r = lk + 4 * v;
@@ -2988,6 +3039,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
     E.sc.cap = st->scratch_cap;
     E.sc.top = 0;
     E.sc.fail = false;
+    CATN_EMU_OP_BEGIN(g, st);
     E.snapshot(op);
     E.nconv = 0;
     E.nstage = 0;

@@ -14,6 +14,7 @@
 //     time limit instead of hanging.
 // Cross-lane reductions give every lane the same value in a fixed order, as
 // the device's butterfly reductions do.
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -55,8 +56,21 @@ struct EmuShared {
   };
   std::vector<GSlot> gs;
   std::vector<double> gx;
+  // wavefront primitives (CATN_EMU_DEV): per-lane shuffle slots, one counter
+  // per offset 1..32, and matrix-core operand slots
+  struct alignas(64) ShSlot {
+    std::atomic<long> cnt[6] = {};
+    double val[6][2] = {};
+  };
+  struct alignas(64) MfSlot {
+    std::atomic<long> cnt{0};
+    double a[2] = {}, b[2] = {};
+    short ha[2][4] = {}, hb[2][4] = {};
+  };
+  std::vector<ShSlot> sx;
+  std::vector<MfSlot> mf;
   explicit EmuShared(int n, int lds_doubles)
-      : nthr(n), lds_cap(lds_doubles), xb(n), sites(n), gs(n), gx(n) {
+      : nthr(n), lds_cap(lds_doubles), xb(n), sites(n), gs(n), gx(n), sx(n), mf(n) {
     lds = new double[lds_doubles > 0 ? lds_doubles : 1];
     const uint64_t sentinel = 0x7ff4deadbeef0001ull;   // a NaN no kernel writes
     for (int i = 0; i < lds_doubles; ++i) memcpy(lds + i, &sentinel, 8);
@@ -251,10 +265,141 @@ double emu_grp_sum(double v, int L) {
 
 double* emu_lds_base() { return t_lane.sh->lds; }
 
+// CATN_EMU_REPOISON=1: at the start of every op and reverse step, lane 0
+// refills the instance's scratch up to its high-water mark and the launch's
+// LDS with a NaN pattern (all lanes wait before and after)
+static bool g_repoison = getenv("CATN_EMU_REPOISON") != nullptr;
+void emu_op_begin(const WG& g, InstState* st) {
+  if (!g_repoison) return;
+  g.sync();
+  if (g.tid == 0) {
+    const int64_t n = std::min<int64_t>(st->scratch_cap, std::max<int64_t>(st->scratch_hwm, 1 << 18));
+    memset(st->scratch, 0xFF, sizeof(double) * (size_t)n);
+    EmuShared* sh = t_lane.sh;
+    memset(sh->lds, 0xFF, sizeof(double) * (size_t)sh->lds_cap);
+  }
+  g.sync();
+}
+
+#if defined(CATN_EMU_DEV)
+// Wavefront primitives of the device code paths (CATN_EMU_DEV build). Each
+// call is a rendezvous of the lanes the hardware instruction reads:
+//   * shfl_xor(v, o): this lane and lane ^ o, matched per offset by a call
+//     counter, so both lanes must execute the same number of shuffles at
+//     that offset (on the device an inactive partner lane returns a stale
+//     register);
+//   * the matrix-core products: all 64 lanes of the wavefront (the device
+//     loops around them have wavefront-uniform bounds).
+// Values are exchanged through per-lane slots alternating between two
+// parities: a lane overwrites a parity only after its partners have passed
+// the following call, which they reach only after reading it. A lane that
+// waits longer than the time limit (a partner never arrives) is reported.
+std::atomic<long> g_mfma_f64{0}, g_mfma_bf16{0}, g_shfl{0};   // emulated instructions
+
+static void spin_until(const std::atomic<long>& c, long target, const char* what) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int spins = 0; c.load(std::memory_order_acquire) < target;) {
+    if (++spins > 64) {
+      std::this_thread::yield();
+      if ((spins & 4095) == 0 &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
+              emu_timeout_s())
+        emu_die(what);
+    }
+  }
+}
+
+double emu_shfl_xor(double v, int o) {
+  EmuShared* sh = t_lane.sh;
+  if (t_lane.tid == 0) g_shfl.fetch_add(1, std::memory_order_relaxed);
+  const int t = t_lane.tid, p = t ^ o;
+  if (o <= 0 || o >= 64 || (o & (o - 1))) emu_die("shuffle offset is not a power of two < 64");
+  if (p >= sh->nthr) emu_die("shuffle partner outside the workgroup");
+  int q = 0;
+  while ((1 << q) != o) ++q;
+  EmuShared::ShSlot& me = sh->sx[t];
+  const long n = me.cnt[q].load(std::memory_order_relaxed);
+  me.val[q][n & 1] = v;
+  me.cnt[q].store(n + 1, std::memory_order_release);
+  EmuShared::ShSlot& other = sh->sx[p];
+  spin_until(other.cnt[q], n + 1, "shuffle partner never arrived");
+  return other.val[q][n & 1];
+}
+
+// the call number of this lane's next matrix-core product
+static long wave_deposit(EmuShared* sh, int t) {
+  return sh->mf[t].cnt.load(std::memory_order_relaxed);
+}
+// wait until every lane of this lane's wavefront deposited call n
+static void wave_wait(EmuShared* sh, int t, long n) {
+  const int w0 = t & ~63;
+  if (w0 + 64 > sh->nthr) emu_die("matrix-core product in a partial wavefront");
+  for (int j = w0; j < w0 + 64; ++j)
+    spin_until(sh->mf[j].cnt, n + 1, "matrix-core product: a lane of the wavefront never arrived");
+}
+
+catn_d4 emu_mfma_f64(double a, double b, catn_d4 c) {
+  EmuShared* sh = t_lane.sh;
+  if ((t_lane.tid & 63) == 0) g_mfma_f64.fetch_add(1, std::memory_order_relaxed);
+  const int t = t_lane.tid, l = t & 63, w0 = t & ~63;
+  const long n = wave_deposit(sh, t);
+  EmuShared::MfSlot& me = sh->mf[t];
+  me.a[n & 1] = a;
+  me.b[n & 1] = b;
+  me.cnt.store(n + 1, std::memory_order_release);
+  wave_wait(sh, t, n);
+  catn_d4 d;
+  for (int v = 0; v < 4; ++v) {
+    const int r = l / 16 + 4 * v, col = l % 16;
+    double s = c[v];
+    for (int k = 0; k < 4; ++k) s += sh->mf[w0 + r + 16 * k].a[n & 1] * sh->mf[w0 + col + 16 * k].b[n & 1];
+    d[v] = s;
+  }
+  return d;
+}
+
+static float bf16_to_f32(short h) {
+  const uint32_t u = (uint32_t)(uint16_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+catn_f32x4 emu_mfma_bf16(catn_bf16x4 a, catn_bf16x4 b, catn_f32x4 c) {
+  EmuShared* sh = t_lane.sh;
+  if ((t_lane.tid & 63) == 0) g_mfma_bf16.fetch_add(1, std::memory_order_relaxed);
+  const int t = t_lane.tid, l = t & 63, w0 = t & ~63;
+  const long n = wave_deposit(sh, t);
+  EmuShared::MfSlot& me = sh->mf[t];
+  for (int v = 0; v < 4; ++v) {
+    me.ha[n & 1][v] = a[v];
+    me.hb[n & 1][v] = b[v];
+  }
+  me.cnt.store(n + 1, std::memory_order_release);
+  wave_wait(sh, t, n);
+  catn_f32x4 d;
+  for (int v = 0; v < 4; ++v) {
+    const int r = 4 * (l / 16) + v, col = l % 16;
+    float s = c[v];
+    for (int k = 0; k < 16; ++k) {
+      const short x = sh->mf[w0 + r + 16 * (k / 4)].ha[n & 1][k % 4];
+      const short y = sh->mf[w0 + col + 16 * (k / 4)].hb[n & 1][k % 4];
+      s += bf16_to_f32(x) * bf16_to_f32(y);
+    }
+    d[v] = s;
+  }
+  return d;
+}
+#endif
+
 class MtExecutor : public Executor {
  public:
   MtExecutor() { threads = 256; }
+#if defined(CATN_EMU_DEV)
+  const char* name() const override { return "cpu-emulator-dev"; }
+#else
   const char* name() const override { return "cpu-emulator-mt"; }
+#endif
   // CATN_EMU_POISON=1 fills new device blocks with 0xFF bytes (NaN doubles,
   // -1 integers), as hipMalloc gives no zeroed memory: a read of memory the
   // engine never wrote then shows up in the results.
@@ -338,5 +483,14 @@ std::atomic<long> g_lds_hwm{0};
 
 // barriers reached from different call sites since the library was loaded
 extern "C" long catn_emu_divergent_barriers(void) { return catn::g_divergent.load(); }
+#if defined(CATN_EMU_DEV)
+// wavefront instructions emulated since the last call (FP64 matrix-core,
+// BF16 matrix-core, shuffles of lane 0); resets
+extern "C" void catn_emu_wave_ops(long* out) {
+  out[0] = catn::g_mfma_f64.exchange(0);
+  out[1] = catn::g_mfma_bf16.exchange(0);
+  out[2] = catn::g_shfl.exchange(0);
+}
+#endif
 // highest LDS extent (doubles) any launch wrote since the last call; resets
 extern "C" long catn_emu_lds_high_water(void) { return catn::g_lds_hwm.exchange(0); }

@@ -165,3 +165,93 @@ def test_loopy_braket_lane_parallel(catn, ctxs):
     assert res[0][2] == res[1][2]
     assert abs(res[0][0] - res[1][0]) <= 1e-12 * abs(res[1][0])
     assert abs(res[0][1] - res[1][1]) <= 1e-9 * max(abs(res[1][1]), 1e-12)
+
+
+# ----------------------------------------------------------------------------
+# Device code paths on the CPU (tests/emul/libcatn_emul_dev.so): the same lane
+# emulator compiled with CATN_EMU_DEV, so engine.h takes its device branches
+# (FP64 and BF16 matrix-core products, wavefront shuffles, one-barrier
+# reductions, reflectors read in place) with each wavefront instruction
+# emulated as a rendezvous of the lanes it reads. The device's own operand and
+# accumulator index arithmetic then runs on host threads, where a build under
+# AddressSanitizer or ThreadSanitizer checks it.
+
+DEV_LIB = os.path.join(ROOT, "tests", "emul", "libcatn_emul_dev.so")
+
+
+@pytest.fixture(scope="module")
+def devctx(catn):
+    if not (os.path.exists(DEV_LIB) and os.path.exists(ST_LIB)):
+        pytest.skip("emulators not built (run __graft_entry__.build())")
+    return catn.Context(lib=DEV_LIB), catn.Context(lib=ST_LIB)
+
+
+def _wave_ops():
+    lib = ctypes.CDLL(DEV_LIB)
+    out = (ctypes.c_long * 3)()
+    lib.catn_emu_wave_ops(out)
+    return list(out)          # FP64 matrix-core, BF16 matrix-core, shuffles (lane 0)
+
+
+def _dev_divergent():
+    lib = ctypes.CDLL(DEV_LIB)
+    lib.catn_emu_divergent_barriers.restype = ctypes.c_long
+    return lib.catn_emu_divergent_barriers()
+
+
+@pytest.mark.parametrize("case", ["sg40_D8_chi16", "sg16_f32x6"])
+def test_device_paths_cut_gradient(catn, devctx, case):
+    """Gradient runs across Dmax cuts through the device code paths at 256
+    lanes. sg40_D8_chi16 is the workload of the cut-gradient case that faulted
+    the GPU: it matches the one-thread build and executes no matrix-core
+    product (every product there is below the tiled-GEMM threshold), only
+    wavefront shuffles. sg16_f32x6 stores FP32 and runs its swap thetas as
+    6-pass split-BF16 matrix-core products."""
+    dev, st = devctx
+    if case == "sg40_D8_chi16":
+        n, edges, w, h = Cf.spin_glass_3reg(40, seed=1)
+        kw = dict(Dmax=8, chi=16)
+    else:
+        n, edges, w, h = Cf.spin_glass_3reg(16, seed=4)
+        kw = dict(Dmax=6, chi=12, dtype="f32x6")
+    _wave_ops()
+    before = _dev_divergent()
+    a = _run(catn, dev, n, edges, w, h, 1.0, **kw)
+    ops = _wave_ops()
+    b = _run(catn, st, n, edges, w, h, 1.0, **kw)
+    assert _dev_divergent() == before
+    assert a[4] == b[4]
+    f32 = "dtype" in kw
+    assert abs(a[0] - b[0]) / n < (1e-7 if f32 else 1e-13)
+    scale = max(1.0, np.abs(b[2]).max(), np.abs(b[3]).max())
+    gtol = 1e-4 if f32 else 1e-9
+    assert np.abs(a[2] - b[2]).max() < gtol * scale
+    assert np.abs(a[3] - b[3]).max() < gtol * scale
+    assert ops[2] > 0                       # the reductions ran as shuffles
+    if f32:
+        assert ops[1] > 0                   # split-BF16 theta products
+    else:
+        assert ops[0] == 0 and ops[1] == 0  # no matrix-core product in this workload
+
+
+def test_device_paths_matrix_cores(catn, devctx):
+    """A 160 x 100 SVD at an LDS budget of 5056 doubles: blocked Householder
+    with the compact-WY products, the T factors and the inverse bound on the
+    FP64 matrix cores (emulated), equal to the one-thread build's tiled FMA
+    products to rounding."""
+    dev, st = devctx
+    A = np.random.default_rng(9).standard_normal((160, 100))
+    res = []
+    _wave_ops()
+    for ctx in (dev, st):
+        ctx.set_launch(256, 5056, 0, 0)
+        try:
+            res.append(catn.tsvd(A, cutoff=0.0, ctx=ctx))
+        finally:
+            ctx.set_launch(256, 0, 0, 0)
+    ops = _wave_ops()
+    assert ops[0] > 1000
+    (U0, S0, V0, _, _), (U1, S1, V1, _, _) = res
+    assert np.allclose(S0, S1, rtol=1e-13, atol=1e-13 * S1[0])
+    assert np.allclose(U0 @ np.diag(S0) @ V0.T, A, atol=1e-11 * S1[0])
+    assert np.allclose(U0.T @ U0, np.eye(100), atol=1e-12)
