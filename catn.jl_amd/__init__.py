@@ -88,6 +88,7 @@ _SIGS = {
     "catn_bwd_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
     "catn_net_info": (C.c_int, [_P, _DP, _I32P, _I32P, _I32P, _DP]),
     "catn_net_edge_log": (C.c_int, [_P, _I32P, _I32P]),
+    "catn_net_plan_trace": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P]),
     "catn_net_mem": (C.c_int, [_P, _I64P, _I64P]),
     "catn_net_profile": (C.c_int, [_P, _I64P, _I32P, _I64P, _I32P, _DP]),
     "catn_net_work": (C.c_int, [_P, _DP, _DP, _I64P]),
@@ -298,6 +299,27 @@ class TensorNetwork:
         cap = C.c_int32(cnt.value)
         self.ctx._check(self.ctx.lib.catn_net_edge_log(self.h, buf.ctypes.data_as(_I32P), C.byref(cap)))
         return [(int(buf[2 * k]), int(buf[2 * k + 1])) for k in range(cnt.value)]
+
+    def plan_trace(self):
+        """Structural trace of contraction! (src/tensor_network.jl:574-708):
+        steps: (sel_i, sel_j, i, j, reversed_i, reversed_j, idx_j_in_i,
+        idx_i_in_j) per step; dups: (step, k, cross) per duplicate bond;
+        cuts: the kept count of every Dmax cut."""
+        n = [C.c_int32(0) for _ in range(3)]
+        self.ctx._check(self.ctx.lib.catn_net_plan_trace(self.h, None, C.byref(n[0]), None,
+                                                         C.byref(n[1]), None, C.byref(n[2])))
+        bufs = [np.zeros(w * max(k.value, 1), dtype=np.int32) for w, k in zip((8, 3, 1), n)]
+        caps = [C.c_int32(k.value) for k in n]
+        self.ctx._check(self.ctx.lib.catn_net_plan_trace(
+            self.h, bufs[0].ctypes.data_as(_I32P), C.byref(caps[0]), bufs[1].ctypes.data_as(_I32P),
+            C.byref(caps[1]), bufs[2].ctypes.data_as(_I32P), C.byref(caps[2])))
+        steps = [tuple(int(x) for x in bufs[0][8 * k:8 * k + 8]) for k in range(n[0].value)]
+        dups = [tuple(int(x) for x in bufs[1][3 * k:3 * k + 3]) for k in range(n[1].value)]
+        return {"steps": steps, "dups": dups, "cuts": [int(x) for x in bufs[2][:n[2].value]]}
+
+    def cut_kept(self):
+        """Kept count of every Dmax cut of the contraction, in order."""
+        return self.plan_trace()["cuts"]
 
     OPNAMES = ["nop", "cano_r", "cano_l", "swap", "permute321", "eat_a", "eat_b", "eat_c",
                "normalize", "merge_fuse", "cut", "compress_opt", "compress", "raw_split",

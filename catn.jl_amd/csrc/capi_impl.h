@@ -1116,6 +1116,22 @@ int catn_net_edge_log(catn_net* net, int32_t* pairs, int32_t* count) {
   return 0;
 }
 
+int catn_net_plan_trace(catn_net* net, int32_t* steps, int32_t* nsteps, int32_t* dups,
+                        int32_t* ndups, int32_t* cuts, int32_t* ncuts) {
+  const auto give = [](const std::vector<int32_t>& v, int w, int32_t* out, int32_t* n) {
+    if (!n) return;
+    const int tot = (int)v.size() / w;
+    if (out)
+      for (int k = 0; k < tot && k < *n; ++k)
+        for (int q = 0; q < w; ++q) out[w * k + q] = v[(size_t)w * k + q];
+    *n = tot;
+  };
+  give(net->pl.trace_steps, 8, steps, nsteps);
+  give(net->pl.trace_dups, 3, dups, ndups);
+  give(net->pl.cut_kept, 1, cuts, ncuts);
+  return 0;
+}
+
 int catn_net_profile(catn_net* net, int64_t* op_ticks, int32_t* op_count, int64_t* svd_ticks,
                      int32_t* svd_count, double* tick_hz) {
   for (int k = 0; k < 16; ++k) {

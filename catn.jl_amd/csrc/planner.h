@@ -98,6 +98,12 @@ class Planner {
   std::vector<Op> prog;
   bool acc_err = true;          // whether swap discards count into `error`
   std::vector<std::pair<int, int>> edge_log;   // selected (i, j) per step
+  // structural trace of every step (for inspection and tests): 8 ints per
+  // step: the selected pair, the pair after the order(i) >= order(j) swap,
+  // whether node i / node j was reversed, and idx_j_in_i / idx_i_in_j after
+  // the reversals; 3 ints per duplicate bond: step, k, cross; and the kept
+  // count of every Dmax cut in order
+  std::vector<int32_t> trace_steps, trace_dups, cut_kept;
 
   // ---- emission helpers ---------------------------------------------------
   void emit(int code, int a = 0, int b = 0, int c = 0, int d = 0, int e = 0, int f = 0,
@@ -440,6 +446,7 @@ class Planner {
 
   // apply the device's kept count to both physical legs of the cut bond
   void apply_cut(int myd) {
+    cut_kept.push_back(myd);
     node(pending_cut_[0]).phys[pending_cut_[1] - 1] = myd;
     node(pending_cut_[2]).phys[pending_cut_[3] - 1] = myd;
   }
@@ -653,6 +660,7 @@ class Planner {
            maxdim_intermediate == o.maxdim_intermediate && acc_err == o.acc_err &&
            std::equal(pending_cut_, pending_cut_ + 5, o.pending_cut_) && nodes == o.nodes &&
            edge_count == o.edge_count && edge_log == o.edge_log && prog.size() == o.prog.size() &&
+           trace_steps == o.trace_steps && trace_dups == o.trace_dups && cut_kept == o.cut_kept &&
            braket == o.braket && bk_edges == o.bk_edges && bk_cuts_ == o.bk_cuts_;
   }
 
@@ -744,17 +752,23 @@ class Planner {
     std::vector<int> neigh_j = nj.nb;
     int idx_j_in_i = find_neighbor(ni, j);
     int idx_i_in_j = find_neighbor(nj, i);
+    bool rev_i = false, rev_j = false;
     if (p.reverse) {                                            // :608-619
       if (idx_j_in_i < cld2((int)ni.nb.size())) {
         reverse(ni);
         idx_j_in_i = find_neighbor(ni, j);
+        rev_i = true;
       }
       if (idx_i_in_j >= cld2((int)nj.nb.size())) {
         reverse(nj);
         neigh_j = nj.nb;
         idx_i_in_j = find_neighbor(nj, i);
+        rev_j = true;
       }
     }
+    const int32_t step_no = (int32_t)edge_log.size() - 1;
+    const int32_t rec[8] = {e.first, e.second, i, j, rev_i, rev_j, idx_j_in_i, idx_i_in_j};
+    trace_steps.insert(trace_steps.end(), rec, rec + 8);
     dups_.clear();
     dup_pos_ = 0;
     acc_err = true;
@@ -770,6 +784,9 @@ class Planner {
       if (idx_k_in_i > 0) {
         dups_.push_back(k);
         const bool cross = idx_i_in_k > idx_j_in_k;
+        trace_dups.push_back(step_no);
+        trace_dups.push_back(k);
+        trace_dups.push_back(cross ? 1 : 0);
         merge(nk, i, cross);
       }
     }

@@ -148,6 +148,17 @@ int catn_net_info(catn_net* net, double* sign, int32_t* maxdim_intermediate,
                   int32_t* num_isolated, int32_t* n, double* beta);
 /* The selected edge sequence (pairs, 2*count ints); *count in: capacity. */
 int catn_net_edge_log(catn_net* net, int32_t* pairs, int32_t* count);
+/* Structural trace of contraction! (src/tensor_network.jl:574-708), for
+ * inspection and tests; no counterpart in the reference, whose locals these
+ * are. `steps` gets 8 ints per step: the selected pair (i, j), the pair after
+ * the order(i) >= order(j) swap, whether reverse! was applied to node i and
+ * to node j (:608-619), and idx_j_in_i, idx_i_in_j after the reversals.
+ * `dups` gets 3 ints per duplicate bond met while re-pointing j's neighbours
+ * (:627-650): step (0-based), k, cross. `cuts` gets the kept count of every
+ * Dmax cut (cut_bondim_opt!, :399-477) in order. Each *n is in: capacity (in
+ * records), out: the number available; any array may be NULL. */
+int catn_net_plan_trace(catn_net* net, int32_t* steps, int32_t* nsteps, int32_t* dups,
+                        int32_t* ndups, int32_t* cuts, int32_t* ncuts);
 /* Device-side profile of the ops executed for this network: wall-clock ticks
  * and counts per opcode (16 entries) and per SVD size bucket (8 entries:
  * min(m,n) <= 4, 8, 16, 32, 64, 128, 256, larger); tick_hz converts ticks. */

@@ -41,6 +41,11 @@ class Schedule:
     def __init__(self):
         self.events: List[tuple] = []
         self.edges: List[Tuple[int, int]] = []
+        # structural trace of contraction!: per step (sel_i, sel_j, i, j,
+        # reversed_i, reversed_j, idx_j_in_i, idx_i_in_j); per duplicate bond
+        # (step, k, cross)
+        self.steps: List[tuple] = []
+        self.dups: List[tuple] = []
         self.enabled = True
 
     def log(self, kind, shape, nkeep, S, cutoff):
@@ -808,6 +813,7 @@ def _contraction(tn, sched):
             i, j = select_edge_sequentially(tn)
         if sched is not None:
             sched.edges.append((i, j))
+        sel = (i, j)
         ni, nj = tn.tensors[i], tn.tensors[j]
         if order(nj) > order(ni):
             i, j = j, i
@@ -818,15 +824,20 @@ def _contraction(tn, sched):
         neigh_j = list(nj.neighbor)
         idx_j_in_i = find_neighbor(ni, j)
         idx_i_in_j = find_neighbor(nj, i)
+        rev_i = rev_j = False
         if tn.reverse:
             # 1-based comparisons of src/tensor_network.jl:609 and :614
             if idx_j_in_i < _cld(len(ni.neighbor), 2):
                 reverse(ni)
                 idx_j_in_i = find_neighbor(ni, j)
+                rev_i = True
             if idx_i_in_j >= _cld(len(nj.neighbor), 2):
                 reverse(nj)
                 neigh_j = list(nj.neighbor)
                 idx_i_in_j = find_neighbor(nj, i)
+                rev_j = True
+        if sched is not None:
+            sched.steps.append(sel + (i, j, int(rev_i), int(rev_j), idx_j_in_i, idx_i_in_j))
         duplicate = []
         for l in range(1, len(neigh_j) + 1):
             if l == idx_i_in_j:
@@ -842,6 +853,8 @@ def _contraction(tn, sched):
             if idx_k_in_i > 0:
                 duplicate.append(k)
                 cross = idx_i_in_k > idx_j_in_k
+                if sched is not None:
+                    sched.dups.append((len(sched.steps) - 1, k, int(cross)))
                 err += merge(nk, i, cross=cross)
         ln, e, phase = eat(ni, nj, idx_j_in_i, idx_i_in_j)
         err += e

@@ -362,8 +362,10 @@ def test_magnetization_correlation_brute_force(env):
 
 
 def test_random_graph_against_oracle_bit_exact_schedule(env):
-    """Planner (C++) vs oracle (Python): identical edge sequences and Dmax-cut
-    kept counts; lnZ/N to 1e-12 on random graphs with truncation."""
+    """Planner (C++) vs oracle (Python): identical edge sequences, structural
+    traces (selected pair, order swap, reversals, leg indices, duplicate bonds
+    with their cross flags) and Dmax-cut kept counts; lnZ/N to 1e-12 on
+    random graphs with truncation."""
     catn, ctx = env
     for seed in range(4):
         n, edges, w, h = Cf.spin_glass_3reg(16, seed=seed)
@@ -374,6 +376,9 @@ def test_random_graph_against_oracle_bit_exact_schedule(env):
             s = O.Schedule()
             lo, eo, po = O.contraction(ref, s)
             assert tn.edge_log() == s.edges
+            t = tn.plan_trace()
+            assert t["steps"] == s.steps and t["dups"] == s.dups
+            assert t["cuts"] == [e[2] for e in s.events if e[0] == "cut_keep"]
             assert abs(lz - lo) / n < 1e-12
             assert np.isclose(err, eo, rtol=1e-6, atol=1e-13)
 
@@ -386,6 +391,7 @@ def test_golden(env, name):
                             Dmax=g["Dmax"], chi=g["chi"], ctx=ctx)
     lz, err, psi = catn.contraction(tn)
     assert [list(e) for e in tn.edge_log()] == g["edge_sequence"]
+    assert tn.cut_kept() == g["cut_kept"]
     assert abs(lz - g["lnZ"]) / g["n"] < 1e-12
     assert np.isclose(err, g["error"], rtol=1e-6, atol=1e-14)
     assert tn.maxdim_intermediate == g["maxdim_intermediate"]
