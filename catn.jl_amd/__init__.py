@@ -53,7 +53,8 @@ class _Params(C.Structure):
         ("svdopt", C.c_int32), ("swapopt", C.c_int32), ("compress", C.c_int32),
         ("cut_bond", C.c_int32), ("seed", C.c_int32), ("arena_elems", C.c_int64),
         ("scratch_elems", C.c_int64), ("rsvd_both", C.c_int32), ("rsvd_any", C.c_int32),
-        ("precision", C.c_int32), ("loops", C.c_int32),
+        ("precision", C.c_int32), ("loops", C.c_int32), ("braket_dmax", C.c_int32),
+        ("eltype", C.c_int32),
     ]
 
 
@@ -224,7 +225,7 @@ def default_context() -> Context:
 def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse=True,
             svdopt=True, swapopt=True, compress=False, cut_bond=False, seed=1,
             arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000, dtype="f64",
-            loops=False):
+            loops=False, eltype="real"):
     p = _Params()
     ctx.lib.catn_params_default(C.byref(p))
     p.Dmax, p.chi, p.cutoff = int(Dmax), int(chi), float(cutoff)
@@ -239,6 +240,10 @@ def _params(ctx, Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse
                         f"3-pass theta) or 'f32x6' (6-pass theta), got {dtype!r}")
     p.precision = levels[dtype]
     p.loops = int(bool(loops))
+    kinds = {"real": 0, "complex": 1}
+    if eltype not in kinds:
+        raise CatnError(f"eltype must be 'real' or 'complex', got {eltype!r}")
+    p.eltype = kinds[eltype]
     return p
 
 
@@ -475,8 +480,9 @@ def braket_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
     dm = np.ascontiguousarray(np.asarray(dims, dtype=np.int64))
     lb = np.ascontiguousarray(np.asarray(labels, dtype=np.int64))
     dt = _f64(np.concatenate(data) if data else np.zeros(0))
-    kw.setdefault("Dmax", -1)
+    braket_dmax = kw.pop("Dmax", -1)
     p = _params(ctx, chi=chi, **kw)
+    p.braket_dmax = int(braket_dmax)
     out = _P()
     ctx._check(ctx.lib.catn_braket_network(ctx.h, len(tensors), nd.ctypes.data_as(_I32P),
                                            dm.ctypes.data_as(_I64P), _dptr(dt),

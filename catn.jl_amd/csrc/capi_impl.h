@@ -694,6 +694,7 @@ void catn_params_default(catn_params* p) {
   p->reverse = 1; p->svdopt = 1; p->swapopt = 1; p->compress = 0; p->cut_bond = 0;
   p->seed = 1; p->arena_elems = 0; p->scratch_elems = 0;
   p->rsvd_both = 7000; p->rsvd_any = 20000; p->precision = 0; p->loops = 0;
+  p->braket_dmax = -1; p->eltype = 0;
 }
 
 const char* catn_last_error(void) { return catn::g_last_error.c_str(); }
@@ -731,6 +732,10 @@ static void apply_params(catn_net* net, const catn_params* p) {
   q.rsvd_both = p->rsvd_both;
   q.rsvd_any = p->rsvd_any;
   q.loops = p->loops != 0;
+  if (p->eltype == 1)
+    throw catn::Error(-2, "complex element types (ComplexF64 / ComplexF32) are not built in this "
+                          "library: the engine computes real networks");
+  if (p->eltype != 0) throw catn::Error(-2, "eltype must be 0 (real) or 1 (complex)");
   if (p->precision < 0 || p->precision > 2)
     throw catn::Error(-2, "precision must be 0 (FP64), 1 (FP32, split-BF16 3-pass) or 2 (FP32, 6-pass)");
   net->f32 = p->precision != 0;
@@ -824,6 +829,11 @@ int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
   std::unique_ptr<catn_net> net(new catn_net);
   net->attach(ctx);
   apply_params(net.get(), p);
+  {   // the double layer's merged-bond Dmax is opt-in (braket_dmax); `Dmax` is not used
+    catn_params d;
+    catn_params_default(&d);
+    net->pl.p.Dmax = (p ? p : &d)->braket_dmax;
+  }
   std::map<int64_t, std::vector<int>> owners;   // label -> tensor ids (1-based)
   std::vector<int64_t> first_seen;
   size_t li = 0;

@@ -52,12 +52,21 @@ typedef struct catn_params {
    * cycle, the two bonds to the shared neighbour are merged on both nodes, ket
    * and bra legs separately, the older bond as the fast index (merge!,
    * src/mps_node.jl:595-622). Two bonds between one pair of tensors are still
-   * rejected. With Dmax > 0, a merged ket or bra bond wider than Dmax is then
-   * cut as cut_bondim_opt! does (the reference's double layer has no Dmax). */
+   * rejected. With braket_dmax > 0, a merged ket or bra bond wider than
+   * braket_dmax is then cut as cut_bondim_opt! does. */
   int32_t loops;
+  /* catn_braket_network only: the Dmax of merged bonds with loops = 1; < 0
+   * (the default) cuts nothing. The reference's double layer has no Dmax, so
+   * this is opt-in; catn_braket_network ignores `Dmax`. */
+  int32_t braket_dmax;
+  /* Element type: 0 real (Float64, or Float32 storage per `precision`), 1
+   * complex (ComplexF64 / ComplexF32). This build computes real networks
+   * only; 1 is rejected with an error naming the missing element type. */
+  int32_t eltype;
 } catn_params;
 
-/* Fill `p` with the reference defaults (Dmax=32, chi=32, cutoff=1e-15, ...). */
+/* Fill `p` with the reference defaults (Dmax=32, chi=32, cutoff=1e-15, ...;
+ * braket_dmax = -1, eltype = 0). */
 void catn_params_default(catn_params* p);
 
 /* Create a context on HIP device `device` (one stream). */

@@ -1,0 +1,267 @@
+# CATNHip — Julia binding of libcatn_hip (include/catn_hip.h) for CATN.jl users.
+#
+# Replaces the reference's GPU dispatch (TensorNetwork{T,AT} with AT = CuArray,
+# src/tensor_network.jl:1, src/adapt.jl:6-24) for the contraction path: networks
+# live on the MI355X and contraction!, free_energy, magnetization and correlation
+# run through the library. ising_lnZ carries a ChainRulesCore rrule whose
+# pullback is the library's hand-written reverse sweep (catn_gradient_batch +
+# catn_ising_gradient), so Zygote / ChainRules users get d lnZ / d(w, h, beta)
+# from one backward pass.
+#
+# No Julia toolchain exists on the build image, so this file is not executed
+# by the test suite; every ccall signature mirrors include/catn_hip.h, and the
+# Python mirror (catn.jl_amd/__init__.py) exercises the same entry points.
+module CATNHip
+
+using ChainRulesCore
+
+export ising_network, tensor_network, braket_network, contraction!, free_energy,
+       magnetization, correlation, ising_lnZ, ising_lnZ_batch, result_tensor, set_launch!
+
+const lib = get(ENV, "CATN_HIP_LIB", joinpath(@__DIR__, "..", "catn.jl_amd", "lib", "libcatn_hip.so"))
+
+# layout of catn_params, field for field (include/catn_hip.h)
+struct Params
+    Dmax::Int32
+    chi::Int32
+    cutoff::Float64
+    norm_method::Int32
+    select::Int32
+    reverse::Int32
+    svdopt::Int32
+    swapopt::Int32
+    compress::Int32
+    cut_bond::Int32
+    seed::Int32
+    arena_elems::Int64
+    scratch_elems::Int64
+    rsvd_both::Int32       # swap!'s randomized-SVD switch (src/mps_node.jl:154)
+    rsvd_any::Int32
+    precision::Int32       # 0 Float64; 1 Float32 storage, 3-pass BF16; 2 Float32, 6-pass
+    loops::Int32           # braket_network only: 1 accepts cyclic bond graphs
+    braket_dmax::Int32     # braket_network only: merged-bond Dmax (< 0: none)
+    eltype::Int32          # 0 real, 1 complex (refused by this build)
+end
+
+lasterr() = unsafe_string(ccall((:catn_last_error, lib), Cstring, ()))
+check(rc) = rc == 0 || error("libcatn_hip: " * lasterr())
+
+const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+function ctx()
+    if CTX[] == C_NULL
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:catn_create, lib), Cint, (Ref{Ptr{Cvoid}}, Cint), r, 0))
+        CTX[] = r[]
+    end
+    CTX[]
+end
+
+function params(; Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, reverse=true,
+                svdopt=true, swapopt=true, compress=false, cut_bond=false, seed=1,
+                arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000,
+                loops=false, braket_dmax=-1, T::Type=Float64)
+    prec = T === Float64 ? 0 : T === Float32 ? 1 :
+           T <: Complex ? error("element type $T: complex networks are not built in libcatn_hip") :
+           error("element type $T is not supported")
+    Params(Dmax, chi, cutoff, norm_method, select, reverse, svdopt, swapopt, compress,
+           cut_bond, seed, arena_elems, scratch_elems, rsvd_both, rsvd_any, prec, loops,
+           braket_dmax, 0)
+end
+
+# stands in for TensorNetwork{T,AT}: the library owns the device state
+mutable struct HipNetwork
+    ptr::Ptr{Cvoid}
+    n::Int
+    beta::Float64
+    nedges::Int
+end
+function wrap(ptr, n, beta, nedges)
+    tn = HipNetwork(ptr, n, beta, nedges)
+    finalizer(t -> ccall((:catn_net_free, lib), Cint, (Ptr{Cvoid},), t.ptr), tn)
+end
+
+# ising_network (src/ising.jl:20-227)
+function ising_network(n, edges, w, h, β; pos1=0, val1=0, pos2=0, val2=0, kw...)
+    e = Int32[x for (i, j) in edges for x in (i, j)]
+    p = Ref(params(; kw...))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:catn_ising_network, lib), Cint,
+                (Ptr{Cvoid}, Int32, Int32, Ptr{Int32}, Ptr{Float64}, Ptr{Float64}, Float64,
+                 Int32, Int32, Int32, Int32, Ref{Params}, Ref{Ptr{Cvoid}}),
+                ctx(), n, length(edges), e, Float64.(collect(w)), Float64.(collect(h)), β,
+                pos1, val1, pos2, val2, p, out))
+    wrap(out[], n, Float64(β), length(edges))
+end
+
+# labels -> Int64 ids in order of first appearance (open labels get the
+# device's sentinels -1, -2, ... in that order)
+function _flatten(tensors, ixs)
+    ids = Dict{Any,Int64}()
+    labels = Int64[]
+    for ix in ixs, l in ix
+        push!(labels, get!(ids, l, length(ids)))
+    end
+    nds = Int32[ndims(t) for t in tensors]
+    dims = Int64[d for t in tensors for d in size(t)]
+    data = reduce(vcat, [vec(Float64.(t)) for t in tensors]; init=Float64[])
+    nds, dims, data, labels
+end
+
+# TensorNetwork(tensors, ixs; kw...) (src/tensor_network.jl:41-140)
+function tensor_network(tensors, ixs; kw...)
+    nd, dm, dt, lb = _flatten(tensors, ixs)
+    p = Ref(params(; kw...))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:catn_tensor_network, lib), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Int64}, Ptr{Float64}, Ptr{Int64},
+                 Ref{Params}, Ref{Ptr{Cvoid}}), ctx(), length(tensors), nd, dm, dt, lb, p, out))
+    wrap(out[], length(tensors), 1.0, 0)
+end
+
+# braket_network(tensors, ixs; chi, cutoff, ...) (src/braket.jl:369-496), real tensors
+function braket_network(tensors, ixs; chi=64, loops=false, Dmax=-1, kw...)
+    nd, dm, dt, lb = _flatten(tensors, ixs)
+    p = Ref(params(; chi=chi, loops=loops, braket_dmax=Dmax, kw...))
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:catn_braket_network, lib), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Int64}, Ptr{Float64}, Ptr{Int64},
+                 Ref{Params}, Ref{Ptr{Cvoid}}), ctx(), length(tensors), nd, dm, dt, lb, p, out))
+    wrap(out[], length(tensors), 1.0, 0)
+end
+
+# contraction!(tn) (src/tensor_network.jl:556) -> (lnZ, error, psi)
+function contraction!(tn::HipNetwork)
+    lz, err, psi = Ref(0.0), Ref(0.0), Ref(0.0)
+    check(ccall((:catn_contraction, lib), Cint,
+                (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}, Ref{Float64}), tn.ptr, lz, err, psi))
+    (lz[], err[], psi[])
+end
+
+# batched contraction! of many networks in one persistent launch per segment
+function contraction!(nets::Vector{HipNetwork})
+    k = length(nets)
+    lz, err, psi = zeros(k), zeros(k), zeros(k)
+    check(ccall((:catn_contraction_batch, lib), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                ctx(), k, [t.ptr for t in nets], lz, err, psi))
+    lz, err, psi
+end
+
+# free_energy (src/ising.jl:239-244) -> (lnZ/n, -lnZ/(n β))
+function free_energy(tn::HipNetwork)
+    lnZ, _, _ = contraction!(tn)
+    (lnZ / tn.n, -lnZ / (tn.n * tn.beta))
+end
+
+# result_tensor (src/tensor_network.jl:517-523), after contraction!
+function result_tensor(tn::HipNetwork)
+    rank = Ref{Int32}(0); dims = zeros(Int32, 64); labs = zeros(Int32, 64)
+    check(ccall((:catn_result_tensor, lib), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Int64, Ref{Int32}, Ptr{Int32}, Ptr{Int32}, Int32),
+                tn.ptr, C_NULL, 0, rank, dims, labs, 64))
+    rank[] == 0 && return fill(1.0)
+    out = zeros(Float64, Tuple(Int.(dims[1:rank[]])))
+    check(ccall((:catn_result_tensor, lib), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Int64, Ref{Int32}, Ptr{Int32}, Ptr{Int32}, Int32),
+                tn.ptr, out, length(out), rank, dims, labs, 64))
+    out
+end
+
+# ---------------------------------------------------------------------------
+# chosen form: lnZ and its gradient from one recorded forward and one reverse
+# sweep per batch (catn_gradient_batch), read back per network as
+# d lnZ / d w (one per edge), d lnZ / d h (one per spin), d lnZ / d beta
+function lnz_and_grad!(nets::Vector{HipNetwork})
+    k = length(nets)
+    lz, err, psi = zeros(k), zeros(k), zeros(k)
+    check(ccall((:catn_gradient_batch, lib), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Int32, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}), ctx(), k, [t.ptr for t in nets], 0, lz, err, psi))
+    map(enumerate(nets)) do (q, t)
+        dw, dh, db = zeros(t.nedges), zeros(t.n), Ref(0.0)
+        check(ccall((:catn_ising_gradient, lib), Cint,
+                    (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ref{Float64}), t.ptr, dw, dh, db))
+        (lz[q], dw, dh, db[])
+    end
+end
+
+"""
+    ising_lnZ(n, edges, w, h, β; kw...) -> lnZ
+
+lnZ of ising_network(n, edges, w, h, β; kw...) by contraction! on the device.
+Differentiable: the rrule's pullback is the library's reverse sweep, giving
+d lnZ/d w = β<s_i s_j>, d lnZ/d h = β<s_i> and d lnZ/d β = <E> at the executed
+schedule. The library refuses a gradient run whose schedule reaches a Dmax cut
+on the gfx950 device (an error is raised); use Dmax = -1 or the pinning route.
+"""
+ising_lnZ(n, edges, w, h, β; kw...) = contraction!(ising_network(n, edges, w, h, β; kw...))[1]
+
+function ChainRulesCore.rrule(::typeof(ising_lnZ), n, edges, w, h, β; kw...)
+    lz, dw, dh, db = lnz_and_grad!([ising_network(n, edges, w, h, β; kw...)])[1]
+    function ising_lnZ_pullback(Δ)
+        δ = unthunk(Δ)
+        (NoTangent(), NoTangent(), NoTangent(), δ .* dw, δ .* dh, δ * db)
+    end
+    lz, ising_lnZ_pullback
+end
+
+"""
+    ising_lnZ_batch(instances; kw...) -> Vector of lnZ
+
+Batched form: `instances` is a vector of (n, edges, w, h, β); every instance's
+forward and reverse sweep run in the same launches.
+"""
+ising_lnZ_batch(instances; kw...) =
+    first.(contraction!([ising_network(i...; kw...) for i in instances]))
+
+function ChainRulesCore.rrule(::typeof(ising_lnZ_batch), instances; kw...)
+    res = lnz_and_grad!([ising_network(i...; kw...) for i in instances])
+    function ising_lnZ_batch_pullback(Δ)
+        δ = unthunk(Δ)
+        grads = [Tangent{Tuple}(NoTangent(), NoTangent(), δ[q] .* r[2], δ[q] .* r[3], δ[q] * r[4])
+                 for (q, r) in enumerate(res)]
+        (NoTangent(), grads)
+    end
+    [r[1] for r in res], ising_lnZ_batch_pullback
+end
+
+# magnetization / correlation (src/ising.jl:263-339). method = :pinning runs the
+# reference's 1 + 2n / 1 + 4|E| contractions in one batch; method = :adjoint uses
+# one reverse sweep (exact isolated-spin values, tanh(β h)).
+function magnetization(n, edges, w, h, β; method=:pinning, kw...)
+    if method === :adjoint
+        _, _, dh, _ = lnz_and_grad!([ising_network(n, edges, w, h, β; kw...)])[1]
+        return dh ./ β
+    end
+    nets = [ising_network(n, edges, w, h, β; kw...)]
+    for k in 1:n, v in (1, 2)
+        push!(nets, ising_network(n, edges, w, h, β; pos1=k, val1=v, kw...))
+    end
+    lz, _, _ = contraction!(nets)
+    # network 2k pins spin k to -1 (val1 = 1 zeroes the +1 value), 2k + 1 to +1
+    [exp(lz[2k + 1] - lz[1]) - exp(lz[2k] - lz[1]) for k in 1:n]
+end
+
+function correlation(n, edges, w, h, β; method=:pinning, kw...)
+    if method === :adjoint
+        _, dw, _, _ = lnz_and_grad!([ising_network(n, edges, w, h, β; kw...)])[1]
+        return dw ./ β
+    end
+    nets = [ising_network(n, edges, w, h, β; kw...)]
+    for (a, b) in edges, v1 in (1, 2), v2 in (1, 2)
+        push!(nets, ising_network(n, edges, w, h, β; pos1=a, val1=v1, pos2=b, val2=v2, kw...))
+    end
+    lz, _, _ = contraction!(nets)
+    map(1:length(edges)) do e
+        p = exp.(lz[2 + 4(e - 1):1 + 4e] .- lz[1])
+        p[1] - p[2] - p[3] + p[4]
+    end
+end
+
+# launch shape: lanes per workgroup, LDS doubles per workgroup, Householder panel
+# width, Jacobi lanes per pair (0 = automatic / default)
+set_launch!(threads=256, lds=0, panel=0, jacobi_lanes=0) =
+    check(ccall((:catn_set_launch, lib), Cint, (Ptr{Cvoid}, Int32, Int32, Int32, Int32), ctx(),
+                threads, lds, panel, jacobi_lanes))
+
+end # module

@@ -34,6 +34,10 @@ def env(request, catn):
 
 EXACT = dict(Dmax=-1, chi=10_000)
 
+
+def err_of(tn):
+    return tn.err_
+
 # Gradient runs across a Dmax cut faulted the GPU and the cause is not found
 # (the CPU emulators, also lane-parallel under ASan/TSan, run them cleanly);
 # the gfx950 executor refuses them before any launch, so their GPU cases are
@@ -49,7 +53,8 @@ def _skip_cut_gradient_on_gpu(ctx):
 
 def grad(catn, ctx, n, edges, w, h, beta, **kw):
     tn = catn.ising_network(n, edges, w, h, beta, ctx=ctx, **kw)
-    lz, _, _ = catn.gradient_batch([tn])
+    lz, err, _ = catn.gradient_batch([tn])
+    tn.err_ = float(err[0])
     dw, dh, db = catn.ising_gradient(tn, len(edges), n)
     return float(lz[0]), dw, dh, db, tn
 
@@ -172,11 +177,31 @@ def _frozen_fd(catn, ctx, n, edges, w, h, beta, idx_w, idx_h, eps=1e-4, **kw):
     return out
 
 
-@pytest.mark.parametrize("case", ["sg40_D8_chi16", "lattice6_pmJ_chi6", "lattice5_Dmax3"])
+@pytest.mark.parametrize("case", ["sg40_D8_chi16", "lattice6_pmJ_chi6", "lattice5_Dmax3",
+                                  "sg40_chi10", "lattice6_pmJ_chi4", "sg24_chi8"])
 def test_truncated_gradient_vs_finite_differences(env, case):
+    """Truncated contractions: the first three reach Dmax cuts (their GPU
+    cases are skipped, see CUT_REVERSE_SKIP); the last three truncate by chi
+    only (Dmax = -1, no cut), so the reverse sweep differentiates truncated
+    swap and canonicalisation SVDs (the broadened SVD adjoint) on every
+    backend."""
     catn, ctx = env
-    _skip_cut_gradient_on_gpu(ctx)
-    if case == "sg40_D8_chi16":
+    if "chi" not in case or "_D" in case or case == "lattice6_pmJ_chi6":
+        _skip_cut_gradient_on_gpu(ctx)
+    if case == "sg40_chi10":
+        n, edges, w, h = Cf.spin_glass_3reg(40, seed=1)
+        beta, kw = 1.0, dict(Dmax=-1, chi=10)
+    elif case == "lattice6_pmJ_chi4":
+        L = 6
+        edges = X.square_edges(L)
+        n = L * L
+        r = np.random.default_rng(0)
+        w, h = r.choice([-1.0, 1.0], len(edges)), r.normal(0, 0.1, n)
+        beta, kw = 0.8, dict(Dmax=-1, chi=4)
+    elif case == "sg24_chi8":
+        n, edges, w, h = Cf.spin_glass_3reg(24, seed=5)
+        beta, kw = 1.0, dict(Dmax=-1, chi=8)
+    elif case == "sg40_D8_chi16":
         n, edges, w, h = Cf.spin_glass_3reg(40, seed=1)
         beta, kw = 1.0, dict(Dmax=8, chi=16)
     elif case == "lattice6_pmJ_chi6":
@@ -197,6 +222,9 @@ def test_truncated_gradient_vs_finite_differences(env, case):
     ops = tn.profile()[0]
     if case == "lattice5_Dmax3":
         assert ops.get("cut", (0,))[0] > 0          # the Dmax cut is exercised
+    if kw["Dmax"] < 0:
+        assert ops.get("cut", (0,))[0] == 0
+        assert tn.mem()[0] > 0 and err_of(tn) > 0     # chi truncates
     iw = list(range(0, len(edges), max(1, len(edges) // 12)))
     ih = list(range(0, n, max(1, n // 6)))
     fd = _frozen_fd(catn, ctx, n, edges, np.array(w, float), np.array(h, float), beta, iw, ih, **kw)
@@ -210,13 +238,16 @@ def test_truncated_gradient_vs_finite_differences(env, case):
     assert checked >= 0.8 * len(fd)
 
 
-def test_adjoint_close_to_pinning_when_truncated(env):
+@pytest.mark.parametrize("kw", [dict(Dmax=6, chi=12), dict(Dmax=-1, chi=8)],
+                         ids=["Dmax6_chi12", "chi8"])
+def test_adjoint_close_to_pinning_when_truncated(env, kw):
     """With truncation the two routes differentiate slightly different
-    approximations; they agree to the truncation level."""
+    approximations; they agree to the truncation level. The chi-only case
+    reaches no Dmax cut."""
     catn, ctx = env
-    _skip_cut_gradient_on_gpu(ctx)
+    if kw["Dmax"] > 0:
+        _skip_cut_gradient_on_gpu(ctx)
     n, edges, w, h = Cf.spin_glass_3reg(24, seed=5)
-    kw = dict(Dmax=6, chi=12)
     m_adj = catn.magnetization(n, edges, w, h, 1.0, ctx=ctx, method="adjoint", **kw)
     m_pin = catn.magnetization(n, edges, w, h, 1.0, ctx=ctx, **kw)
     mag, _ = X.brute_observables(n, edges, w, h, 1.0)
