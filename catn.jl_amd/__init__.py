@@ -116,6 +116,8 @@ _SIGS = {
     "catn_result_tensor": (C.c_int, [_P, _DP, C.c_int64, _I32P, _I32P, _I32P, C.c_int32]),
     "catn_tsvd": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_double, C.c_int32, _I32P, _DP,
                             _DP, _DP, _DP]),
+    "catn_tsvd_fixed": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_void_p, C.c_double, C.c_int32,
+                                  _I32P, C.c_void_p, C.c_void_p, C.c_void_p, _DP]),
     "catn_rsvd": (C.c_int, [_P, C.c_int32, C.c_int32, _DP, C.c_int32, C.c_int32, C.c_int32,
                             C.c_int32, _I32P, _DP, _DP, _DP]),
 }

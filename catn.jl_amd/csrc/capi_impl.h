@@ -74,6 +74,18 @@ class Executor {
   }
   // the largest LDS budget (doubles) a launch of this executor may take
   virtual int lds_limit() const { return kLdsMaxDoubles; }
+  // device-to-device copy on the executor's stream (returns after completion)
+  virtual void d2d(void* dst, const void* src, size_t bytes) = 0;
+  // The library's fixed device region of the no-allocation entry points: its
+  // device address on this executor's device and its size in bytes (null if
+  // there is none). The library owns it for the life of the process.
+  virtual void* fixed_region(size_t* bytes) = 0;
+  // Run `op` on one instance whose state lives in caller-provided device
+  // memory: hs (host mirror) is copied to d_state, the op to d_op and its
+  // range to d_ranges (8 bytes), one workgroup runs it, and hs is read back.
+  // Allocates nothing.
+  virtual void run_fixed(InstState* hs, void* d_state, const Op& op, void* d_op,
+                         void* d_ranges) = 0;
 };
 
 Executor* make_executor(int device);   // provided by the executor TU
@@ -1438,6 +1450,78 @@ int catn_tsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, double cutof
   if (U) ctx->ex->d2h(U, net.hs.arena + du.off, sizeof(double) * (size_t)m * kk);
   if (V) ctx->ex->d2h(V, net.hs.arena + dv.off, sizeof(double) * (size_t)n * kk);
   if (S) ctx->ex->d2h(S, net.hs.arena + dsd.off, sizeof(double) * (size_t)dsd.dl);
+  if (k) *k = kk;
+  if (discarded) *discarded = disc;
+  return 0;
+  CATN_CATCH
+}
+
+int catn_tsvd_fixed(catn_ctx* ctx, int32_t m, int32_t n, const double* dA, double cutoff,
+                    int32_t maxdim, int32_t* k, double* dU, double* dS, double* dV,
+                    double* discarded) {
+  CATN_TRY
+  if (m <= 0 || n <= 0) throw catn::Error(-2, "tsvd_fixed: m and n must be positive");
+  catn::Executor* ex = ctx->ex.get();
+  size_t cap = 0;
+  char* base = (char*)ex->fixed_region(&cap);
+  if (!base) throw catn::Error(-2, "tsvd_fixed: this library has no fixed device region");
+  // layout of the fixed region, 256-byte aligned blocks: instance state, op,
+  // op range, 4 site descriptors, result slots, arena, scratch (the same
+  // arena and scratch sizes catn_tsvd allocates)
+  const int64_t mn = (int64_t)m * n, r = std::min(m, n), mx = std::max(m, n);
+  const int64_t acap = 4 * mn + 4 * ((int64_t)m + n) * r + 4096;
+  const int64_t scap = 8 * mn + 8 * mx * mx + 4096;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t o_state = 0;
+  const size_t o_op = al(o_state + sizeof(catn::InstState));
+  const size_t o_rng = al(o_op + sizeof(catn::Op));
+  const size_t o_sites = al(o_rng + 2 * sizeof(int32_t));
+  const size_t o_res = al(o_sites + 4 * sizeof(catn::SiteDesc));
+  const size_t o_dres = al(o_res + sizeof(int32_t));
+  const size_t o_arena = al(o_dres + sizeof(double));
+  const size_t o_scr = al(o_arena + sizeof(double) * (size_t)acap);
+  const size_t need = o_scr + sizeof(double) * (size_t)scap;
+  if (need > cap)
+    throw catn::Error(-3, "tsvd_fixed: a " + std::to_string(m) + " x " + std::to_string(n) +
+                              " matrix needs " + std::to_string(need) + " bytes of the fixed "
+                              "device region, which holds " + std::to_string(cap) +
+                              " (at most about 512 x 512)");
+  catn::InstState hs{};
+  hs.psi = 1.0;
+  hs.sign = 1.0;
+  hs.arena = (double*)(base + o_arena);
+  hs.scratch = (double*)(base + o_scr);
+  hs.sites = (catn::SiteDesc*)(base + o_sites);
+  hs.results = (int32_t*)(base + o_res);
+  hs.dresults = (double*)(base + o_dres);
+  hs.arena_cap = acap;
+  hs.scratch_cap = scap;
+  hs.nsites = 4;
+  hs.nresults = 1;
+  hs.seed = 1;
+  hs.arena_top = (mn + 31) & ~int64_t(31);   // A at offset 0
+  // A into the arena, the descriptors (A, then three placeholders written by the op)
+  ex->d2d(hs.arena, dA, sizeof(double) * (size_t)mn);
+  catn::SiteDesc sd[4] = {};
+  sd[0].off = 0; sd[0].dl = m; sd[0].d = 1; sd[0].dr = n; sd[0].live = 1;
+  ex->h2d(hs.sites, sd, sizeof(sd));
+  catn::Op op{};
+  op.code = catn::OP_TSVD;
+  op.a = 0; op.b = 1; op.c = 2; op.f = 3; op.e = 0;
+  op.d = maxdim <= 0 ? (1 << 30) : maxdim;
+  op.x = cutoff;
+  ex->run_fixed(&hs, base + o_state, op, base + o_op, base + o_rng);
+  if (hs.status != catn::STATUS_OK)
+    throw catn::Error(-10 - hs.status, "tsvd_fixed: device op failed with status " +
+                                           std::to_string(hs.status));
+  int32_t kk = 0;
+  double disc = 0.0;
+  ex->d2h(&kk, hs.results, sizeof(int32_t));
+  ex->d2h(&disc, hs.dresults, sizeof(double));
+  ex->d2h(sd, hs.sites, sizeof(sd));
+  if (dU) ex->d2d(dU, hs.arena + sd[1].off, sizeof(double) * (size_t)m * kk);
+  if (dV) ex->d2d(dV, hs.arena + sd[2].off, sizeof(double) * (size_t)n * kk);
+  if (dS) ex->d2d(dS, hs.arena + sd[3].off, sizeof(double) * (size_t)sd[3].dl);
   if (k) *k = kk;
   if (discarded) *discarded = disc;
   return 0;

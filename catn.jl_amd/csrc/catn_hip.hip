@@ -15,9 +15,20 @@
 #include "engine.h"
 #include "capi_impl.h"
 
+// Fixed device region of the no-allocation entry points (catn_tsvd_fixed):
+// 64 MiB per GPU, held until the process ends. In host code `scratch` is a
+// host-side stand-in for the device array: the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed to
+// copies and kernels.
+constexpr size_t kFixedDoubles = size_t(8) << 20;
+static __device__ __align__(256) double scratch[kFixedDoubles];
+
 namespace catn {
 
 constexpr int kThreads = 256;
+constexpr int kMaxDevices = 64;
+// device address of `scratch` per device (null until the device's first executor)
+static void* g_fixed_ptr[kMaxDevices] = {};
 // static LDS of both kernels: red[16] doubles + flag, padded
 constexpr int kStaticLdsBytes = 256;
 
@@ -86,6 +97,12 @@ class HipExecutor : public Executor {
     if (lds_limit_ > kLdsMaxDoubles) lds_limit_ = kLdsMaxDoubles;
     if (lds_limit_ < kLdsMinDoubles)
       throw Error(-100, "device LDS per workgroup is below the interpreter's minimum");
+    if (device < 0 || device >= kMaxDevices) throw Error(-100, "device index out of range");
+    if (!g_fixed_ptr[device]) {   // once per device, with the device current
+      void* ptr = nullptr;
+      hipcheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+      g_fixed_ptr[device] = ptr;
+    }
     hipcheck(hipStreamCreate(&stream_), "hipStreamCreate");
     hipcheck(hipEventCreate(&ev0_), "hipEventCreate");
     hipcheck(hipEventCreate(&ev1_), "hipEventCreate");
@@ -129,6 +146,34 @@ class HipExecutor : public Executor {
   void d2h(void* dst, const void* src, size_t bytes) override {
     hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_), "d2h");
     hipcheck(hipStreamSynchronize(stream_), "d2h sync");
+  }
+  void d2d(void* dst, const void* src, size_t bytes) override {
+    hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_), "d2d");
+    hipcheck(hipStreamSynchronize(stream_), "d2d sync");
+  }
+  void* fixed_region(size_t* bytes) override {
+    if (bytes) *bytes = sizeof(double) * kFixedDoubles;
+    return g_fixed_ptr[device_];
+  }
+  // one instance in caller-provided device memory: no allocation (the
+  // interpreter kernel, its launch shape for a batch of one, the copies)
+  void run_fixed(InstState* hs, void* d_state, const Op& op, void* d_op,
+                 void* d_ranges) override {
+    const int lds_cap = lds_for(1);
+    if (lds_cap > lds_limit_) throw Error(-2, "launch LDS budget above the device limit");
+    const int32_t rng[2] = {0, 1};
+    hipcheck(hipMemcpyAsync(d_op, &op, sizeof(Op), hipMemcpyHostToDevice, stream_), "op");
+    hipcheck(hipMemcpyAsync(d_ranges, rng, sizeof(rng), hipMemcpyHostToDevice, stream_), "range");
+    hipcheck(hipMemcpyAsync(d_state, hs, sizeof(InstState), hipMemcpyHostToDevice, stream_), "state");
+    hipcheck(hipEventRecord(ev0_, stream_), "event record");
+    hipcheck(hipGetLastError(), "before launch exec_programs");
+    hipLaunchKernelGGL(exec_programs, dim3(1), dim3(threads), sizeof(double) * lds_cap, stream_,
+                       (InstState*)d_state, (const Op*)d_op, (const int*)d_ranges, lds_cap, qb,
+                       jac_lanes);
+    finish_launch("exec_programs (fixed region)");
+    hipcheck(hipMemcpyAsync(hs, d_state, sizeof(InstState), hipMemcpyDeviceToHost, stream_),
+             "state back");
+    hipcheck(hipStreamSynchronize(stream_), "state back sync");
   }
   void run(const std::vector<InstState*>& states,
            const std::vector<const std::vector<Op>*>& programs) override {

@@ -255,6 +255,21 @@ int catn_rsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, int32_t k, i
 int catn_tsvd(catn_ctx* ctx, int32_t m, int32_t n, const double* A, double cutoff,
               int32_t maxdim, int32_t* k, double* U, double* S, double* V, double* discarded);
 
+/* catn_tsvd on device memory with no allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * dA (m x n), dU (m x k), dS (min(m, n)) and dV (n x k) are device pointers the
+ * caller owns; every other buffer (instance state, op, descriptors, arena,
+ * scratch) lives in the library's fixed device region, a 64 MiB file-scope
+ * device array per GPU held until the process ends, which bounds the matrix
+ * at about 512 x 512 (larger sizes are refused with an error that gives the
+ * bytes needed). Results equal catn_tsvd's on the same matrix. Calls on one
+ * GPU share the region: calls that overlap on one GPU (other host threads,
+ * other contexts) race and must be serialised by the caller. */
+int catn_tsvd_fixed(catn_ctx* ctx, int32_t m, int32_t n, const double* dA, double cutoff,
+                    int32_t maxdim, int32_t* k, double* dU, double* dS, double* dV,
+                    double* discarded);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,21 @@ class CpuExecutor : public Executor {
   bool cut_reverse_enabled() const override { return !getenv("CATN_EMU_REFUSE_CUT_REVERSE"); }
   void h2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
   void d2h(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
+  void d2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
+  void* fixed_region(size_t* bytes) override {
+    static double region[size_t(8) << 20];   // the device library's 64 MiB fixed region
+    if (bytes) *bytes = sizeof(region);
+    return region;
+  }
+  void run_fixed(InstState* hs, void* d_state, const Op& op, void* d_op, void*) override {
+    memcpy(d_state, hs, sizeof(InstState));
+    memcpy(d_op, &op, sizeof(Op));
+    std::vector<InstState*> st = {(InstState*)d_state};
+    std::vector<Op> prog = {op};
+    std::vector<const std::vector<Op>*> pr = {&prog};
+    run(st, pr);
+    memcpy(hs, d_state, sizeof(InstState));
+  }
   void run(const std::vector<InstState*>& states,
            const std::vector<const std::vector<Op>*>& programs) override {
     double red[16];
