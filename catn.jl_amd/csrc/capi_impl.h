@@ -135,6 +135,18 @@ struct catn_net {
   std::vector<double> iw, ih;
   int ipos1 = 0, ival1 = 0, ipos2 = 0, ival2 = 0;
   bool have_adjoint = false;
+  // final planner after contraction!, shared by the networks that followed one
+  // plan (read-only); plan() reads it, own_plan() takes a private copy
+  std::shared_ptr<const catn::Planner> done_pl;
+  const catn::Planner& plan() const { return done_pl ? *done_pl : pl; }
+  catn::Planner& own_plan() {
+    if (done_pl) {
+      pl = *done_pl;
+      done_pl.reset();
+    }
+    return pl;
+  }
+  void finish_plan() { done_pl = std::make_shared<const catn::Planner>(std::move(pl)); }
   bool f32 = false;        // FP32 site storage (catn_params.precision = 1 or 2)
   int bf16_terms = 2;      // BF16 terms per FP32 operand of the theta GEMM
   size_t elem_bytes() const { return f32 ? sizeof(float) : sizeof(double); }
@@ -562,13 +574,14 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     n->ensure_device();
     n->reset_acc();
   }
-  parallel_for((int)nets.size(), [&](int k) { nets[k]->pl.start(); });
   // Planning is a deterministic function of the planner state and of the cut
-  // results applied between segments. Networks whose states are equal at the
-  // start (pinned networks of one ising instance differ only in site values)
-  // follow one owner: only the owner plans, the followers run its uploaded
-  // program, and a follower takes a copy of the owner's planner only when its
-  // cut result parts from the owner's, or once at the end.
+  // results applied between segments. Networks whose states are equal before
+  // the start (pinned networks of one ising instance differ only in site
+  // values) follow one owner: only the owner starts and plans, the followers
+  // run its uploaded program, a follower takes a copy of the owner's planner
+  // only when its cut result parts from the owner's, and at the end every
+  // follower shares its owner's final planner (read-only, copied on a later
+  // mutation).
   std::vector<catn_net*> active = nets;
   const int n0 = (int)active.size();
   std::vector<int> att(n0, -1);   // index in `active` of the owner followed, or -1
@@ -585,6 +598,10 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     parallel_for(n0, [&](int k) {   // equal fingerprints must be equal states
       if (att[k] >= 0 && !active[k]->pl.same_state(active[att[k]]->pl)) att[k] = -1;
     });
+    std::vector<int> owners;
+    for (int k = 0; k < n0; ++k)
+      if (att[k] < 0) owners.push_back(k);
+    parallel_for((int)owners.size(), [&](int q) { active[owners[q]]->pl.start(); });
     ex->plan_seconds += wall() - t0;
   }
   while (!active.empty()) {
@@ -636,13 +653,19 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     // maxdim); programs were cleared by run_nets, so nothing large is copied
     std::vector<int> copies, applies;
     for (int k = 0; k < na; ++k) {
-      if (copy_from[k] >= 0 || (!cut[k] && att[k] >= 0)) copies.push_back(k);
+      if (copy_from[k] >= 0) copies.push_back(k);
       if (cut[k] && own[k] == k) applies.push_back(k);
     }
     parallel_for((int)copies.size(), [&](int q) {
       const int k = copies[q];
-      active[k]->pl = active[copy_from[k] >= 0 ? copy_from[k] : att[k]]->pl;
+      active[k]->pl = active[copy_from[k]]->pl;
     });
+    // finished: an owner's final planner becomes shared and read-only; its
+    // finished followers refer to it
+    for (int k = 0; k < na; ++k)
+      if (!cut[k] && att[k] < 0) active[k]->finish_plan();
+    for (int k = 0; k < na; ++k)
+      if (!cut[k] && att[k] >= 0) active[k]->done_pl = active[att[k]]->done_pl;
     parallel_for((int)applies.size(), [&](int q) {
       const int k = applies[q];
       active[k]->pl.apply_cut(active[k]->hs.last_cut);
@@ -1120,7 +1143,7 @@ int catn_net_bwd_profile(catn_net* net, int64_t* ticks, double* replay_dev, int3
 int catn_net_info(catn_net* net, double* sign, int32_t* maxdim, int32_t* num_isolated, int32_t* n,
                   double* beta) {
   if (sign) *sign = net->hs.sign;
-  if (maxdim) *maxdim = net->pl.maxdim_intermediate;
+  if (maxdim) *maxdim = net->plan().maxdim_intermediate;
   if (num_isolated) *num_isolated = net->num_isolated;
   if (n) *n = net->n;
   if (beta) *beta = net->beta;
@@ -1129,10 +1152,11 @@ int catn_net_info(catn_net* net, double* sign, int32_t* maxdim, int32_t* num_iso
 
 int catn_net_edge_log(catn_net* net, int32_t* pairs, int32_t* count) {
   const int cap = *count;
-  const int tot = (int)net->pl.edge_log.size();
+  const catn::Planner& pl = net->plan();
+  const int tot = (int)pl.edge_log.size();
   for (int k = 0; k < tot && k < cap; ++k) {
-    pairs[2 * k] = net->pl.edge_log[k].first;
-    pairs[2 * k + 1] = net->pl.edge_log[k].second;
+    pairs[2 * k] = pl.edge_log[k].first;
+    pairs[2 * k + 1] = pl.edge_log[k].second;
   }
   *count = tot;
   return 0;
@@ -1148,9 +1172,10 @@ int catn_net_plan_trace(catn_net* net, int32_t* steps, int32_t* nsteps, int32_t*
         for (int q = 0; q < w; ++q) out[w * k + q] = v[(size_t)w * k + q];
     *n = tot;
   };
-  give(net->pl.trace_steps, 8, steps, nsteps);
-  give(net->pl.trace_dups, 3, dups, ndups);
-  give(net->pl.cut_kept, 1, cuts, ncuts);
+  const catn::Planner& pl = net->plan();
+  give(pl.trace_steps, 8, steps, nsteps);
+  give(pl.trace_dups, 3, dups, ndups);
+  give(pl.cut_kept, 1, cuts, ncuts);
   return 0;
 }
 
@@ -1256,6 +1281,7 @@ int catn_net_mem(catn_net* net, int64_t* used, int64_t* cap) {
 
 #define NODE_OP(body)                      \
   CATN_TRY                                 \
+  net->own_plan();                         \
   catn::PNode& nd = net->pl.node(id);      \
   (void)nd;                                \
   net->ensure_device();                    \
@@ -1301,6 +1327,7 @@ int catn_node_reverse(catn_net* net, int32_t id) {
 int catn_node_eat(catn_net* net, int32_t id_i, int32_t id_j, int32_t idx, int32_t idxi,
                   double* lognorm, double* error, double* phase) {
   CATN_TRY
+  net->own_plan();
   catn::PNode& ni = net->pl.node(id_i);
   catn::PNode& nj = net->pl.node(id_j);
   net->ensure_device();
@@ -1317,6 +1344,7 @@ int catn_node_eat(catn_net* net, int32_t id_i, int32_t id_j, int32_t idx, int32_
 
 int catn_cut_bondim(catn_net* net, int32_t i, int32_t idx_j_in_i, int32_t opt, double* error) {
   CATN_TRY
+  net->own_plan();
   if (net->pl.p.Dmax < 0) {   // no-op in exact mode (:343, :400)
     if (error) *error = 0.0;
     return 0;
@@ -1334,6 +1362,7 @@ int catn_cut_bondim(catn_net* net, int32_t i, int32_t idx_j_in_i, int32_t opt, d
 int catn_node_get(catn_net* net, int32_t id, int32_t* order, int32_t* cano, int32_t* neighbor,
                   int32_t* dims, int32_t cap) {
   CATN_TRY
+  net->own_plan();
   catn::PNode& nd = net->pl.node(id);
   if (order) *order = nd.order();
   if (cano) *cano = nd.cano;
@@ -1352,6 +1381,7 @@ int catn_node_get(catn_net* net, int32_t id, int32_t* order, int32_t* cano, int3
 
 int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t cap) {
   CATN_TRY
+  net->own_plan();
   catn::PNode& nd = net->pl.node(id);
   if (k < 1 || k > nd.order()) throw catn::Error(-2, "site index out of range");
   net->ensure_device();
@@ -1372,6 +1402,7 @@ int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t ca
 int catn_result_tensor(catn_net* net, double* out, int64_t cap, int32_t* rank, int32_t* dims,
                        int32_t* labels, int32_t dims_cap) {
   CATN_TRY
+  net->own_plan();
   // open-leg nodes: those with a site whose physical dimension is not 1
   // (src/tensor_network.jl:518-519), in ascending id
   std::vector<int> open;
