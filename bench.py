@@ -12,9 +12,11 @@ One step = one pass of each form over one batch of synthetic instances:
                (src/ising.jl:263-339): 1 + 2n + 4|E| = 801 contraction! runs
                per solve, all 801 in one batch. Reported beside the chosen form.
 
-Networks are built and uploaded to HBM before the timed region; the timed
-region is the batched contraction (host planner + device kernels), the reverse
-sweep and the host chain rule to d lnZ/d(w, h), until results are on the host.
+Networks are built before the timed region, in packed batches whose site
+values are computed on the device (catn_ising_network_batch), so they are in
+HBM when it starts; the timed region is the batched contraction (host planner
++ device kernels), the reverse sweep and the host chain rule to d lnZ/d(w, h),
+until results are on the host.
 
 Prints one JSON line. `--gpus N` with N > 1 exits: the library runs on one GPU.
 """
@@ -45,17 +47,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pinned_batch(catn, ctx, n, edges, w, h):
-    nets = [catn.ising_network(n, edges, w, h, BETA, ctx=ctx, **NET_KW)]
-    for k in range(1, n + 1):
-        for v in (1, 2):
-            nets.append(catn.ising_network(n, edges, w, h, BETA, pos1=k, val1=v, ctx=ctx, **NET_KW))
-    for (a, b) in edges:
-        for v1 in (1, 2):
-            for v2 in (1, 2):
-                nets.append(catn.ising_network(n, edges, w, h, BETA, pos1=a, val1=v1, pos2=b,
-                                               val2=v2, ctx=ctx, **NET_KW))
-    return nets
+def pinned_batch(catn, ctx, n, edges, w, h, kw=None):
+    """The 1 + 2n + 4|E| pinned networks of one solve, built in one packed batch
+    (site values computed on the device)."""
+    return catn.ising_network_batch([(n, edges, w, h)], catn.pinned_networks(n, edges, w, h, BETA),
+                                    ctx=ctx, **(kw or NET_KW))
 
 
 def gradient_from_lnz(lz, n, edges):
@@ -140,8 +136,8 @@ def run_chosen(catn, ctx, instances, steps, warmup):
     ctx.kernel_stats(reset=True)
     ctx.bwd_kernel_stats(reset=True)
     for step in range(warmup + steps):
-        nets = [catn.ising_network(n, e, w, h, BETA, ctx=ctx, **NET_KW) for (n, e, w, h) in instances]
-        _prefetch(nets)
+        nets = catn.ising_network_batch(instances, [(k, BETA) for k in range(len(instances))],
+                                        ctx=ctx, **NET_KW)
         timed = step >= warmup
         if step == warmup:
             ctx.kernel_stats(reset=True)
@@ -189,23 +185,13 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
     ct, pt = [], []
     g_adj = g_pin = lz_c = None
     for step in range(warmup + steps):
-        nets = [catn.ising_network(n, edges, w, h, BETA, ctx=ctx, **kw) for _ in range(batch)]
-        _prefetch(nets)
+        nets = catn.ising_network_batch([(n, edges, w, h)], [(0, BETA)] * batch, ctx=ctx, **kw)
         t0 = time.perf_counter()
         lz, _, _ = catn.gradient_batch(nets)
         grads = [catn.ising_gradient(t, len(edges), n)[:2] for t in nets]
         t1 = time.perf_counter()
         del nets
-        pn = [catn.ising_network(n, edges, w, h, BETA, ctx=ctx, **kw)]
-        for k in range(1, n + 1):
-            for v in (1, 2):
-                pn.append(catn.ising_network(n, edges, w, h, BETA, pos1=k, val1=v, ctx=ctx, **kw))
-        for (a, b) in edges:
-            for v1 in (1, 2):
-                for v2 in (1, 2):
-                    pn.append(catn.ising_network(n, edges, w, h, BETA, pos1=a, val1=v1, pos2=b,
-                                                 val2=v2, ctx=ctx, **kw))
-        _prefetch(pn)
+        pn = pinned_batch(catn, ctx, n, edges, w, h, kw)
         t2 = time.perf_counter()
         plz, _, _ = catn.contraction_batch(pn)
         gp = gradient_from_lnz(plz, n, edges)
@@ -277,11 +263,10 @@ def run_plain(catn, ctx, insts, steps, warmup, profile_path="", tune=True):
     tuned = None
 
     def build():
-        nets = []
-        for (nk, ek, wk, hk) in insts:
-            nets += pinned_batch(catn, ctx, nk, ek, wk, hk)
-        _prefetch(nets)
-        return nets
+        spec = []
+        for q, (nk, ek, wk, hk) in enumerate(insts):
+            spec += catn.pinned_networks(nk, ek, wk, hk, BETA, inst=q)
+        return catn.ising_network_batch(list(insts), spec, ctx=ctx, **NET_KW)
     if tune:
         # launch shape (lanes, LDS per workgroup) picked by timing this step's
         # batch once per shape, before the warmup and the timed steps

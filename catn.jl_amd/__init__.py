@@ -72,6 +72,9 @@ _SIGS = {
     "catn_ising_network": (C.c_int, [_P, C.c_int32, C.c_int32, _I32P, _DP, _DP, C.c_double,
                                      C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                      C.POINTER(_Params), C.POINTER(_P)]),
+    "catn_ising_network_batch": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _DP, _DP,
+                                           C.c_int32, _I32P, _DP, _I32P, C.POINTER(_Params),
+                                           C.POINTER(_P)]),
     "catn_tensor_network": (C.c_int, [_P, C.c_int32, _I32P, _I64P, _DP, _I64P,
                                       C.POINTER(_Params), C.POINTER(_P)]),
     "catn_net_create": (C.c_int, [_P, C.POINTER(_Params), C.POINTER(_P)]),
@@ -409,6 +412,46 @@ def ising_network(n, edges, weights, fields, beta, *, pos1=0, val1=0, pos2=0, va
         ctx.h, int(n), int(e.shape[0]), e.ctypes.data_as(_I32P), _dptr(w), _dptr(h),
         float(beta), int(pos1), int(val1), int(pos2), int(val2), C.byref(p), C.byref(out)))
     return TensorNetwork(ctx, out)
+
+
+def ising_network_batch(instances, networks, *, ctx: Optional[Context] = None,
+                        **kw) -> List[TensorNetwork]:
+    """ising_network for a batch with the site values built on the device
+    (catn_ising_network_batch). instances: (n, edges, weights, fields) tuples;
+    networks: (instance index, beta) or (instance index, beta, (pos1, val1,
+    pos2, val2)) tuples. Returns one network per entry of `networks`."""
+    ctx = ctx or default_context()
+    ns = _i32([int(q[0]) for q in instances])
+    nes = _i32([len(q[1]) for q in instances])
+    e = _i32(np.concatenate([np.asarray(q[1], np.int32).reshape(-1, 2) for q in instances
+                             if len(q[1])] or [np.zeros((0, 2), np.int32)]).reshape(-1))
+    w = _f64(np.concatenate([np.asarray(q[2], float).reshape(-1) for q in instances] or [[]]))
+    h = _f64(np.concatenate([np.asarray(q[3], float).reshape(-1) for q in instances]))
+    inst = _i32([int(t[0]) for t in networks])
+    beta = _f64([float(t[1]) for t in networks])
+    pins = _i32([list(t[2]) if len(t) > 2 else [0, 0, 0, 0] for t in networks]).reshape(-1)
+    p = _params(ctx, **kw)
+    out = (_P * len(networks))()
+    ctx._check(ctx.lib.catn_ising_network_batch(
+        ctx.h, len(instances), ns.ctypes.data_as(_I32P), nes.ctypes.data_as(_I32P),
+        e.ctypes.data_as(_I32P), _dptr(w), _dptr(h), len(networks), inst.ctypes.data_as(_I32P),
+        _dptr(beta), pins.ctypes.data_as(_I32P), C.byref(p), out))
+    return [TensorNetwork(ctx, C.c_void_p(out[k])) for k in range(len(networks))]
+
+
+def pinned_networks(n, edges, weights, fields, beta, *, inst=0):
+    """The network list of the pinning route for one instance (src/ising.jl:
+    263-339): the unpinned network, then (k, 1), (k, 2) for every spin k, then
+    (a, v1, b, v2) for every edge (a, b) and v1, v2 in (1, 2)."""
+    nets = [(inst, beta)]
+    for k in range(1, n + 1):
+        for v in (1, 2):
+            nets.append((inst, beta, (k, v, 0, 0)))
+    for (a, b) in edges:
+        for v1 in (1, 2):
+            for v2 in (1, 2):
+                nets.append((inst, beta, (a, v1, b, v2)))
+    return nets
 
 
 def tensor_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,

@@ -28,6 +28,51 @@
 
 namespace catn {
 
+// Packed description of a batch of ising networks whose site values are
+// computed on the device (catn_ising_network_batch): per instance its sites'
+// records (arena offset, kind, edge, side, node) and its slices of the
+// concatenated edge / coupling / field arrays; per network its instance, arena,
+// storage type, beta and pinning.
+struct IsingSiteRec { int64_t off; int32_t kind, edge, side, node; };
+struct IsingInstRec { int32_t site_begin, site_end; int64_t edge_off, field_off; };
+struct IsingNetRec {
+  double* arena;
+  double beta;
+  int32_t inst, f32, pos1, val1, pos2, val2;
+};
+struct IsingBuild {
+  std::vector<IsingSiteRec> sites;
+  std::vector<IsingInstRec> insts;
+  std::vector<IsingNetRec> nets;
+  const int32_t* edges = nullptr;
+  const double* w = nullptr;
+  const double* h = nullptr;
+  int64_t nedges = 0, nfields = 0;   // lengths of the concatenated arrays
+  int32_t max_sites = 0;             // largest site count of an instance
+};
+
+// one site of a packed ising build (device kernel thread or host loop)
+CATN_HD void ising_build_site(const IsingNetRec& nr, const IsingInstRec& ir, const IsingSiteRec& sr,
+                             const int32_t* edges, const double* w, const double* h) {
+  double sn[2] = {1.0, -1.0}, so[2] = {1.0, -1.0}, v[8];
+  double wv = 0.0;
+  const double hv = h[ir.field_off + sr.node - 1];
+  if (sr.kind != 4) {
+    const int i = edges[2 * (ir.edge_off + sr.edge)], j = edges[2 * (ir.edge_off + sr.edge) + 1];
+    ising_spins(sr.side == 0 ? i : j, nr.pos1, nr.val1, nr.pos2, nr.val2, sn);
+    ising_spins(sr.side == 0 ? j : i, nr.pos1, nr.val1, nr.pos2, nr.val2, so);
+    wv = w[ir.edge_off + sr.edge];
+  }
+  const int cnt = ising_site_entries(sr.kind, sr.side, sn, so, wv, hv, nr.beta, v);
+  if (nr.f32) {
+    float* dst = (float*)nr.arena + sr.off;
+    for (int q = 0; q < cnt; ++q) dst[q] = (float)v[q];
+  } else {
+    double* dst = nr.arena + sr.off;
+    for (int q = 0; q < cnt; ++q) dst[q] = v[q];
+  }
+}
+
 class Executor {
  public:
   virtual ~Executor() = default;
@@ -76,6 +121,10 @@ class Executor {
   virtual int lds_limit() const { return kLdsMaxDoubles; }
   // device-to-device copy on the executor's stream (returns after completion)
   virtual void d2d(void* dst, const void* src, size_t bytes) = 0;
+  // Fill the device-built sites of ising networks (catn_ising_network_batch):
+  // for every network and every site of its instance, ising_site_entries
+  // written into the network's arena. Returns after completion.
+  virtual void build_ising(const IsingBuild& b) = 0;
   // The library's fixed device region of the no-allocation entry points: its
   // device address on this executor's device and its size in bytes (null if
   // there is none). The library owns it for the life of the process.
@@ -114,7 +163,8 @@ struct catn_net {
   int num_isolated = 0;
   bool consumed = false;
   // host staging of site tensors not yet on the device
-  struct Staged { int slot; int dl, d, dr; std::vector<double> data; };
+  // dev: allocated in the arena and filled on the device (no host data)
+  struct Staged { int slot; int dl, d, dr; std::vector<double> data; bool dev = false; };
   std::vector<Staged> staged;
   int nslots = 0;
   // device state
@@ -187,6 +237,15 @@ struct catn_net {
     hs.record = 1;
   }
 
+  int new_device_slot(int dl, int d, int dr) {
+    Staged s;
+    s.slot = nslots++;
+    s.dl = dl; s.d = d; s.dr = dr;
+    s.dev = true;
+    staged.push_back(std::move(s));
+    return staged.back().slot;
+  }
+
   int new_slot(int dl, int d, int dr, const double* data) {
     Staged s;
     s.slot = nslots++;
@@ -203,7 +262,8 @@ struct catn_net {
     catn::Executor* ex = ctx->ex.get();
     if (!allocated) {
       int64_t staged_elems = 0;
-      for (auto& s : staged) staged_elems += ((int64_t)s.data.size() + 31) / 32 * 32;
+      for (auto& s : staged)
+        staged_elems += ((s.dev ? (int64_t)s.dl * s.d * s.dr : (int64_t)s.data.size()) + 31) / 32 * 32;
       if (arena_cap <= 0) arena_cap = std::max<int64_t>(int64_t(1) << 24, 4 * staged_elems);
       if (scratch_cap <= 0) scratch_cap = int64_t(1) << 22;
       arena_cap = std::max<int64_t>(arena_cap, 2 * staged_elems + 1024);
@@ -237,8 +297,13 @@ struct catn_net {
       const int64_t base = top;
       for (auto& s : staged) {
         catn::SiteDesc d{};
-        d.dl = s.dl; d.d = s.d; d.dr = s.dr; d.live = s.data.empty() ? 0 : 1;
-        if (!s.data.empty()) {
+        d.dl = s.dl; d.d = s.d; d.dr = s.dr; d.live = (s.dev || !s.data.empty()) ? 1 : 0;
+        if (s.dev) {                // filled on the device after the upload
+          d.off = top;
+          if ((int)off0.size() <= s.slot) off0.resize(s.slot + 1, -1);
+          off0[s.slot] = top;
+          top = (top + (int64_t)s.dl * s.d * s.dr + 31) & ~int64_t(31);
+        } else if (!s.data.empty()) {
           d.off = top;
           if ((int)off0.size() <= s.slot) off0.resize(s.slot + 1, -1);
           off0[s.slot] = top;
@@ -377,9 +442,14 @@ void parallel_for(int n, F f) {
   if (err) std::rethrow_exception(err);
 }
 
-// ising_network site construction (src/ising.jl:20-227)
+// ising_network site construction (src/ising.jl:20-227). The structure (per
+// node: one site per incident edge in edge-list order, its kind and shape) is
+// built here; the values come from ising_site_entries, on the host or, with
+// device_values, by the device builder (catn_ising_network_batch): the sites
+// are then allocated in the arena and filled on the device.
 inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, const double* w,
-                        const double* h, double beta, int pos1, int val1, int pos2, int val2) {
+                        const double* h, double beta, int pos1, int val1, int pos2, int val2,
+                        bool device_values = false) {
   std::vector<int> degree(n + 1, 0);
   for (int e = 0; e < nedges; ++e) {
     const int i = edges[2 * e], j = edges[2 * e + 1];
@@ -388,48 +458,30 @@ inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, 
     degree[j]++;
   }
   for (int k = 1; k <= n; ++k) net->pl.nodes[k] = PNode{};
+  auto add_site = [&](int id, int dl, int d, int dr, int kind, int e, int side, const double* v) {
+    const int slot = device_values ? net->new_device_slot(dl, d, dr) : net->new_slot(dl, d, dr, v);
+    net->isites.push_back({slot, e, side, id, kind});
+    return slot;
+  };
   for (int e = 0; e < nedges; ++e) {
     const int i = edges[2 * e], j = edges[2 * e + 1];
-    double si[2] = {1.0, -1.0}, sj[2] = {1.0, -1.0};
-    if (i == pos1) si[val1 - 1] = 0.0;
-    else if (i == pos2) si[val2 - 1] = 0.0;
-    if (j == pos1) sj[val1 - 1] = 0.0;
-    else if (j == pos2) sj[val2 - 1] = 0.0;
-    const double we = w[e];
-    double B[2][2], Rm[2][2] = {{1.0, 0.0}, {0.0, 1.0}};
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) B[a][b] = std::exp(beta * we * (si[a] * sj[b]));
+    double si[2], sj[2];
+    ising_spins(i, pos1, val1, pos2, val2, si);
+    ising_spins(j, pos1, val1, pos2, val2, sj);
     for (int side = 0; side < 2; ++side) {
       const int id = side == 0 ? i : j;
       const int other = side == 0 ? j : i;
-      const double* s = side == 0 ? si : sj;
-      double (*mat)[2] = side == 0 ? B : Rm;
       PNode& nd = net->pl.nodes[id];
-      const double hv = h[id - 1];
-      const double f[2] = {std::exp(beta * hv * s[0]), std::exp(beta * hv * s[1])};
+      int dl, d = 2, dr, kind;
+      if (degree[id] == 1) { dl = 1; dr = 1; kind = 0; }                           // leaf
+      else if (nd.nb.empty()) { dl = 1; dr = 2; kind = 1; }                       // first
+      else if ((int)nd.nb.size() == degree[id] - 1) { dl = 2; dr = 1; kind = 2; } // last
+      else { dl = 2; dr = 2; kind = 3; }                                          // interior
       double site[8];
-      int dl, d, dr, kind;
-      if (degree[id] == 1) {            // leaf: column sums of field*Q
-        site[0] = f[0] * mat[0][0] + f[1] * mat[1][0];
-        site[1] = f[0] * mat[0][1] + f[1] * mat[1][1];
-        dl = 1; d = 2; dr = 1; kind = 0;
-      } else if (nd.nb.empty()) {       // first: (field*Q)' reshaped (1,2,2)
-        // site[1, a, b] = (fQ)[b, a]  -> linear a + 2b
-        for (int a = 0; a < 2; ++a)
-          for (int b = 0; b < 2; ++b) site[a + 2 * b] = f[b] * mat[b][a];
-        dl = 1; d = 2; dr = 2; kind = 1;
-      } else if ((int)nd.nb.size() == degree[id] - 1) {   // last: Q reshaped (2,2,1)
-        for (int a = 0; a < 2; ++a)
-          for (int b = 0; b < 2; ++b) site[a + 2 * b] = mat[a][b];
-        dl = 2; d = 2; dr = 1; kind = 2;
-      } else {                          // interior: t3[a, s, b] = delta_ab Q[a, s]
-        for (int q = 0; q < 8; ++q) site[q] = 0.0;
-        for (int a = 0; a < 2; ++a)
-          for (int sv = 0; sv < 2; ++sv) site[a + 2 * sv + 4 * a] = mat[a][sv];
-        dl = 2; d = 2; dr = 2; kind = 3;
-      }
-      const int slot = net->new_slot(dl, d, dr, site);
-      net->isites.push_back({slot, e, side, id, kind});
+      if (!device_values)
+        ising_site_entries(kind, side, side == 0 ? si : sj, side == 0 ? sj : si, w[e], h[id - 1],
+                           beta, site);
+      const int slot = add_site(id, dl, d, dr, kind, e, side, site);
       nd.slot.push_back(slot);
       nd.phys.push_back(d);
       nd.nb.push_back(other);
@@ -437,10 +489,9 @@ inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, 
   }
   for (int k = 1; k <= n; ++k) {
     if (degree[k] == 0) {
-      const double hv = h[k - 1];
-      const double v = std::exp(beta * hv) + std::exp(-beta * hv);
-      const int slot = net->new_slot(1, 1, 1, &v);
-      net->isites.push_back({slot, -1, 0, k, 4});
+      double v = 0.0;
+      if (!device_values) ising_site_entries(4, 0, nullptr, nullptr, 0.0, h[k - 1], beta, &v);
+      const int slot = add_site(k, 1, 1, 1, 4, -1, 0, &v);
       net->pl.nodes[k].slot.push_back(slot);
       net->pl.nodes[k].phys.push_back(1);
     }
@@ -789,6 +840,80 @@ int catn_ising_network(catn_ctx* ctx, int32_t n, int32_t nedges, const int32_t* 
   apply_params(net.get(), p);
   catn::build_ising(net.get(), n, nedges, edges, w, h, beta, pos1, val1, pos2, val2);
   *out = net.release();
+  return 0;
+  CATN_CATCH
+}
+
+int catn_ising_network_batch(catn_ctx* ctx, int32_t ninst, const int32_t* inst_n,
+                             const int32_t* inst_nedges, const int32_t* edges, const double* w,
+                             const double* h, int32_t nnets, const int32_t* net_inst,
+                             const double* net_beta, const int32_t* net_pins,
+                             const catn_params* p, catn_net** out) {
+  CATN_TRY
+  if (ninst <= 0 || nnets <= 0) throw catn::Error(-2, "ising_network_batch: empty batch");
+  std::vector<int64_t> eoff(ninst + 1, 0), foff(ninst + 1, 0);
+  for (int q = 0; q < ninst; ++q) {
+    if (inst_n[q] <= 0 || inst_nedges[q] < 0)
+      throw catn::Error(-2, "ising_network_batch: instance sizes must be positive");
+    eoff[q + 1] = eoff[q] + inst_nedges[q];
+    foff[q + 1] = foff[q] + inst_n[q];
+  }
+  std::vector<std::unique_ptr<catn_net>> nets(nnets);
+  for (int k = 0; k < nnets; ++k) {
+    const int q = net_inst[k];
+    if (q < 0 || q >= ninst) throw catn::Error(-2, "ising_network_batch: instance index out of range");
+    nets[k].reset(new catn_net);
+    nets[k]->attach(ctx);
+    apply_params(nets[k].get(), p);
+    const int32_t* pin = net_pins ? net_pins + 4 * k : nullptr;
+    catn::build_ising(nets[k].get(), inst_n[q], inst_nedges[q], edges + 2 * eoff[q], w + eoff[q],
+                      h + foff[q], net_beta[k], pin ? pin[0] : 0, pin ? pin[1] : 0,
+                      pin ? pin[2] : 0, pin ? pin[3] : 0, true);
+  }
+  for (auto& n : nets) n->ensure_device();
+  // site records per instance from its first network; a network whose arena
+  // layout differs (it cannot for equal structure and parameters) gets its own
+  catn::IsingBuild b;
+  b.edges = edges; b.w = w; b.h = h;
+  b.nedges = eoff[ninst]; b.nfields = foff[ninst];
+  std::vector<int> rec_of(ninst, -1);   // instance -> record index
+  auto add_inst = [&](const catn_net* n, int q) {
+    catn::IsingInstRec ir;
+    ir.site_begin = (int32_t)b.sites.size();
+    for (const auto& is : n->isites)
+      b.sites.push_back({n->off0[is.slot], is.kind, is.edge, is.side, is.node});
+    ir.site_end = (int32_t)b.sites.size();
+    ir.edge_off = eoff[q];
+    ir.field_off = foff[q];
+    b.max_sites = std::max(b.max_sites, ir.site_end - ir.site_begin);
+    b.insts.push_back(ir);
+    return (int)b.insts.size() - 1;
+  };
+  for (int k = 0; k < nnets; ++k) {
+    const catn_net* n = nets[k].get();
+    const int q = net_inst[k];
+    int r = rec_of[q];
+    if (r < 0) {
+      r = rec_of[q] = add_inst(n, q);
+    } else {
+      const catn::IsingInstRec& ir = b.insts[r];
+      bool same = (int)n->isites.size() == ir.site_end - ir.site_begin;
+      for (size_t t = 0; same && t < n->isites.size(); ++t)
+        same = n->off0[n->isites[t].slot] == b.sites[ir.site_begin + t].off;
+      if (!same) r = add_inst(n, q);
+    }
+    const int32_t* pin = net_pins ? net_pins + 4 * k : nullptr;
+    catn::IsingNetRec nr;
+    nr.arena = n->hs.arena;
+    nr.beta = net_beta[k];
+    nr.inst = r;
+    nr.f32 = n->f32 ? 1 : 0;
+    nr.pos1 = pin ? pin[0] : 0; nr.val1 = pin ? pin[1] : 0;
+    nr.pos2 = pin ? pin[2] : 0; nr.val2 = pin ? pin[3] : 0;
+    b.nets.push_back(nr);
+  }
+  ctx->ex->build_ising(b);
+  for (int k = 0; k < nnets; ++k) out[k] = nets[k].release();
   return 0;
   CATN_CATCH
 }

@@ -7,6 +7,8 @@
 // because virtual bond dimensions are produced by the device SVDs.
 #include <hip/hip_runtime.h>
 
+#include <string.h>
+
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -67,6 +69,20 @@ __global__ __launch_bounds__(kThreads, 4) void exec_backward(InstState* states, 
   InstState* st = states + blockIdx.x;
   g.sub = st->sub_ticks;
   run_backward(g, st);
+}
+
+// Device builder of ising_network sites: block (x, y) handles sites
+// x * blockDim.x + threadIdx.x of network y's instance.
+__global__ __launch_bounds__(256) void build_ising_sites(const IsingNetRec* nets,
+                                                         const IsingInstRec* insts,
+                                                         const IsingSiteRec* sites,
+                                                         const int32_t* edges, const double* w,
+                                                         const double* h) {
+  const IsingNetRec nr = nets[blockIdx.y];
+  const IsingInstRec ir = insts[nr.inst];
+  const int s = ir.site_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (s >= ir.site_end) return;
+  ising_build_site(nr, ir, sites[s], edges, w, h);
 }
 
 static void hipcheck(hipError_t e, const char* what) {
@@ -150,6 +166,42 @@ class HipExecutor : public Executor {
   void d2d(void* dst, const void* src, size_t bytes) override {
     hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_), "d2d");
     hipcheck(hipStreamSynchronize(stream_), "d2d sync");
+  }
+  void build_ising(const IsingBuild& b) override {
+    if (b.nets.empty() || b.max_sites == 0) return;
+    if (b.nets.size() > 65535) throw Error(-2, "ising_network_batch: more than 65535 networks");
+    // one upload of the packed records and arrays, one launch
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_n = 0, o_i = al(o_n + sizeof(IsingNetRec) * b.nets.size());
+    const size_t o_s = al(o_i + sizeof(IsingInstRec) * b.insts.size());
+    const size_t o_e = al(o_s + sizeof(IsingSiteRec) * b.sites.size());
+    const size_t o_w = al(o_e + sizeof(int32_t) * 2 * (size_t)b.nedges);
+    const size_t o_h = al(o_w + sizeof(double) * (size_t)b.nedges);
+    const size_t total = o_h + sizeof(double) * (size_t)b.nfields;
+    std::vector<char> host(total);
+    memcpy(host.data() + o_n, b.nets.data(), sizeof(IsingNetRec) * b.nets.size());
+    memcpy(host.data() + o_i, b.insts.data(), sizeof(IsingInstRec) * b.insts.size());
+    memcpy(host.data() + o_s, b.sites.data(), sizeof(IsingSiteRec) * b.sites.size());
+    if (b.nedges) {
+      memcpy(host.data() + o_e, b.edges, sizeof(int32_t) * 2 * (size_t)b.nedges);
+      memcpy(host.data() + o_w, b.w, sizeof(double) * (size_t)b.nedges);
+    }
+    memcpy(host.data() + o_h, b.h, sizeof(double) * (size_t)b.nfields);
+    char* d = (char*)dmalloc(total);
+    try {
+      h2d(d, host.data(), total);
+      const dim3 grid((unsigned)((b.max_sites + 255) / 256), (unsigned)b.nets.size());
+      hipcheck(hipGetLastError(), "before launch build_ising_sites");
+      hipLaunchKernelGGL(build_ising_sites, grid, dim3(256), 0, stream_, (const IsingNetRec*)(d + o_n),
+                         (const IsingInstRec*)(d + o_i), (const IsingSiteRec*)(d + o_s),
+                         (const int32_t*)(d + o_e), (const double*)(d + o_w),
+                         (const double*)(d + o_h));
+      finish_launch("build_ising_sites");
+    } catch (...) {
+      dfree(d);
+      throw;
+    }
+    dfree(d);
   }
   void* fixed_region(size_t* bytes) override {
     if (bytes) *bytes = sizeof(double) * kFixedDoubles;

@@ -22,9 +22,11 @@
 #if defined(__HIPCC__)
 #define CATN_D __device__ __forceinline__
 #define CATN_DN __device__ __noinline__
+#define CATN_HD __host__ __device__ inline
 #else
 #define CATN_D inline
 #define CATN_DN
+#define CATN_HD inline
 #endif
 
 // CATN_DEV selects the device code paths: FP64/BF16 matrix-core products,
@@ -2140,6 +2142,56 @@ CATN_DN bool wg_factor_fullrank(const WG& g, Scratch& sc, const double* A, int m
   F.Rt = Rt;
   g.sync();
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// ising_network site values (src/ising.jl:68-195), shared by the host builder
+// and the device builder kernel. One (edge, side) site of a node: side 0 (the
+// edge's first spin i) carries B = exp(beta w s_i s_j'), side 1 the identity;
+// the field vector f = exp(beta h s) multiplies the leaf and first sites.
+// s_node / s_other: the two spins' values (1, -1, a pinned value 0).
+// kind: 0 leaf (1,2,1), 1 first (1,2,2), 2 last (2,2,1), 3 interior (2,2,2),
+// 4 isolated spin (1,1,1) = 2 cosh(beta h). Returns the entry count.
+// ---------------------------------------------------------------------------
+CATN_HD int ising_site_entries(int kind, int side, const double* s_node, const double* s_other,
+                              double w, double h, double beta, double* site) {
+  if (kind == 4) {
+    site[0] = exp(beta * h) + exp(-beta * h);
+    return 1;
+  }
+  double mat[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      mat[a][b] = side == 0 ? exp(beta * w * (s_node[a] * s_other[b])) : (a == b ? 1.0 : 0.0);
+  const double f[2] = {exp(beta * h * s_node[0]), exp(beta * h * s_node[1])};
+  if (kind == 0) {                  // column sums of field * Q
+    site[0] = f[0] * mat[0][0] + f[1] * mat[1][0];
+    site[1] = f[0] * mat[0][1] + f[1] * mat[1][1];
+    return 2;
+  }
+  if (kind == 1) {                  // (field * Q)' as (1,2,2): site[a + 2b] = (fQ)[b, a]
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) site[a + 2 * b] = f[b] * mat[b][a];
+    return 4;
+  }
+  if (kind == 2) {                  // Q as (2,2,1)
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) site[a + 2 * b] = mat[a][b];
+    return 4;
+  }
+  for (int q = 0; q < 8; ++q) site[q] = 0.0;   // interior: t3[a, s, b] = delta_ab Q[a, s]
+  for (int a = 0; a < 2; ++a)
+    for (int sv = 0; sv < 2; ++sv) site[a + 2 * sv + 4 * a] = mat[a][sv];
+  return 8;
+}
+
+// spins of node `id` under pinning (src/ising.jl:273: the pinned value is a
+// ghost 0, not an exclusion)
+CATN_HD void ising_spins(int id, int pos1, int val1, int pos2, int val2, double* s) {
+  s[0] = 1.0;
+  s[1] = -1.0;
+  if (id == pos1) s[val1 - 1] = 0.0;
+  else if (id == pos2) s[val2 - 1] = 0.0;
 }
 
 // ---------------------------------------------------------------------------

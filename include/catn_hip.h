@@ -84,6 +84,23 @@ int catn_ising_network(catn_ctx* ctx, int32_t n, int32_t nedges, const int32_t* 
                        int32_t val1, int32_t pos2, int32_t val2, const catn_params* p,
                        catn_net** out);
 
+/* ising_network for a batch, with the site values built on the device
+ * (src/ising.jl:68-195, site by site). Packed instance format: instance q
+ * has inst_n[q] spins and inst_nedges[q] edges; `edges` (2 ints per edge),
+ * `w` and `h` hold the instances' edge lists, couplings and fields
+ * concatenated in instance order. Network k is instance net_inst[k] at
+ * inverse temperature net_beta[k] with pinning net_pins[4k..4k+3] = (pos1,
+ * val1, pos2, val2) (NULL: none). The host builds only the integer structure
+ * (sites per node in edge-list order); one kernel computes every site of the
+ * batch into the networks' arenas. out receives nnets networks, each as
+ * catn_ising_network would build it. No reference counterpart (the reference
+ * builds one network at a time on the host). */
+int catn_ising_network_batch(catn_ctx* ctx, int32_t ninst, const int32_t* inst_n,
+                             const int32_t* inst_nedges, const int32_t* edges, const double* w,
+                             const double* h, int32_t nnets, const int32_t* net_inst,
+                             const double* net_beta, const int32_t* net_pins,
+                             const catn_params* p, catn_net** out);
+
 /* TensorNetwork(tensors, ixs; ...) (src/tensor_network.jl:41-140): ntensors
  * tensors, tensor t has ndims[t] legs with sizes dims[...] (concatenated)
  * and data (concatenated, column-major); labels[...] one int64 label per leg.

@@ -33,6 +33,13 @@ class CpuExecutor : public Executor {
   void h2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
   void d2h(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
   void d2d(void* dst, const void* src, size_t bytes) override { memcpy(dst, src, bytes); }
+  void build_ising(const IsingBuild& b) override {
+    for (const IsingNetRec& nr : b.nets) {
+      const IsingInstRec& ir = b.insts[nr.inst];
+      for (int s = ir.site_begin; s < ir.site_end; ++s)
+        ising_build_site(nr, ir, b.sites[s], b.edges, b.w, b.h);
+    }
+  }
   void* fixed_region(size_t* bytes) override {
     static double region[size_t(8) << 20];   // the device library's 64 MiB fixed region
     if (bytes) *bytes = sizeof(region);
