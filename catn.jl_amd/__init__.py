@@ -151,6 +151,21 @@ def _f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
 
+def _elems(a, cplx: bool):
+    """Column-major data of an array as the C ABI takes it: doubles, or for a
+    complex network interleaved (re, im) doubles (Julia's ComplexF64 layout)."""
+    a = np.asarray(a)
+    if cplx:
+        return np.ascontiguousarray(a.astype(np.complex128).ravel(order="F")).view(np.float64)
+    if np.iscomplexobj(a):
+        raise CatnError("complex tensor given to a real network (eltype='real')")
+    return np.ascontiguousarray(a.astype(np.float64).ravel(order="F"))
+
+
+def _from_elems(buf, cplx: bool):
+    return buf.view(np.complex128) if cplx else buf
+
+
 class Context:
     """One device context (one HIP stream). `lib` selects the shared library
     (default: the in-tree libcatn_hip.so)."""
@@ -256,9 +271,10 @@ class TensorNetwork:
     """Handle of a device-resident network (TensorNetwork{T,AT} of the
     reference with AT = device storage)."""
 
-    def __init__(self, ctx: Context, handle):
+    def __init__(self, ctx: Context, handle, cplx: bool = False):
         self.ctx = ctx
         self.h = handle
+        self.cplx = bool(cplx)   # complex element type (ComplexF64)
         self.lnZ = 0.0
         self.psi = 1.0
 
@@ -274,12 +290,13 @@ class TensorNetwork:
             pass
 
     def info(self):
-        sign = C.c_double()
+        sign = np.zeros(2)
         md, ni, n = C.c_int32(), C.c_int32(), C.c_int32()
         beta = C.c_double()
-        self.ctx._check(self.ctx.lib.catn_net_info(self.h, C.byref(sign), C.byref(md),
+        self.ctx._check(self.ctx.lib.catn_net_info(self.h, _dptr(sign), C.byref(md),
                                                    C.byref(ni), C.byref(n), C.byref(beta)))
-        return dict(sign=sign.value, maxdim_intermediate=md.value, num_isolated=ni.value,
+        sg = complex(sign[0], sign[1]) if self.cplx else float(sign[0])
+        return dict(sign=sg, maxdim_intermediate=md.value, num_isolated=ni.value,
                     n=n.value, beta=beta.value)
 
     @property
@@ -459,15 +476,19 @@ def tensor_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
     """TensorNetwork(tensors, ixs; ...) (src/tensor_network.jl:41-140). Tensors
     are taken with Julia (column-major) index order: t[i1, i2, ...]."""
     ctx = ctx or default_context()
+    # complex tensors make a complex network (TensorNetwork{ComplexF64}) unless
+    # eltype is given
+    cplx = kw.pop("eltype", None)
+    cplx = any(np.iscomplexobj(t) for t in tensors) if cplx is None else cplx == "complex"
     lab_ids = {}
     labels, ndims, dims, data = [], [], [], []
     for t, ix in zip(tensors, ixs):
-        t = np.asarray(t, dtype=np.float64)
+        t = np.asarray(t)
         if t.ndim != len(ix):
             raise CatnError("tensor rank does not match its label list")
         ndims.append(t.ndim)
         dims.extend(t.shape)
-        data.append(t.ravel(order="F"))
+        data.append(_elems(t, cplx))
         for l in ix:
             if l not in lab_ids:
                 lab_ids[l] = len(lab_ids)
@@ -476,12 +497,12 @@ def tensor_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
     dm = np.ascontiguousarray(np.asarray(dims, dtype=np.int64))
     lb = np.ascontiguousarray(np.asarray(labels, dtype=np.int64))
     dt = _f64(np.concatenate(data) if data else np.zeros(0))
-    p = _params(ctx, **kw)
+    p = _params(ctx, eltype="complex" if cplx else "real", **kw)
     out = _P()
     ctx._check(ctx.lib.catn_tensor_network(ctx.h, len(tensors), nd.ctypes.data_as(_I32P),
                                            dm.ctypes.data_as(_I64P), _dptr(dt),
                                            lb.ctypes.data_as(_I64P), C.byref(p), C.byref(out)))
-    tn = TensorNetwork(ctx, out)
+    tn = TensorNetwork(ctx, out, cplx)
     # open labels in order of first appearance: the device's sentinels -1, -2, ...
     count = {}
     for ix in ixs:
@@ -497,26 +518,25 @@ TensorNetwork_ = tensor_network
 def braket_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
                    ctx: Optional[Context] = None, chi: int = 64, **kw) -> TensorNetwork:
     """braket_network(tensors, ixs; chi, cutoff, norm_method, svdopt, swapopt)
-    (src/braket.jl:369-496) for real tensors: the double-layer network of
-    <psi|psi>. contraction() of the result returns (ln <psi|psi>, error, psi).
+    (src/braket.jl:369-496): the double-layer network of <psi|psi>, the bra
+    chains built from conj(T); complex tensors make a complex network. contraction() of the result returns (ln <psi|psi>, error, psi).
     Tensors are taken with Julia (column-major) index order. loops=True accepts
     graphs with cycles, which the reference rejects: when an eat closes a
     cycle, the doubled bonds to the shared neighbour are merged (ket and bra
     legs separately, the older bond as the fast index), and with Dmax > 0 a
     merged bond wider than Dmax is cut as cut_bondim_opt! does."""
     ctx = ctx or default_context()
+    cplx = kw.pop("eltype", None)
+    cplx = any(np.iscomplexobj(t) for t in tensors) if cplx is None else cplx == "complex"
     lab_ids = {}
     labels, ndims, dims, data = [], [], [], []
     for t, ix in zip(tensors, ixs):
         t = np.asarray(t)
-        if np.iscomplexobj(t):
-            raise CatnError("braket_network: complex tensors are not supported on the device")
-        t = t.astype(np.float64)
         if t.ndim != len(ix):
             raise CatnError("tensor rank does not match its label list")
         ndims.append(t.ndim)
         dims.extend(t.shape)
-        data.append(t.ravel(order="F"))
+        data.append(_elems(t, cplx))
         for l in ix:
             if l not in lab_ids:
                 lab_ids[l] = len(lab_ids)
@@ -526,21 +546,23 @@ def braket_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
     lb = np.ascontiguousarray(np.asarray(labels, dtype=np.int64))
     dt = _f64(np.concatenate(data) if data else np.zeros(0))
     braket_dmax = kw.pop("Dmax", -1)
-    p = _params(ctx, chi=chi, **kw)
+    p = _params(ctx, chi=chi, eltype="complex" if cplx else "real", **kw)
     p.braket_dmax = int(braket_dmax)
     out = _P()
     ctx._check(ctx.lib.catn_braket_network(ctx.h, len(tensors), nd.ctypes.data_as(_I32P),
                                            dm.ctypes.data_as(_I64P), _dptr(dt),
                                            lb.ctypes.data_as(_I64P), C.byref(p), C.byref(out)))
-    return TensorNetwork(ctx, out)
+    return TensorNetwork(ctx, out, cplx)
 
 
 def contraction(tn: TensorNetwork):
-    """contraction!(tn) -> (lnZ, error, psi)."""
-    lz, err, psi = C.c_double(), C.c_double(), C.c_double()
-    tn.ctx._check(tn.ctx.lib.catn_contraction(tn.h, C.byref(lz), C.byref(err), C.byref(psi)))
-    tn.lnZ, tn.psi = lz.value, psi.value
-    return lz.value, err.value, psi.value
+    """contraction!(tn) -> (lnZ, error, psi); psi is complex for a complex network."""
+    lz, err = C.c_double(), C.c_double()
+    psi = np.zeros(2)
+    tn.ctx._check(tn.ctx.lib.catn_contraction(tn.h, C.byref(lz), C.byref(err), _dptr(psi)))
+    ps = complex(psi[0], psi[1]) if tn.cplx else float(psi[0])
+    tn.lnZ, tn.psi = lz.value, ps
+    return lz.value, err.value, ps
 
 
 def result_tensor(tn: TensorNetwork):
@@ -559,15 +581,18 @@ def result_tensor(tn: TensorNetwork):
     tn.ctx._check(tn.ctx.lib.catn_result_tensor(tn.h, None, 0, C.byref(rank),
                                                 dims.ctypes.data_as(_I32P),
                                                 labs.ctypes.data_as(_I32P), kmax))
+    one = np.array(1.0 + 0j) if tn.cplx else np.array(1.0)
     if rank.value == 0:
-        return np.array(1.0), []
-    out = np.zeros(int(np.prod(dims[: rank.value].astype(np.int64))))
+        return one, []
+    ed = 2 if tn.cplx else 1
+    out = np.zeros(ed * int(np.prod(dims[: rank.value].astype(np.int64))))
     tn.ctx._check(tn.ctx.lib.catn_result_tensor(tn.h, _dptr(out), out.size, C.byref(rank),
                                                 dims.ctypes.data_as(_I32P),
                                                 labs.ctypes.data_as(_I32P), kmax))
+    out = _from_elems(out, tn.cplx)
     r = rank.value
     if r == 0:
-        return np.array(1.0), []
+        return one, []
     names = getattr(tn, "open_labels", None)
     labels = [names[-int(v) - 1] if names is not None and v < 0 else int(v) for v in labs[:r]]
     return out.reshape(tuple(int(x) for x in dims[:r]), order="F"), labels
@@ -578,11 +603,13 @@ def contraction_batch(tns: Sequence[TensorNetwork]):
     if not tns:
         return np.zeros(0), np.zeros(0), np.zeros(0)
     ctx = tns[0].ctx
+    cplx = tns[0].cplx
     arr = (_P * len(tns))(*[t.h for t in tns])
-    lz, err, psi = np.zeros(len(tns)), np.zeros(len(tns)), np.zeros(len(tns))
+    lz, err, psi = np.zeros(len(tns)), np.zeros(len(tns)), np.zeros((2 if cplx else 1) * len(tns))
     ctx._check(ctx.lib.catn_contraction_batch(ctx.h, len(tns), arr, _dptr(lz), _dptr(err), _dptr(psi)))
+    psi = _from_elems(psi, cplx)
     for k, t in enumerate(tns):
-        t.lnZ, t.psi = float(lz[k]), float(psi[k])
+        t.lnZ, t.psi = float(lz[k]), psi[k].item()
     return lz, err, psi
 
 
@@ -790,21 +817,31 @@ class MPSNet:
     """Hand-built MPSNode objects on the device, for the per-primitive API of
     src/mps_node.jl. Node ids and positions are 1-based as in the reference."""
 
+    cplx = False   # eltype="complex": MPSNode{ComplexF64}
+
     def __init__(self, ctx: Optional[Context] = None, **kw):
         self.ctx = ctx or default_context()
         p = _params(self.ctx, **kw)
+        self.cplx = p.eltype == 1
         out = _P()
         self.ctx._check(self.ctx.lib.catn_net_create(self.ctx.h, C.byref(p), C.byref(out)))
-        self.net = TensorNetwork(self.ctx, out)
+        self.net = TensorNetwork(self.ctx, out, self.cplx)
         self.h = out
+
+    @classmethod
+    def view(cls, tn: TensorNetwork) -> "MPSNet":
+        """The per-node primitives and readers on an existing network."""
+        v = cls.__new__(cls)
+        v.ctx, v.h, v.net, v.cplx = tn.ctx, tn.h, tn, tn.cplx
+        return v
 
     def add_node(self, id, sites: Sequence[np.ndarray], neighbor, cano=None):
         dims, data = [], []
         for s in sites:
-            s = np.asarray(s, dtype=np.float64)
+            s = np.asarray(s)
             assert s.ndim == 3
             dims.extend(s.shape)
-            data.append(s.ravel(order="F"))
+            data.append(_elems(s, self.cplx))
         d = _i32(dims)
         dt = _f64(np.concatenate(data) if data else np.zeros(0))
         nb = _i32(neighbor)
@@ -814,9 +851,9 @@ class MPSNet:
 
     def add_tensor(self, id, tensor, neighbor):
         """MPSNode(tensor, neighbor; chi, cutoff) via raw2mps on the device."""
-        t = np.asarray(tensor, dtype=np.float64)
+        t = np.asarray(tensor)
         dm = np.ascontiguousarray(np.asarray(t.shape, dtype=np.int64))
-        dt = _f64(t.ravel(order="F"))
+        dt = _elems(t, self.cplx)
         nb = _i32(neighbor)
         self.ctx._check(self.ctx.lib.catn_net_add_tensor_node(
             self.h, int(id), t.ndim, dm.ctypes.data_as(_I64P), _dptr(dt), nb.ctypes.data_as(_I32P)))
@@ -857,10 +894,11 @@ class MPSNet:
         self.ctx._check(self.ctx.lib.catn_node_reverse(self.h, int(id)))
 
     def eat(self, i, j, idx, idxi):
-        ln, err, ph = C.c_double(), C.c_double(), C.c_double()
+        ln, err = C.c_double(), C.c_double()
+        ph = np.zeros(2)
         self.ctx._check(self.ctx.lib.catn_node_eat(self.h, int(i), int(j), int(idx), int(idxi),
-                                                   C.byref(ln), C.byref(err), C.byref(ph)))
-        return ln.value, err.value, ph.value
+                                                   C.byref(ln), C.byref(err), _dptr(ph)))
+        return ln.value, err.value, complex(ph[0], ph[1]) if self.cplx else float(ph[0])
 
     def node(self, id):
         """(order, cano, neighbor list, site dims list)"""
@@ -883,7 +921,7 @@ class MPSNet:
         o, c, _, dims = self.node(id)
         out = []
         for k, (dl, d, dr) in enumerate(dims):
-            buf = np.zeros(int(dl) * int(d) * int(dr))
+            buf = np.zeros((2 if self.cplx else 1) * int(dl) * int(d) * int(dr))
             self.ctx._check(self.ctx.lib.catn_node_site(self.h, int(id), k + 1, _dptr(buf), buf.size))
-            out.append(buf.reshape((int(dl), int(d), int(dr)), order="F"))
+            out.append(_from_elems(buf, self.cplx).reshape((int(dl), int(d), int(dr)), order="F"))
         return out

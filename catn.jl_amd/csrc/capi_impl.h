@@ -199,7 +199,10 @@ struct catn_net {
   void finish_plan() { done_pl = std::make_shared<const catn::Planner>(std::move(pl)); }
   bool f32 = false;        // FP32 site storage (catn_params.precision = 1 or 2)
   int bf16_terms = 2;      // BF16 terms per FP32 operand of the theta GEMM
-  size_t elem_bytes() const { return f32 ? sizeof(float) : sizeof(double); }
+  bool cplx = false;       // complex site tensors (catn_params.eltype = 1)
+  size_t elem_bytes() const { return cplx ? 2 * sizeof(double) : f32 ? sizeof(float) : sizeof(double); }
+  // doubles per element of host data (interleaved re, im when complex)
+  int ed() const { return cplx ? 2 : 1; }
 
   ~catn_net() {
     release();
@@ -250,7 +253,7 @@ struct catn_net {
     Staged s;
     s.slot = nslots++;
     s.dl = dl; s.d = d; s.dr = dr;
-    const size_t cnt = (size_t)dl * d * dr;
+    const size_t cnt = (size_t)dl * d * dr * ed();
     if (data) s.data.assign(data, data + cnt);
     else s.data.clear();   // placeholder slot, descriptor written by a device op
     staged.push_back(std::move(s));
@@ -263,7 +266,7 @@ struct catn_net {
     if (!allocated) {
       int64_t staged_elems = 0;
       for (auto& s : staged)
-        staged_elems += ((s.dev ? (int64_t)s.dl * s.d * s.dr : (int64_t)s.data.size()) + 31) / 32 * 32;
+        staged_elems += ((s.dev ? (int64_t)s.dl * s.d * s.dr : (int64_t)s.data.size() / ed()) + 31) / 32 * 32;
       if (arena_cap <= 0) arena_cap = std::max<int64_t>(int64_t(1) << 24, 4 * staged_elems);
       if (scratch_cap <= 0) scratch_cap = int64_t(1) << 22;
       arena_cap = std::max<int64_t>(arena_cap, 2 * staged_elems + 1024);
@@ -273,9 +276,10 @@ struct catn_net {
       hs.psi = 1.0; hs.sign = 1.0;
       hs.seed = pl.p.seed;
       hs.f32 = f32 ? (bf16_terms == 3 ? 2 : 1) : 0;
+      hs.cplx = cplx ? 1 : 0;
       hs.arena_cap = arena_cap;
       hs.scratch_cap = scratch_cap;
-      hs.arena = (double*)ex->dmalloc(elem_bytes() * arena_cap);   // floats when f32
+      hs.arena = (double*)ex->dmalloc(elem_bytes() * arena_cap);   // floats when f32, pairs when complex
       hs.scratch = (double*)ex->dmalloc(sizeof(double) * scratch_cap);
       hs.sites = (catn::SiteDesc*)ex->dmalloc(sizeof(catn::SiteDesc) * slot_cap);
       hs.results = (int32_t*)ex->dmalloc(sizeof(int32_t) * res_cap);
@@ -307,9 +311,9 @@ struct catn_net {
           d.off = top;
           if ((int)off0.size() <= s.slot) off0.resize(s.slot + 1, -1);
           off0[s.slot] = top;
-          buf.resize((size_t)(top - base), 0.0);
+          buf.resize((size_t)(top - base) * ed(), 0.0);
           buf.insert(buf.end(), s.data.begin(), s.data.end());
-          top = (top + (int64_t)s.data.size() + 31) & ~int64_t(31);
+          top = (top + (int64_t)s.data.size() / ed() + 31) & ~int64_t(31);
         }
         descs.push_back({s.slot, d});
       }
@@ -319,7 +323,7 @@ struct catn_net {
           std::vector<float> fb(buf.begin(), buf.end());
           ex->h2d((float*)hs.arena + base, fb.data(), sizeof(float) * fb.size());
         } else {
-          ex->h2d(hs.arena + base, buf.data(), sizeof(double) * buf.size());
+          ex->h2d(hs.arena + base * ed(), buf.data(), sizeof(double) * buf.size());
         }
       }
       // staged slots are always the contiguous id range [first, nslots)
@@ -353,6 +357,7 @@ struct catn_net {
 
   void reset_acc() {
     hs.lnZ = 0.0; hs.err = 0.0; hs.psi = 1.0; hs.sign = 1.0; hs.lntail = 0.0;
+    hs.psi_i = 0.0; hs.sign_i = 0.0;
   }
 
   int read_result(int idx) {
@@ -450,6 +455,7 @@ void parallel_for(int n, F f) {
 inline void build_ising(catn_net* net, int n, int nedges, const int32_t* edges, const double* w,
                         const double* h, double beta, int pos1, int val1, int pos2, int val2,
                         bool device_values = false) {
+  if (net->cplx) throw Error(-2, "ising_network: the Ising site tensors are real (eltype must be 0)");
   std::vector<int> degree(n + 1, 0);
   for (int e = 0; e < nedges; ++e) {
     const int i = edges[2 * e], j = edges[2 * e + 1];
@@ -743,6 +749,9 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
 inline void gradient_nets(Executor* ex, const std::vector<catn_net*>& nets, int tape_cap) {
   for (catn_net* n : nets) {
     if (n->consumed) throw Error(-5, "contraction!: network already contracted");
+    if (n->cplx)
+      throw Error(-2, "gradient: the reverse sweep is built for real networks; complex networks "
+                      "run the plain forward (catn_contraction)");
     n->enable_record(tape_cap);
   }
   contract_nets(ex, nets);
@@ -818,12 +827,13 @@ static void apply_params(catn_net* net, const catn_params* p) {
   q.rsvd_both = p->rsvd_both;
   q.rsvd_any = p->rsvd_any;
   q.loops = p->loops != 0;
-  if (p->eltype == 1)
-    throw catn::Error(-2, "complex element types (ComplexF64 / ComplexF32) are not built in this "
-                          "library: the engine computes real networks");
-  if (p->eltype != 0) throw catn::Error(-2, "eltype must be 0 (real) or 1 (complex)");
+  if (p->eltype != 0 && p->eltype != 1) throw catn::Error(-2, "eltype must be 0 (real) or 1 (complex)");
   if (p->precision < 0 || p->precision > 2)
     throw catn::Error(-2, "precision must be 0 (FP64), 1 (FP32, split-BF16 3-pass) or 2 (FP32, 6-pass)");
+  if (p->eltype == 1 && p->precision != 0)
+    throw catn::Error(-2, "complex networks are computed in ComplexF64 (precision 0); ComplexF32 "
+                          "storage is not built");
+  net->cplx = p->eltype == 1;
   net->f32 = p->precision != 0;
   net->bf16_terms = p->precision == 2 ? 3 : 2;
   net->arena_cap = p->arena_elems;
@@ -968,7 +978,7 @@ int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
     for (int v : nbv[t]) all_neg = all_neg && v < 0;
     if (all_neg) ++iso;
     di += ndims[t];
-    dd += (size_t)total;
+    dd += (size_t)total * net->ed();
   }
   net->num_isolated = iso;
   net->n = ntensors;
@@ -1055,15 +1065,18 @@ int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
     int64_t total = 1;
     for (int k = 0; k < nd; ++k) total *= tdims[k];
     // column-major permutation: out axes = perm, out[i_out] = in[i_in]
-    auto permuted = [&](const std::vector<int>& perm) {
-      std::vector<double> outv((size_t)total);
+    // (complex tensors: interleaved pairs; conj negates the imaginary parts)
+    const int ed = net->ed();
+    auto permuted = [&](const std::vector<int>& perm, bool conj) {
+      std::vector<double> outv((size_t)total * ed);
       std::vector<int64_t> stride(nd, 1);
       for (int k = 1; k < nd; ++k) stride[k] = stride[k - 1] * tdims[k - 1];
       std::vector<int64_t> idx(nd, 0);
       for (int64_t q = 0; q < total; ++q) {
         int64_t src = 0;
         for (int k = 0; k < nd; ++k) src += idx[k] * stride[perm[k]];
-        outv[(size_t)q] = data[dd + (size_t)src];
+        outv[(size_t)q * ed] = data[dd + (size_t)src * ed];
+        if (ed == 2) outv[(size_t)q * 2 + 1] = (conj ? -1.0 : 1.0) * data[dd + (size_t)src * 2 + 1];
         for (int k = 0; k < nd; ++k) {   // next output multi-index
           if (++idx[k] < tdims[perm[k]]) break;
           idx[k] = 0;
@@ -1075,8 +1088,8 @@ int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
     pk.push_back(phys_pos);
     pb.push_back(phys_pos);
     pb.insert(pb.end(), virt.begin(), virt.end());
-    const std::vector<double> tk = permuted(pk);   // (D1, ..., D_deg, d)
-    const std::vector<double> tb = permuted(pb);   // (d, D1, ..., D_deg): conj of a real tensor
+    const std::vector<double> tk = permuted(pk, false);   // (D1, ..., D_deg, d)
+    const std::vector<double> tb = permuted(pb, true);    // conj, (d, D1, ..., D_deg)
     const int d = (int)tdims[phys_pos];
     catn::PNode node;
     // ket chain (:116-130): splits over D1..D_{deg-1}; the remainder becomes the
@@ -1107,7 +1120,7 @@ int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
     pl.nodes[t + 1] = node;
     pl.left_canonical(pl.nodes[t + 1]);   // :183
     di += nd;
-    dd += (size_t)total;
+    dd += (size_t)total * ed;
     li += nd;
   }
   pl.braket = true;
@@ -1141,7 +1154,7 @@ int catn_net_add_node(catn_net* net, int32_t id, int32_t nsites, const int32_t* 
     node.slot.push_back(net->new_slot(dl, d, dr, data + off));
     node.phys.push_back(d);
     node.nb.push_back(neighbor[k]);
-    off += (size_t)dl * d * dr;
+    off += (size_t)dl * d * dr * net->ed();
   }
   node.cano = cano;
   net->pl.nodes[id] = node;
@@ -1176,7 +1189,10 @@ int catn_contraction(catn_net* net, double* lnZ, double* error, double* psi) {
   catn::contract_nets(catn::exec_of(net), {net});
   if (lnZ) *lnZ = net->hs.lnZ + net->hs.lntail;
   if (error) *error = net->hs.err;
-  if (psi) *psi = net->hs.psi;
+  if (psi) {
+    psi[0] = net->hs.psi;
+    if (net->cplx) psi[1] = net->hs.psi_i;
+  }
   return 0;
   CATN_CATCH
 }
@@ -1185,11 +1201,20 @@ int catn_contraction_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, 
                            double* error, double* psi) {
   CATN_TRY
   std::vector<catn_net*> v(nets, nets + count);
+  for (int k = 1; k < count; ++k)
+    if (v[k]->cplx != v[0]->cplx)
+      throw catn::Error(-2, "contraction_batch: real and complex networks in one batch");
   catn::contract_nets(ctx->ex.get(), v);
+  const bool cx = count > 0 && v[0]->cplx;
   for (int k = 0; k < count; ++k) {
     if (lnZ) lnZ[k] = v[k]->hs.lnZ + v[k]->hs.lntail;
     if (error) error[k] = v[k]->hs.err;
-    if (psi) psi[k] = v[k]->hs.psi;
+    if (psi && cx) {
+      psi[2 * k] = v[k]->hs.psi;
+      psi[2 * k + 1] = v[k]->hs.psi_i;
+    } else if (psi) {
+      psi[k] = v[k]->hs.psi;
+    }
   }
   return 0;
   CATN_CATCH
@@ -1267,7 +1292,10 @@ int catn_net_bwd_profile(catn_net* net, int64_t* ticks, double* replay_dev, int3
 
 int catn_net_info(catn_net* net, double* sign, int32_t* maxdim, int32_t* num_isolated, int32_t* n,
                   double* beta) {
-  if (sign) *sign = net->hs.sign;
+  if (sign) {
+    sign[0] = net->hs.sign;
+    if (net->cplx) sign[1] = net->hs.sign_i;
+  }
   if (maxdim) *maxdim = net->plan().maxdim_intermediate;
   if (num_isolated) *num_isolated = net->num_isolated;
   if (n) *n = net->n;
@@ -1462,7 +1490,10 @@ int catn_node_eat(catn_net* net, int32_t id_i, int32_t id_j, int32_t idx, int32_
   catn::run_one(net);
   if (lognorm) *lognorm = net->hs.lnZ;
   if (error) *error = net->hs.err;
-  if (phase) *phase = net->hs.psi;
+  if (phase) {
+    phase[0] = net->hs.psi;
+    if (net->cplx) phase[1] = net->hs.psi_i;
+  }
   return 0;
   CATN_CATCH
 }
@@ -1512,13 +1543,13 @@ int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t ca
   net->ensure_device();
   catn::SiteDesc d = net->read_desc(nd.slot[k - 1]);
   const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
-  if (cnt > cap) throw catn::Error(-2, "output buffer too small");
+  if (cnt * net->ed() > cap) throw catn::Error(-2, "output buffer too small");
   if (net->f32) {
     std::vector<float> fb(cnt);
     catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off, sizeof(float) * cnt);
-    for (size_t q = 0; q < cnt; ++q) out[q] = fb[q];
+    for (int64_t q = 0; q < cnt; ++q) out[q] = fb[q];
   } else {
-    catn::exec_of(net)->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
+    catn::exec_of(net)->d2h(out, net->hs.arena + d.off * net->ed(), net->elem_bytes() * cnt);
   }
   return 0;
   CATN_CATCH
@@ -1540,8 +1571,9 @@ int catn_result_tensor(catn_net* net, double* out, int64_t cap, int32_t* rank, i
   if (open.empty()) {   // closed network: 0-d one() (:520)
     if (rank) *rank = 0;
     if (!out) return 0;
-    if (cap < 1) throw catn::Error(-2, "result_tensor: output buffer too small");
+    if (cap < net->ed()) throw catn::Error(-2, "result_tensor: output buffer too small");
     out[0] = 1.0;
+    if (net->cplx) out[1] = 0.0;
     return 0;
   }
   if (open.size() != 1)   // :521
@@ -1559,13 +1591,15 @@ int catn_result_tensor(catn_net* net, double* out, int64_t cap, int32_t* rank, i
   }
   if (rank) *rank = r;
   if (!out) return 0;   // query: rank, dims and labels only
-  if (total > cap) throw catn::Error(-2, "result_tensor: output buffer too small");
+  if (total * net->ed() > cap) throw catn::Error(-2, "result_tensor: output buffer too small");
   net->ensure_device();
   if (r > 1) {   // mps2raw on the device: the sites fused into one
     const double lz = net->hs.lnZ, er = net->hs.err, ps = net->hs.psi, sg = net->hs.sign;
+    const double psi_i = net->hs.psi_i, sg_i = net->hs.sign_i;
     net->pl.fuse_all(nd);
     catn::run_one(net);
     net->hs.lnZ = lz; net->hs.err = er; net->hs.psi = ps; net->hs.sign = sg;
+    net->hs.psi_i = psi_i; net->hs.sign_i = sg_i;
   }
   const catn::SiteDesc d = net->read_desc(nd.slot[0]);
   const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
@@ -1575,7 +1609,7 @@ int catn_result_tensor(catn_net* net, double* out, int64_t cap, int32_t* rank, i
     catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off, sizeof(float) * cnt);
     for (int64_t q = 0; q < cnt; ++q) out[q] = fb[q];
   } else {
-    catn::exec_of(net)->d2h(out, net->hs.arena + d.off, sizeof(double) * cnt);
+    catn::exec_of(net)->d2h(out, net->hs.arena + d.off * net->ed(), net->elem_bytes() * cnt);
   }
   return 0;
   CATN_CATCH

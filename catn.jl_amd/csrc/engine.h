@@ -2916,7 +2916,10 @@ CATN_DN void op_cut(Exec& E, const Op& op) {
   E.rec_commit();
   E.set_site(op.a, offa, dal, myd, dar);
   E.set_site(op.b, offb, dbl, myd, dbr);
-  if (g.tid == 0) { E.st->results[op.e] = myd; E.st->last_cut = myd; }
+  if (g.tid == 0) {
+    if (op.e >= 0 && op.e < E.st->nresults) E.st->results[op.e] = myd;
+    E.st->last_cut = myd;
+  }
   g.sync();
 }
 
@@ -3071,6 +3074,12 @@ CATN_DN void op_tsvd(Exec& E, const Op& op) {
   g.sync();
 }
 
+}  // namespace catn
+
+#include "zengine.h"
+
+namespace catn {
+
 // Execute ops [begin, end) of one instance. Returns the status.
 CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, int end) {
   Exec E;
@@ -3112,7 +3121,8 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
         break;
       }
     }
-    switch (op.code) {
+    if (st->cplx) zrun_op(E, op);
+    else switch (op.code) {
       case OP_CANO_R: op_cano_r(E, op); break;
       case OP_CANO_L: op_cano_l(E, op); break;
       case OP_SWAP: op_swap(E, op); break;

@@ -152,6 +152,10 @@ struct InstState {
   // and the swap theta GEMM runs as split-BF16 products accumulated in FP32,
   // with 2 (3 passes) or, for 2, 3 BF16 terms per operand (6 passes)
   int32_t f32;
+  // 1: complex site tensors (ComplexF64), interleaved (re, im) doubles at
+  // twice the element offset; psi and sign then carry imaginary parts
+  int32_t cplx;
+  double psi_i, sign_i;
 };
 
 enum Status : int32_t {

@@ -10,6 +10,12 @@
  * leg positions are 1-based like the reference. Every call returns 0 on
  * success or a negative status; catn_last_error() gives the message.
  *
+ * Element type: a network built with catn_params.eltype = 1 is complex
+ * (TensorNetwork{ComplexF64}). Its tensor data arguments and outputs are then
+ * interleaved (re, im) doubles, the memory layout of Julia's ComplexF64
+ * arrays (pass a Ptr{ComplexF64} where `const double*` is declared), and its
+ * scalar phase outputs (psi, sign, eat!'s phase) take 2 doubles (re, im).
+ *
  * Threading: a context is thread-compatible, not thread-safe (one host
  * thread, one HIP stream), as the reference is single-threaded.
  */
@@ -60,8 +66,10 @@ typedef struct catn_params {
    * this is opt-in; catn_braket_network ignores `Dmax`. */
   int32_t braket_dmax;
   /* Element type: 0 real (Float64, or Float32 storage per `precision`), 1
-   * complex (ComplexF64 / ComplexF32). This build computes real networks
-   * only; 1 is rejected with an error naming the missing element type. */
+   * complex (ComplexF64; precision must be 0, ComplexF32 storage is refused).
+   * Complex networks run the plain forward of every entry point below except
+   * ising_network (real site tensors) and the reverse sweep (real only),
+   * which refuse them with an error that names the element type. */
   int32_t eltype;
 } catn_params;
 
@@ -111,7 +119,8 @@ int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
 
 /* braket_network(tensors, ixs; chi, cutoff, norm_method, svdopt, swapopt)
  * (src/braket.jl:369-496): the double-layer network <psi|psi> of a ket state
- * given like catn_tensor_network (real tensors; each has exactly one open,
+ * given like catn_tensor_network (real or complex tensors, the bra chains
+ * built from conj(T); each has exactly one open,
  * physical label; the virtual-bond graph must be acyclic). catn_contraction
  * then runs contraction!(bk) (src/braket.jl:960-1078): lnZ = ln <psi|psi>. */
 int catn_braket_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
@@ -133,10 +142,13 @@ int catn_net_upload(catn_net* net);
 int catn_net_free(catn_net* net);
 
 /* contraction!(tn) -> (lnZ, error, psi) (src/tensor_network.jl:556-717).
- * Consumes the network like the reference. */
+ * Consumes the network like the reference. psi: 1 double, or 2 (re, im) for
+ * a complex network. */
 int catn_contraction(catn_net* net, double* lnZ, double* error, double* psi);
 /* Same for `count` independent networks in ONE persistent launch per planning
- * segment (one workgroup per network). Outputs are arrays of length count. */
+ * segment (one workgroup per network). Outputs are arrays of length count;
+ * for complex networks psi holds 2 doubles per network. Real and complex
+ * networks cannot share a batch. */
 int catn_contraction_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, double* lnZ,
                            double* error, double* psi);
 
@@ -169,7 +181,8 @@ int catn_net_bwd_profile(catn_net* net, int64_t* ticks, double* replay_dev, int3
  * record's scalar (norm, cutoff, phase or final scalar). */
 int catn_net_tape(catn_net* net, int32_t* rec, double* val, int32_t cap);
 
-/* Per-network results after contraction. */
+/* Per-network results after contraction (sign: 2 doubles for a complex
+ * network, the product of the final scalar nodes' z / |z|). */
 int catn_net_info(catn_net* net, double* sign, int32_t* maxdim_intermediate,
                   int32_t* num_isolated, int32_t* n, double* beta);
 /* The selected edge sequence (pairs, 2*count ints); *count in: capacity. */
@@ -234,7 +247,8 @@ int catn_node_move2head(catn_net* net, int32_t id, int32_t idx, double* error); 
 int catn_node_reverse(catn_net* net, int32_t id);                                  /* reverse! :238 */
 int catn_node_merge(catn_net* net, int32_t id, int32_t j, int32_t cross, double* error); /* merge! :595 */
 int catn_node_compress(catn_net* net, int32_t id, int32_t opt, double* error);     /* compress(_opt)! :356/:394 */
-/* eat!(node_i, node_j, idx, idxi) -> (lognorm, error, phase) (:463-575) */
+/* eat!(node_i, node_j, idx, idxi) -> (lognorm, error, phase) (:463-575);
+ * phase: 2 doubles for a complex network */
 int catn_node_eat(catn_net* net, int32_t id_i, int32_t id_j, int32_t idx, int32_t idxi,
                   double* lognorm, double* error, double* phase);
 /* cut_bondim! / cut_bondim_opt! (src/tensor_network.jl:342, :399) */
@@ -242,7 +256,7 @@ int catn_cut_bondim(catn_net* net, int32_t i, int32_t idx_j_in_i, int32_t opt, d
 /* Node structure: order, cano, neighbours (cap entries), per-site dims (3*cap) */
 int catn_node_get(catn_net* net, int32_t id, int32_t* order, int32_t* cano, int32_t* neighbor,
                   int32_t* dims, int32_t cap);
-/* Download site k (1-based) data (dl*d*dr doubles). */
+/* Download site k (1-based) data (dl*d*dr elements; cap in doubles). */
 int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t cap);
 /* result_tensor(tn) (src/tensor_network.jl:504-523), after catn_contraction:
  * the dense tensor of the one surviving node with an open leg (a site whose

@@ -40,7 +40,7 @@ struct Params
     precision::Int32       # 0 Float64; 1 Float32 storage, 3-pass BF16; 2 Float32, 6-pass
     loops::Int32           # braket_network only: 1 accepts cyclic bond graphs
     braket_dmax::Int32     # braket_network only: merged-bond Dmax (< 0: none)
-    eltype::Int32          # 0 real, 1 complex (refused by this build)
+    eltype::Int32          # 0 real, 1 complex (ComplexF64)
 end
 
 lasterr() = unsafe_string(ccall((:catn_last_error, lib), Cstring, ()))
@@ -60,12 +60,14 @@ function params(; Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, revers
                 svdopt=true, swapopt=true, compress=false, cut_bond=false, seed=1,
                 arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000,
                 loops=false, braket_dmax=-1, T::Type=Float64)
-    prec = T === Float64 ? 0 : T === Float32 ? 1 :
-           T <: Complex ? error("element type $T: complex networks are not built in libcatn_hip") :
+    prec = T === Float64 || T === ComplexF64 ? 0 : T === Float32 ? 1 :
+           T === ComplexF32 ? error("element type ComplexF32: complex networks are computed " *
+                                    "in ComplexF64 by libcatn_hip") :
            error("element type $T is not supported")
+    elt = T <: Complex ? 1 : 0
     Params(Dmax, chi, cutoff, norm_method, select, reverse, svdopt, swapopt, compress,
            cut_bond, seed, arena_elems, scratch_elems, rsvd_both, rsvd_any, prec, loops,
-           braket_dmax, 0)
+           braket_dmax, elt)
 end
 
 # stands in for TensorNetwork{T,AT}: the library owns the device state
@@ -74,11 +76,14 @@ mutable struct HipNetwork
     n::Int
     beta::Float64
     nedges::Int
+    T::Type                # element type: Float64, Float32 or ComplexF64
 end
-function wrap(ptr, n, beta, nedges)
-    tn = HipNetwork(ptr, n, beta, nedges)
+function wrap(ptr, n, beta, nedges, T=Float64)
+    tn = HipNetwork(ptr, n, beta, nedges, T)
     finalizer(t -> ccall((:catn_net_free, lib), Cint, (Ptr{Cvoid},), t.ptr), tn)
 end
+# a phase output of the library: 1 double, or (re, im) for complex networks
+phase(buf, T) = T <: Complex ? complex(buf[1], buf[2]) : buf[1]
 
 # ising_network (src/ising.jl:20-227)
 function ising_network(n, edges, w, h, β; pos1=0, val1=0, pos2=0, val2=0, kw...)
@@ -103,48 +108,54 @@ function _flatten(tensors, ixs)
     end
     nds = Int32[ndims(t) for t in tensors]
     dims = Int64[d for t in tensors for d in size(t)]
-    data = reduce(vcat, [vec(Float64.(t)) for t in tensors]; init=Float64[])
-    nds, dims, data, labels
+    # complex tensors go over as interleaved (re, im) doubles: ComplexF64's layout
+    T = any(t -> eltype(t) <: Complex, tensors) ? ComplexF64 : Float64
+    data = reduce(vcat, [vec(T.(t)) for t in tensors]; init=T[])
+    nds, dims, reinterpret(Float64, data) |> collect, labels, T
 end
 
 # TensorNetwork(tensors, ixs; kw...) (src/tensor_network.jl:41-140)
 function tensor_network(tensors, ixs; kw...)
-    nd, dm, dt, lb = _flatten(tensors, ixs)
-    p = Ref(params(; kw...))
+    nd, dm, dt, lb, T = _flatten(tensors, ixs)
+    p = Ref(params(; T=T, kw...))
     out = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:catn_tensor_network, lib), Cint,
                 (Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Int64}, Ptr{Float64}, Ptr{Int64},
                  Ref{Params}, Ref{Ptr{Cvoid}}), ctx(), length(tensors), nd, dm, dt, lb, p, out))
-    wrap(out[], length(tensors), 1.0, 0)
+    wrap(out[], length(tensors), 1.0, 0, T)
 end
 
-# braket_network(tensors, ixs; chi, cutoff, ...) (src/braket.jl:369-496), real tensors
+# braket_network(tensors, ixs; chi, cutoff, ...) (src/braket.jl:369-496), real or
+# complex tensors
 function braket_network(tensors, ixs; chi=64, loops=false, Dmax=-1, kw...)
-    nd, dm, dt, lb = _flatten(tensors, ixs)
-    p = Ref(params(; chi=chi, loops=loops, braket_dmax=Dmax, kw...))
+    nd, dm, dt, lb, T = _flatten(tensors, ixs)
+    p = Ref(params(; chi=chi, loops=loops, braket_dmax=Dmax, T=T, kw...))
     out = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:catn_braket_network, lib), Cint,
                 (Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Int64}, Ptr{Float64}, Ptr{Int64},
                  Ref{Params}, Ref{Ptr{Cvoid}}), ctx(), length(tensors), nd, dm, dt, lb, p, out))
-    wrap(out[], length(tensors), 1.0, 0)
+    wrap(out[], length(tensors), 1.0, 0, T)
 end
 
-# contraction!(tn) (src/tensor_network.jl:556) -> (lnZ, error, psi)
+# contraction!(tn) (src/tensor_network.jl:556) -> (lnZ, error, psi); psi::T
 function contraction!(tn::HipNetwork)
-    lz, err, psi = Ref(0.0), Ref(0.0), Ref(0.0)
+    lz, err, psi = Ref(0.0), Ref(0.0), zeros(2)
     check(ccall((:catn_contraction, lib), Cint,
-                (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}, Ref{Float64}), tn.ptr, lz, err, psi))
-    (lz[], err[], psi[])
+                (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}, Ptr{Float64}), tn.ptr, lz, err, psi))
+    (lz[], err[], phase(psi, tn.T))
 end
 
 # batched contraction! of many networks in one persistent launch per segment
+# (one element type per batch)
 function contraction!(nets::Vector{HipNetwork})
     k = length(nets)
-    lz, err, psi = zeros(k), zeros(k), zeros(k)
+    T = isempty(nets) ? Float64 : nets[1].T
+    w = T <: Complex ? 2 : 1
+    lz, err, psi = zeros(k), zeros(k), zeros(w * k)
     check(ccall((:catn_contraction_batch, lib), Cint,
                 (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
                 ctx(), k, [t.ptr for t in nets], lz, err, psi))
-    lz, err, psi
+    lz, err, (w == 2 ? reinterpret(ComplexF64, psi) |> collect : psi)
 end
 
 # free_energy (src/ising.jl:239-244) -> (lnZ/n, -lnZ/(n β))
@@ -159,11 +170,13 @@ function result_tensor(tn::HipNetwork)
     check(ccall((:catn_result_tensor, lib), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Int64, Ref{Int32}, Ptr{Int32}, Ptr{Int32}, Int32),
                 tn.ptr, C_NULL, 0, rank, dims, labs, 64))
-    rank[] == 0 && return fill(1.0)
-    out = zeros(Float64, Tuple(Int.(dims[1:rank[]])))
+    T = tn.T <: Complex ? ComplexF64 : Float64
+    rank[] == 0 && return fill(one(T))
+    out = zeros(T, Tuple(Int.(dims[1:rank[]])))
+    buf = reinterpret(Float64, vec(out))     # ComplexF64 as (re, im) pairs
     check(ccall((:catn_result_tensor, lib), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Int64, Ref{Int32}, Ptr{Int32}, Ptr{Int32}, Int32),
-                tn.ptr, out, length(out), rank, dims, labs, 64))
+                tn.ptr, buf, length(buf), rank, dims, labs, 64))
     out
 end
 

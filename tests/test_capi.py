@@ -45,14 +45,19 @@ def test_hip_library_has_no_cpu_fallback():
 
 def test_params_defaults_and_element_type(catn, emu_ctx):
     """catn_params_default: the reference's TensorNetwork defaults, the double
-    layer's merged-bond Dmax off (braket_dmax = -1) and real elements; a
-    complex element type is refused with an error that names it."""
+    layer's merged-bond Dmax off (braket_dmax = -1) and real elements. An
+    Ising network (real site tensors) refuses the complex element type, and
+    an element type other than 0 / 1 is refused."""
     p = catn._Params()
     emu_ctx.lib.catn_params_default(ctypes.byref(p))
     assert (p.Dmax, p.chi, p.select, p.norm_method) == (32, 32, 1, 1)
     assert p.braket_dmax == -1 and p.eltype == 0 and p.loops == 0
-    with pytest.raises(catn.CatnError, match="complex element types"):
+    with pytest.raises(catn.CatnError, match="real"):
         catn.ising_network(2, [(1, 2)], [1.0], [0.0, 0.0], 0.5, ctx=emu_ctx, eltype="complex")
+    p.eltype = 7
+    out = ctypes.c_void_p()
+    assert emu_ctx.lib.catn_net_create(emu_ctx.h, ctypes.byref(p), ctypes.byref(out)) != 0
+    assert "eltype" in emu_ctx.lib.catn_last_error().decode()
 
 
 def test_braket_ignores_tensor_network_dmax(catn, emu_ctx):

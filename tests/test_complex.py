@@ -1,0 +1,337 @@
+"""Complex element types (ComplexF64), after test/test_complex.jl.
+
+The oracle's restatement is pinned first on complex tensors by the checks of
+test/test_complex.jl (:7-64): eat! of two leaves is the non-conjugating
+sum(u .* v) and differs from dot(u, v); cut_bondim(_opt)! keeps a rank-2
+complex bond at Dmax = 2 and preserves the product; a loop of four rank-2
+complex matrices contracts losslessly at Dmax = 2; and generic chain, star
+and loop networks match the dense contraction (numpy einsum, the semantics of
+test/exact.jl's exact_contract) for select 0, 1, 2. A planted mistake (a
+conjugating leaf dot) fails the pin.
+
+The device path is then checked on the CPU emulator and, marked gpu, on the
+MI355X: the same reference checks; random complex graphs with chi and Dmax
+truncation against the oracle (edge sequence, maxdim_intermediate, Dmax-cut
+kept counts, lnZ to 1e-12, phase psi to 1e-10, discarded weight); complex
+double-layer networks (braket_network on complex PEPS, the bra built from
+conj(T)) against the dense <psi|psi> and the oracle; and open-leg readout of
+a complex network against the dense contraction.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import braket_oracle as B
+from oracle import catn_oracle as O
+from oracle import configs as Cf
+
+
+def _c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _dense(ts, ixs, out=()):
+    letters = {}
+    for ix in ixs:
+        for l in ix:
+            if l not in letters:
+                letters[l] = chr(ord("a") + len(letters))
+    spec = ",".join("".join(letters[l] for l in ix) for ix in ixs)
+    return np.einsum(spec + "->" + "".join(letters[l] for l in out), *ts)
+
+
+def _generic_networks(rng):
+    return [
+        ([_c(rng, 3, 4), _c(rng, 4, 5), _c(rng, 5, 3)], [["a", "b"], ["b", "c"], ["c", "a"]]),
+        ([_c(rng, 2, 3, 4), _c(rng, 2), _c(rng, 3), _c(rng, 4)],
+         [["a", "b", "c"], ["a"], ["b"], ["c"]]),
+        ([_c(rng, 2, 3), _c(rng, 3, 2), _c(rng, 2, 3), _c(rng, 3, 2)],
+         [["a", "b"], ["b", "c"], ["c", "d"], ["d", "a"]]),
+    ]
+
+
+def _loop_rank2(rng):
+    mats = [_c(rng, 4, 2) @ _c(rng, 2, 4) for _ in range(4)]
+    return mats, [["a", "b"], ["b", "c"], ["c", "d"], ["d", "a"]]
+
+
+def _random_graph(rng, n, D):
+    """Random 3-regular graph with a complex Gaussian tensor of bond dimension
+    D per node (legs in edge-list order) and one open-free closed network."""
+    edges = Cf.random_regular_edges(n, 3, rng)
+    ixs = [[] for _ in range(n)]
+    for e, (i, j) in enumerate(edges):
+        ixs[i - 1].append(f"e{e}")
+        ixs[j - 1].append(f"e{e}")
+    ts = [_c(rng, *([D] * len(ix))) / math.sqrt(D) for ix in ixs]
+    return ts, ixs
+
+
+# ----------------------------- oracle pins ----------------------------------
+
+def test_oracle_eat_leaves_non_conjugating():
+    """test/test_complex.jl:8-16."""
+    rng = np.random.default_rng(1)
+    u, v = _c(rng, 4), _c(rng, 4)
+    ni = O.MPSNode.from_tensor(u, [2], chi=32, cutoff=1e-15, norm_method=0)
+    nj = O.MPSNode.from_tensor(v, [1], chi=32, cutoff=1e-15, norm_method=0)
+    lg, err, ph = O.eat(ni, nj, 1, 1)
+    assert not ni.mps
+    assert abs(math.exp(lg) * ph - np.sum(u * v)) < 1e-12 * abs(np.sum(u * v))
+    assert abs(math.exp(lg) * ph - np.vdot(u, v)) > 1e-3
+
+
+@pytest.mark.parametrize("opt", [True, False])
+def test_oracle_cut_keeps_complex_low_rank_bond(opt):
+    """test/test_complex.jl:18-30."""
+    rng = np.random.default_rng(2)
+    P = _c(rng, 3, 2) @ _c(rng, 2, 4)
+    Qm = _c(rng, 4, 2) @ _c(rng, 2, 3)
+    tn = O.tensor_network([P, Qm], [["a", "m"], ["m", "b"]], chi=1000, Dmax=2)
+    (O.cut_bondim_opt if opt else O.cut_bondim)(tn, 1, 2)
+    assert tn.tensors[1].mps[1].shape[1] == 2
+    got = O.mps2raw(tn.tensors[1]) @ O.mps2raw(tn.tensors[2])
+    assert np.allclose(got, P @ Qm, atol=1e-8)
+
+
+@pytest.mark.parametrize("svdopt", [True, False])
+def test_oracle_finite_dmax_loop(svdopt):
+    """test/test_complex.jl:32-43."""
+    ts, ixs = _loop_rank2(np.random.default_rng(3))
+    ref = _dense(ts, ixs)
+    tn = O.tensor_network(ts, ixs, Dmax=2, chi=64, select=1, compress=True, svdopt=svdopt)
+    lnz, err, psi = O.contraction(tn)
+    assert abs(math.exp(lnz) * psi - ref) <= 1e-6 * abs(ref)
+
+
+@pytest.mark.parametrize("sel", [0, 1, 2])
+def test_oracle_generic_exact(sel):
+    """test/test_complex.jl:45-63."""
+    for ts, ixs in _generic_networks(np.random.default_rng(4)):
+        ref = _dense(ts, ixs)
+        tn = O.tensor_network(ts, ixs, Dmax=-1, chi=10_000, select=sel, reverse=True,
+                              compress=True, norm_method=1)
+        lnz, err, psi = O.contraction(tn)
+        assert abs(math.exp(lnz) * psi - ref) <= 1e-10 * abs(ref)
+
+
+def test_oracle_plants_a_mistake(monkeypatch):
+    """A conjugating leaf dot (dot(vi, vj) in eat! case (a)) fails the exact pin."""
+    ts, ixs = _generic_networks(np.random.default_rng(4))[0]
+    ref = _dense(ts, ixs)
+    orig = O.eat
+
+    def eat_conj(node, nodej, idx, idxi):
+        if len(node.mps) == 1:
+            node.mps[0] = node.mps[0].conj()
+        return orig(node, nodej, idx, idxi)
+
+    monkeypatch.setattr(O, "eat", eat_conj)
+    tn = O.tensor_network(ts, ixs, Dmax=-1, chi=10_000, select=1)
+    lnz, err, psi = O.contraction(tn)
+    assert abs(math.exp(lnz) * psi - ref) > 1e-3 * abs(ref)
+
+
+# ----------------------------- device path ----------------------------------
+
+@pytest.fixture(scope="module", params=["emu", pytest.param("gpu", marks=pytest.mark.gpu)])
+def env(request, catn):
+    if request.param == "emu":
+        return catn, request.getfixturevalue("emu_ctx")
+    return catn, request.getfixturevalue("gpu_ctx")
+
+
+def test_device_eat_leaves_non_conjugating(env):
+    """test/test_complex.jl:8-16 on the device: the leaf dot of eat! case (a)."""
+    catn, ctx = env
+    rng = np.random.default_rng(1)
+    u, v = _c(rng, 4), _c(rng, 4)
+    net = catn.MPSNet(ctx=ctx, eltype="complex", norm_method=0)
+    net.add_node(1, [u.reshape(1, 4, 1)], [2])
+    net.add_node(2, [v.reshape(1, 4, 1)], [1])
+    lg, err, ph = net.eat(1, 2, 1, 1)
+    assert isinstance(ph, complex)
+    assert abs(math.exp(lg) * ph - np.sum(u * v)) < 1e-12 * abs(np.sum(u * v))
+    assert abs(math.exp(lg) * ph - np.vdot(u, v)) > 1e-3
+
+
+@pytest.mark.parametrize("opt", [True, False])
+def test_device_cut_keeps_complex_low_rank_bond(env, opt):
+    """test/test_complex.jl:18-30 on the device."""
+    catn, ctx = env
+    rng = np.random.default_rng(2)
+    P = _c(rng, 3, 2) @ _c(rng, 2, 4)
+    Qm = _c(rng, 4, 2) @ _c(rng, 2, 3)
+    net = catn.MPSNet(ctx=ctx, eltype="complex", chi=1000, Dmax=2)
+    net.add_tensor(1, P, [-1, 2])
+    net.add_tensor(2, Qm, [1, -2])
+    net.cut_bondim(1, 2, opt=opt)
+    s1, s2 = net.sites(1), net.sites(2)
+    assert s1[1].shape[1] == 2 and s2[0].shape[1] == 2
+    A = np.einsum("xai,iby->ab", s1[0], s1[1])        # mps2raw of node 1: (a, m)
+    Bm = np.einsum("xai,iby->ab", s2[0], s2[1])       # node 2: (m, b)
+    assert np.allclose(A @ Bm, P @ Qm, atol=1e-8)
+
+
+@pytest.mark.parametrize("svdopt", [True, False])
+def test_device_finite_dmax_loop(env, svdopt):
+    """test/test_complex.jl:32-43 on the device."""
+    catn, ctx = env
+    ts, ixs = _loop_rank2(np.random.default_rng(3))
+    ref = _dense(ts, ixs)
+    tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=2, chi=64, select=1, compress=True,
+                             svdopt=svdopt)
+    lnz, err, psi = catn.contraction(tn)
+    assert isinstance(psi, complex)
+    assert abs(math.exp(lnz) * psi - ref) <= 1e-6 * abs(ref)
+
+
+@pytest.mark.parametrize("sel", [0, 1, 2])
+@pytest.mark.parametrize("compress", [True, False])
+def test_device_generic_exact(env, sel, compress):
+    """test/test_complex.jl:45-63 on the device, with and without compress."""
+    catn, ctx = env
+    for ts, ixs in _generic_networks(np.random.default_rng(4)):
+        ref = _dense(ts, ixs)
+        tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=-1, chi=10_000, select=sel, reverse=True,
+                                 compress=compress, norm_method=1)
+        lnz, err, psi = catn.contraction(tn)
+        assert abs(math.exp(lnz) * psi - ref) <= 1e-10 * abs(ref)
+
+
+@pytest.mark.parametrize("n,D,Dmax,chi,sel,norm,seed", [
+    (10, 2, -1, 10_000, 1, 1, 1),    # exact
+    (12, 2, 3, 6, 1, 1, 2),          # Dmax cuts and chi truncation
+    (12, 3, 4, 8, 0, 2, 3),          # max-abs normalisation
+    (14, 2, 2, 4, 2, 1, 4),          # sequential selection
+    (10, 3, 5, 12, 1, 0, 5),         # no normalisation
+])
+def test_device_random_graph_matches_oracle(env, n, D, Dmax, chi, sel, norm, seed):
+    """Random complex 3-regular graphs, truncated by chi and Dmax: the device
+    and the oracle take the same schedule (edge sequence, maxdim, cut kept
+    counts) and give the same lnZ, phase and discarded weight."""
+    catn, ctx = env
+    ts, ixs = _random_graph(np.random.default_rng(seed), n, D)
+    kw = dict(Dmax=Dmax, chi=chi, select=sel, norm_method=norm)
+    ref = O.tensor_network(ts, ixs, **kw)
+    s = O.Schedule()
+    lo, eo, po = O.contraction(ref, s)
+    tn = catn.tensor_network(ts, ixs, ctx=ctx, **kw)
+    lnz, err, psi = catn.contraction(tn)
+    assert tn.edge_log() == s.edges
+    assert tn.maxdim_intermediate == ref.maxdim_intermediate
+    assert tn.cut_kept() == [e[2] for e in s.events if e[0] == "cut_keep"]
+    assert abs(lnz - lo) <= 1e-12 * max(1.0, abs(lo))
+    assert abs(psi * tn.sign - complex(po) * complex(ref.sign)) < 1e-10
+    assert abs(err - eo) <= 1e-8 * max(1.0, abs(eo))
+    if Dmax < 0:
+        val = _dense(ts, ixs)
+        assert abs(lnz - math.log(abs(val))) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["chain4_cplx", "chain5_cplx", "star_cplx"])
+@pytest.mark.parametrize("svdopt", [True, False])
+def test_device_braket_norm_matches_exact(env, name, svdopt):
+    """test/test_braket.jl:246-305 with complex tensors: <psi|psi> of complex
+    chains and a star, the bra chains built from conj(T)."""
+    catn, ctx = env
+    ts, ixs = _braket_case(name)
+    ref = B.exact_norm(ts, ixs)
+    tn = catn.braket_network(ts, ixs, ctx=ctx, chi=10_000, svdopt=svdopt)
+    lnz, err, psi = catn.contraction(tn)
+    assert abs(math.exp(lnz) * psi - ref) <= 1e-10 * ref
+    assert abs(psi - 1.0) < 1e-10
+
+
+def _braket_case(name):
+    """complex chains (p0 b0, b0 p1 b1, ...) and a star, as in test/test_braket.jl"""
+    rng = np.random.default_rng({"chain4_cplx": 11, "chain5_cplx": 12, "star_cplx": 13}[name])
+    if name == "star_cplx":
+        return ([_c(rng, 2, 2, 2, 2), _c(rng, 2, 2), _c(rng, 2, 2), _c(rng, 2, 2)],
+                [["a", "b", "c", "pc"], ["a", "p1"], ["b", "p2"], ["c", "p3"]])
+    n, D, d = (4, 3, 2) if name == "chain4_cplx" else (5, 3, 2)
+    ts, ixs = [], []
+    for k in range(n):
+        if k == 0:
+            ts.append(_c(rng, d, D)); ixs.append([f"p{k}", f"b{k}"])
+        elif k == n - 1:
+            ts.append(_c(rng, D, d)); ixs.append([f"b{k - 1}", f"p{k}"])
+        else:
+            ts.append(_c(rng, D, d, D)); ixs.append([f"b{k - 1}", f"p{k}", f"b{k}"])
+    return ts, ixs
+
+
+def _lattice(rng, Lx, Ly, D, d):
+    """open Lx x Ly PEPS, complex Gaussian entries, physical leg in the middle"""
+    ts, ixs = [], []
+    for x in range(Lx):
+        for y in range(Ly):
+            ix = []
+            if x > 0:
+                ix.append(f"v{x - 1}_{y}")
+            if y > 0:
+                ix.append(f"h{x}_{y - 1}")
+            ix.append(f"p{x}_{y}")
+            if y < Ly - 1:
+                ix.append(f"h{x}_{y}")
+            if x < Lx - 1:
+                ix.append(f"v{x}_{y}")
+            ts.append(_c(rng, *[d if l.startswith("p") else D for l in ix]) / math.sqrt(2))
+            ixs.append(ix)
+    return ts, ixs
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi,loops", [(2, 3, 2, 10_000, True), (3, 3, 2, 10_000, True),
+                                               (3, 3, 3, 8, True)])
+def test_device_complex_peps_norm_matches_oracle(env, Lx, Ly, D, chi, loops):
+    """A complex PEPS norm (the double layer of config 5, at lattice sizes the
+    dense norm and the oracle reach): the device matches the oracle's edge
+    sequence, lnZ and phase; exact runs also match the dense <psi|psi>."""
+    catn, ctx = env
+    ts, ixs = _lattice(np.random.default_rng(300 + Lx * Ly + D), Lx, Ly, D, 2)
+    bk = B.braket_network(ts, ixs, chi=chi, loops=loops)
+    lo, eo, po = B.contraction(bk)
+    tn = catn.braket_network(ts, ixs, ctx=ctx, chi=chi, loops=loops)
+    lnz, err, psi = catn.contraction(tn)
+    assert [tuple(e) for e in tn.edge_log()] == bk.edge_log
+    assert abs(lnz - lo) <= 1e-12 * max(1.0, abs(lo))
+    assert abs(psi - complex(po)) < 1e-10
+    if chi >= 10_000:
+        assert abs(lnz - math.log(B.exact_norm(ts, ixs))) < 1e-10
+
+
+def test_device_complex_open_legs(env):
+    """result_tensor (src/tensor_network.jl:504-523) of a complex chain with open
+    physical legs against the dense contraction."""
+    catn, ctx = env
+    rng = np.random.default_rng(11)
+    ts = [_c(rng, 2, 4), _c(rng, 4, 2, 4), _c(rng, 4, 2)]
+    ixs = [["p1", "a"], ["a", "p2", "b"], ["b", "p3"]]
+    tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=-1, chi=10_000)
+    lnz, err, psi = catn.contraction(tn)
+    arr, labels = catn.result_tensor(tn)
+    assert arr.dtype == np.complex128 and sorted(labels) == ["p1", "p2", "p3"]
+    ref = _dense(ts, ixs, labels)
+    got = arr * (math.exp(lnz) * psi)
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_device_complex_refusals(env):
+    """No reverse sweep, Ising networks or mixed batches for complex networks:
+    each is refused with an error that names it."""
+    catn, ctx = env
+    rng = np.random.default_rng(7)
+    ts, ixs = _generic_networks(rng)[0]
+    tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=-1, chi=10_000)
+    with pytest.raises(catn.CatnError, match="real networks"):
+        catn.gradient_batch([tn])
+    with pytest.raises(catn.CatnError, match="real"):
+        catn.ising_network(3, [(1, 2), (2, 3)], [1.0, 1.0], [0.0] * 3, 0.5, ctx=ctx,
+                           eltype="complex")
+    a = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=-1, chi=10_000)
+    b = catn.tensor_network([t.real for t in ts], ixs, ctx=ctx, Dmax=-1, chi=10_000)
+    with pytest.raises(catn.CatnError, match="real and complex"):
+        catn.contraction_batch([a, b])
+    with pytest.raises(catn.CatnError, match="ComplexF32"):
+        catn.tensor_network(ts, ixs, ctx=ctx, dtype="f32x6")

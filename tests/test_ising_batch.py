@@ -28,8 +28,7 @@ def _instances():
 
 def _sites(catn, tn, n):
     """Every node's site tensors, read back before contraction."""
-    view = catn.MPSNet.__new__(catn.MPSNet)    # the per-node readers on this handle
-    view.ctx, view.h = tn.ctx, tn.h
+    view = catn.MPSNet.view(tn)    # the per-node readers on this handle
     out = {}
     for node in range(1, n + 1):
         for k, site in enumerate(view.sites(node)):
