@@ -81,6 +81,7 @@ _SIGS = {
     "catn_net_add_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I32P, _DP, _I32P, C.c_int32]),
     "catn_net_add_tensor_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I64P, _DP, _I32P]),
     "catn_net_upload": (C.c_int, [_P]),
+    "catn_net_set_fault": (C.c_int, [_P, C.c_int32]),
     "catn_net_free": (C.c_int, [_P]),
     "catn_contraction": (C.c_int, [_P, _DP, _DP, _DP]),
     "catn_contraction_batch": (C.c_int, [_P, C.c_int32, C.POINTER(_P), _DP, _DP, _DP]),
@@ -370,7 +371,7 @@ class TensorNetwork:
         hzv = hz.value
         buckets["_large_svd"] = {"qr_s": sub[0] / hzv, "jacobi_s": sub[1] / hzv,
                                  "finish_s": sub[2] / hzv, "sweeps": int(sub[3]),
-                                 "global_sweeps": int(sub[11])}
+                                 "global_sweeps": int(sub[11]), "fallbacks": int(sub[12])}
         buckets["_certified_qr"] = {"qr_s": sub[4] / hzv, "inverse_s": sub[5] / hzv,
                                     "qform_s": sub[6] / hzv}
         buckets["_householder"] = {"panel_s": sub[8] / hzv, "update_s": sub[9] / hzv,
@@ -403,6 +404,11 @@ class TensorNetwork:
         out = np.zeros(size)
         self.ctx._check(self.ctx.lib.catn_net_site_adjoint(self.h, int(slot), _dptr(out), size))
         return out
+
+    def set_fault(self, flags: int):
+        """Test hook of tsvd's fallback: flags = 1 makes every SVD of this
+        network run its fallback path (catn_net_set_fault)."""
+        self.ctx._check(self.ctx.lib.catn_net_set_fault(self.h, int(flags)))
 
     def work(self):
         """(algorithmic flops, algorithmic bytes, scratch high-water mark)."""

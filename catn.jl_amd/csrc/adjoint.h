@@ -760,6 +760,7 @@ CATN_DN int run_backward(const WG& g, InstState* st) {
   E.bytes = 0.0;
   E.tape_len = st->tape_len;
   E.rp = nullptr;
+  E.g.svd_fault = st->fault & 1;   // replays take the forward's SVD path
   wg_fill(g, st->adj, 0.0, st->arena_top);
   if (g.tid == 0) st->replay_dev = 0.0;
   g.sync();

@@ -1172,6 +1172,16 @@ int catn_net_add_tensor_node(catn_net* net, int32_t id, int32_t ndim, const int6
   CATN_CATCH
 }
 
+int catn_net_set_fault(catn_net* net, int32_t flags) {
+  CATN_TRY
+  if (flags & ~1) throw catn::Error(-2, "set_fault: unknown flag bits");
+  if (net->cplx && flags) throw catn::Error(-2, "set_fault: the SVD fallback is built for real networks");
+  net->ensure_device();
+  net->hs.fault = flags;
+  return 0;
+  CATN_CATCH
+}
+
 int catn_net_upload(catn_net* net) {
   CATN_TRY
   net->ensure_device();

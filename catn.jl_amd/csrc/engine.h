@@ -114,6 +114,7 @@ struct WG {
   int64_t* sub;    // sub-step tick accumulators (instance state), or null
   int qb = kQBDefault;               // Householder panel width (8, 16 or 32)
   int jac_lanes = kJacLanesDefault;  // global Jacobi lanes per pair (8..64)
+  int svd_fault = 0;                 // 1: every SVD's primary path counts as not converged
 #if defined(CATN_EMU_MT)
   // the emulator checks that every lane arrives from the same call site
   void sync(const char* file = __builtin_FILE(), int line = __builtin_LINE()) const {
@@ -1672,19 +1673,60 @@ CATN_DN int wg_svd_tall(const WG& g, Scratch& sc, const double* A, int m, int n,
   return sw < 0 ? -1 : n;
 }
 
-// A (m x n, lda) -> SVD of A with U (m x r), V (n x r), r = min(m,n).
+// Fallback SVD (tsvd's retry with LAPACK's QR-iteration driver when the
+// divide-and-conquer one fails, src/linalg_utils.jl:16-21): one-sided Jacobi
+// straight on B = A, or A^T when m < n, held in scratch, with no QR
+// preconditioner and up to 200 sweeps (the primary paths stop at 60). Same
+// output conventions as wg_svd. Returns min(m, n), or -1.
+CATN_DN int wg_svd_fallback(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
+                            SvdResult& out, int* sweeps_used) {
+  const bool tr = m < n;
+  const int tm = tr ? n : m, tn = tr ? m : n;
+  double* W = sc.alloc((int64_t)tm * tn);
+  double* V = sc.alloc((int64_t)tn * tn);
+  double* U = sc.alloc((int64_t)tm * tn);
+  double* Ss = sc.alloc(tn);
+  double* Vs = sc.alloc((int64_t)tn * tn);
+  double* key = sc.alloc(tn);
+  int* rk = (int*)sc.alloc(tn);
+  if (sc.fail) return -1;
+  for (int64_t idx = g.tid; idx < (int64_t)tm * tn; idx += g.nthr) {
+    const int i = (int)(idx % tm), j = (int)(idx / tm);
+    W[idx] = tr ? A[j + (int64_t)i * lda] : A[i + (int64_t)j * lda];
+  }
+  g.sync();
+  const int sw = wg_jacobi(g, W, tm, tn, tm, V, tn, 200);
+  if (sweeps_used) *sweeps_used = sw;
+  jacobi_finish(g, W, tm, tn, tm, V, tn, key, rk, U, tm, Ss, Vs);
+  if (g.tid == 0 && g.sub) count_add(g.sub + 12, 1);
+  out.r = tn;
+  out.S = Ss;
+  out.U = tr ? Vs : U;   // m x r (ld m)
+  out.V = tr ? U : Vs;   // n x r (ld n)
+  return sw < 0 ? -1 : tn;
+}
+
+// A (m x n, lda) -> SVD of A with U (m x r), V (n x r), r = min(m,n). A
+// primary path that does not converge (or, with g.svd_fault, any primary
+// path) hands over to wg_svd_fallback.
 CATN_DN int wg_svd(const WG& g, Scratch& sc, const double* A, int m, int n, int64_t lda,
                    SvdResult& out, int* sweeps_used) {
-  if ((int64_t)m * n <= kTinySvd) return wg_svd_tiny(g, sc, A, m, n, lda, out, sweeps_used);
-  if (m >= n) return wg_svd_tall(g, sc, A, m, n, lda, false, out, sweeps_used);
-  // A^T = U' S V'^T  ->  A = V' S U'^T
-  SvdResult t;
-  const int r = wg_svd_tall(g, sc, A, n, m, lda, true, t, sweeps_used);
-  out.r = t.r;
-  out.U = t.V;   // m x r (ld m)
-  out.S = t.S;
-  out.V = t.U;   // n x r (ld n)
-  return r;
+  int r;
+  if ((int64_t)m * n <= kTinySvd) {
+    r = wg_svd_tiny(g, sc, A, m, n, lda, out, sweeps_used);
+  } else if (m >= n) {
+    r = wg_svd_tall(g, sc, A, m, n, lda, false, out, sweeps_used);
+  } else {
+    // A^T = U' S V'^T  ->  A = V' S U'^T
+    SvdResult t;
+    r = wg_svd_tall(g, sc, A, n, m, lda, true, t, sweeps_used);
+    out.r = t.r;
+    out.U = t.V;   // m x r (ld m)
+    out.S = t.S;
+    out.V = t.U;   // n x r (ld n)
+  }
+  if (sc.fail || (r >= 0 && !g.svd_fault)) return r;
+  return wg_svd_fallback(g, sc, A, m, n, lda, out, sweeps_used);
 }
 
 
@@ -3090,6 +3132,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
   E.flops = 0.0;
   E.bytes = 0.0;
   E.tape_len = st->tape_len;
+  E.g.svd_fault = st->fault & 1;
   if (g.tid == 0 && g.lds_cap > st->lds_max) st->lds_max = g.lds_cap;
   int64_t shwm = 0;
   for (int i = begin; i < end && E.status == STATUS_OK; ++i) {

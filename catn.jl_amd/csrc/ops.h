@@ -156,6 +156,9 @@ struct InstState {
   // twice the element offset; psi and sign then carry imaginary parts
   int32_t cplx;
   double psi_i, sign_i;
+  // test hook (catn_net_set_fault): bit 0 makes every SVD's primary path
+  // count as not converged, so that the fallback SVD runs
+  int32_t fault;
 };
 
 enum Status : int32_t {

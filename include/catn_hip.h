@@ -139,6 +139,13 @@ int catn_net_add_tensor_node(catn_net* net, int32_t id, int32_t ndim, const int6
 /* Allocate the network's device state and upload its site tensors now
  * (otherwise done lazily by the first call that runs on the device). */
 int catn_net_upload(catn_net* net);
+/* Test hook of tsvd's fallback (src/linalg_utils.jl:16-21: svd() retried with
+ * the QR-iteration driver on a LAPACKException). flags bit 0: every SVD of
+ * this network treats its primary path (QR-preconditioned one-sided Jacobi)
+ * as not converged, so the fallback (one-sided Jacobi on the operand itself,
+ * up to 200 sweeps) runs, as it does on a real non-convergence. Fallbacks
+ * taken are counted in catn_net_svd_profile entry 12. Real networks only. */
+int catn_net_set_fault(catn_net* net, int32_t flags);
 int catn_net_free(catn_net* net);
 
 /* contraction!(tn) -> (lnZ, error, psi) (src/tensor_network.jl:556-717).
@@ -213,7 +220,8 @@ int catn_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t
 /* Sub-step ticks (16 entries): SVD path [QR, Jacobi, finish + Q apply, Jacobi
  * sweep count], certified-QR path [QR, inverse bound, Q formation], swap theta,
  * blocked Householder [panels, T factors + trailing updates], blocked Q
- * application, Jacobi sweeps on global-memory operands (count); rest zero. */
+ * application, Jacobi sweeps on global-memory operands (count), fallback SVDs
+ * taken (count); rest zero. */
 int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks);
 /* Host time spent planning and in device round trips (launch, copies,
  * waits) since the last catn_kernel_stats reset, in seconds. */
