@@ -200,7 +200,7 @@ struct catn_net {
   bool f32 = false;        // FP32 site storage (catn_params.precision = 1 or 2)
   int bf16_terms = 2;      // BF16 terms per FP32 operand of the theta GEMM
   bool cplx = false;       // complex site tensors (catn_params.eltype = 1)
-  size_t elem_bytes() const { return cplx ? 2 * sizeof(double) : f32 ? sizeof(float) : sizeof(double); }
+  size_t elem_bytes() const { return (cplx ? 2 : 1) * (f32 ? sizeof(float) : sizeof(double)); }
   // doubles per element of host data (interleaved re, im when complex)
   int ed() const { return cplx ? 2 : 1; }
 
@@ -321,7 +321,7 @@ struct catn_net {
       if (!buf.empty()) {
         if (f32) {
           std::vector<float> fb(buf.begin(), buf.end());
-          ex->h2d((float*)hs.arena + base, fb.data(), sizeof(float) * fb.size());
+          ex->h2d((float*)hs.arena + base * ed(), fb.data(), sizeof(float) * fb.size());
         } else {
           ex->h2d(hs.arena + base * ed(), buf.data(), sizeof(double) * buf.size());
         }
@@ -830,12 +830,12 @@ static void apply_params(catn_net* net, const catn_params* p) {
   if (p->eltype != 0 && p->eltype != 1) throw catn::Error(-2, "eltype must be 0 (real) or 1 (complex)");
   if (p->precision < 0 || p->precision > 2)
     throw catn::Error(-2, "precision must be 0 (FP64), 1 (FP32, split-BF16 3-pass) or 2 (FP32, 6-pass)");
-  if (p->eltype == 1 && p->precision != 0)
-    throw catn::Error(-2, "complex networks are computed in ComplexF64 (precision 0); ComplexF32 "
-                          "storage is not built");
   net->cplx = p->eltype == 1;
   net->f32 = p->precision != 0;
-  net->bf16_terms = p->precision == 2 ? 3 : 2;
+  // ComplexF32 storage (eltype 1, precision 1 or 2): FP32 (re, im) pairs in
+  // the arena, every product in FP64, so the storage's noise floor (level 2)
+  // applies whichever precision level was asked for
+  net->bf16_terms = (p->precision == 2 || net->cplx) ? 3 : 2;
   net->arena_cap = p->arena_elems;
   net->scratch_cap = p->scratch_elems;
 }
@@ -1555,9 +1555,10 @@ int catn_node_site(catn_net* net, int32_t id, int32_t k, double* out, int64_t ca
   const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
   if (cnt * net->ed() > cap) throw catn::Error(-2, "output buffer too small");
   if (net->f32) {
-    std::vector<float> fb(cnt);
-    catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off, sizeof(float) * cnt);
-    for (int64_t q = 0; q < cnt; ++q) out[q] = fb[q];
+    const int64_t nv = cnt * net->ed();
+    std::vector<float> fb(nv);
+    catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off * net->ed(), sizeof(float) * nv);
+    for (int64_t q = 0; q < nv; ++q) out[q] = fb[q];
   } else {
     catn::exec_of(net)->d2h(out, net->hs.arena + d.off * net->ed(), net->elem_bytes() * cnt);
   }
@@ -1615,9 +1616,10 @@ int catn_result_tensor(catn_net* net, double* out, int64_t cap, int32_t* rank, i
   const int64_t cnt = (int64_t)d.dl * d.d * d.dr;
   if (cnt != total) throw catn::Error(-3, "result_tensor: fused site has the wrong size");
   if (net->f32) {
-    std::vector<float> fb(cnt);
-    catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off, sizeof(float) * cnt);
-    for (int64_t q = 0; q < cnt; ++q) out[q] = fb[q];
+    const int64_t nv = cnt * net->ed();
+    std::vector<float> fb(nv);
+    catn::exec_of(net)->d2h(fb.data(), (float*)net->hs.arena + d.off * net->ed(), sizeof(float) * nv);
+    for (int64_t q = 0; q < nv; ++q) out[q] = fb[q];
   } else {
     catn::exec_of(net)->d2h(out, net->hs.arena + d.off * net->ed(), net->elem_bytes() * cnt);
   }

@@ -2331,15 +2331,18 @@ struct Exec {
   CATN_D double noise_floor() const {
     return st->f32 == 2 ? 0x1p-20 : st->f32 == 1 ? 0x1p-18 : 0.0;
   }
+  // values per element: 2 (re, im) for complex networks
+  CATN_D int ew() const { return st->cplx ? 2 : 1; }
   // FP64 copy of the tensor version at arena offset `off` (n elements) into
   // scratch; every lane calls it. False when the scratch is exhausted.
   CATN_D bool convert_in(int64_t off, int64_t n) {
     for (int q = 0; q < nconv; ++q)
       if (conv_off[q] == off) return true;
     if (nconv >= kStage) { status = STATUS_UNSUPPORTED; return false; }
+    n *= ew();
     double* buf = sc.alloc(n > 0 ? n : 1);
     if (sc.fail) { status = STATUS_SCRATCH_FULL; return false; }
-    const float* src = arena_f() + off;
+    const float* src = arena_f() + off * ew();
     for (int64_t i = g.tid; i < n; i += g.nthr) buf[i] = (double)src[i];
     conv_off[nconv] = off;
     conv_buf[nconv] = buf;
@@ -2359,7 +2362,7 @@ struct Exec {
     top = a + n;
     if (st->f32) {
       if (nstage >= kStage) { status = STATUS_UNSUPPORTED; return 0; }
-      double* buf = sc.alloc(n > 0 ? n : 1);
+      double* buf = sc.alloc(n > 0 ? n * ew() : 1);
       if (sc.fail) { status = STATUS_SCRATCH_FULL; return 0; }
       stage_off[nstage] = a;
       stage_n[nstage] = n;
@@ -2379,9 +2382,10 @@ struct Exec {
   CATN_D void flush_out() {
     if (!st->f32) return;
     float* dst = arena_f();
+    const int w = ew();
     for (int q = 0; q < nstage; ++q)
-      for (int64_t i = g.tid; i < stage_n[q]; i += g.nthr)
-        dst[stage_off[q] + i] = (float)stage_buf[q][i];
+      for (int64_t i = g.tid; i < stage_n[q] * w; i += g.nthr)
+        dst[stage_off[q] * w + i] = (float)stage_buf[q][i];
   }
   CATN_D void set_site(int s, int64_t off, int dl, int d, int dr) {
     if (g.tid == 0) {

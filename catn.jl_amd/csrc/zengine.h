@@ -1,9 +1,12 @@
-// Complex element type (ComplexF64) of the device interpreter.
+// Complex element types (ComplexF64, ComplexF32 storage) of the device
+// interpreter.
 //
 // A complex network (InstState::cplx = 1) stores every site tensor as
-// interleaved (re, im) doubles: element offset e of a site lives at doubles
-// 2e, 2e + 1 of the arena, so descriptors, slot tables, programs and the
-// planner are the ones of the real engine. Each opcode has a complex
+// interleaved (re, im) values: element offset e of a site lives at values
+// 2e, 2e + 1 of the arena (doubles, or floats with ComplexF32 storage, where
+// each op reads FP64 copies of its inputs and its outputs are rounded into
+// the arena after it, as for the real FP32 storage), so descriptors, slot
+// tables, programs and the planner are the ones of the real engine. Each opcode has a complex
 // counterpart here with the reference's semantics for complex T:
 //   * every factorisation is a truncated SVD A = U S V^H (one-sided Jacobi on
 //     the columns, complex rotations), handed on as L R = A with the real
@@ -42,8 +45,15 @@ CATN_D double zabs2(zd a) { return a.re * a.re + a.im * a.im; }
 CATN_D double zabs(zd a) { return hypot(a.re, a.im); }
 
 CATN_D zd* zalloc(Scratch& sc, int64_t n) { return (zd*)sc.alloc(2 * n); }
-CATN_D zd* zsite(Exec& E, const SiteDesc& s) { return (zd*)E.st->arena + s.off; }
-CATN_D zd* zsite_at(Exec& E, int64_t off) { return (zd*)E.st->arena + off; }
+// an op's input tensor, and where it writes the output allocated at `off`:
+// the arena itself, or with ComplexF32 storage (f32) the FP64 copies and
+// staging buffers of the op (Exec::convert_in, Exec::alloc)
+CATN_D zd* zsite(Exec& E, const SiteDesc& s) {
+  return E.st->f32 ? (zd*)E.view(s.off) : (zd*)E.st->arena + s.off;
+}
+CATN_D zd* zsite_at(Exec& E, int64_t off) {
+  return E.st->f32 ? (zd*)E.out(off) : (zd*)E.st->arena + off;
+}
 
 // C (m x n, ldc) = op(A) op(B) (+ C when acc), op = transpose when ta / tb,
 // never conjugated; one lane per entry of C, summed over q in order.
@@ -226,6 +236,14 @@ CATN_DN bool zsplit(Exec& E, const zd* A, int m, int n, int64_t lda, int form, i
     return false;
   }
   E.work(4.0 * svd_flops(m, n), 0.0);
+  // ComplexF32 storage: singular values below the storage's noise floor
+  // count as zero, as in the real engine's FP32 storage
+  const double noise = E.noise_floor();
+  if (noise > 0.0) {
+    double f2 = 0.0;
+    for (int i = 0; i < R.r; ++i) f2 += R.S[i] * R.S[i];
+    cutoff = fmax(cutoff, noise * sqrt(f2));
+  }
   const int k = rule == 1 ? keep_cano(R.S, R.r, cutoff) : keep_tsvd(R.S, R.r, cutoff, maxdim);
   zd* L = zalloc(E.sc, (int64_t)m * k);
   zd* Rm = zalloc(E.sc, (int64_t)k * n);

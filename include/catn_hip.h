@@ -66,8 +66,10 @@ typedef struct catn_params {
    * this is opt-in; catn_braket_network ignores `Dmax`. */
   int32_t braket_dmax;
   /* Element type: 0 real (Float64, or Float32 storage per `precision`), 1
-   * complex (ComplexF64; precision must be 0, ComplexF32 storage is refused).
-   * Complex networks run the plain forward of every entry point below except
+   * complex: ComplexF64, or with precision 1 or 2 ComplexF32 storage (FP32
+   * (re, im) pairs in the arena, every product computed in FP64; host data
+   * and outputs stay interleaved doubles). Complex networks run the plain
+   * forward of every entry point below except
    * ising_network (real site tensors) and the reverse sweep (real only),
    * which refuse them with an error that names the element type. */
   int32_t eltype;

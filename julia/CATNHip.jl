@@ -40,7 +40,7 @@ struct Params
     precision::Int32       # 0 Float64; 1 Float32 storage, 3-pass BF16; 2 Float32, 6-pass
     loops::Int32           # braket_network only: 1 accepts cyclic bond graphs
     braket_dmax::Int32     # braket_network only: merged-bond Dmax (< 0: none)
-    eltype::Int32          # 0 real, 1 complex (ComplexF64)
+    eltype::Int32          # 0 real, 1 complex (ComplexF64; ComplexF32 with precision 1)
 end
 
 lasterr() = unsafe_string(ccall((:catn_last_error, lib), Cstring, ()))
@@ -60,9 +60,7 @@ function params(; Dmax=32, chi=32, cutoff=1e-15, norm_method=1, select=1, revers
                 svdopt=true, swapopt=true, compress=false, cut_bond=false, seed=1,
                 arena_elems=0, scratch_elems=0, rsvd_both=7000, rsvd_any=20000,
                 loops=false, braket_dmax=-1, T::Type=Float64)
-    prec = T === Float64 || T === ComplexF64 ? 0 : T === Float32 ? 1 :
-           T === ComplexF32 ? error("element type ComplexF32: complex networks are computed " *
-                                    "in ComplexF64 by libcatn_hip") :
+    prec = T === Float64 || T === ComplexF64 ? 0 : T === Float32 || T === ComplexF32 ? 1 :
            error("element type $T is not supported")
     elt = T <: Complex ? 1 : 0
     Params(Dmax, chi, cutoff, norm_method, select, reverse, svdopt, swapopt, compress,
@@ -108,9 +106,13 @@ function _flatten(tensors, ixs)
     end
     nds = Int32[ndims(t) for t in tensors]
     dims = Int64[d for t in tensors for d in size(t)]
-    # complex tensors go over as interleaved (re, im) doubles: ComplexF64's layout
-    T = any(t -> eltype(t) <: Complex, tensors) ? ComplexF64 : Float64
-    data = reduce(vcat, [vec(T.(t)) for t in tensors]; init=T[])
+    # complex tensors go over as interleaved (re, im) doubles: ComplexF64's
+    # layout; Float32 / ComplexF32 inputs select FP32 storage on the device
+    cx = any(t -> eltype(t) <: Complex, tensors)
+    f32 = !isempty(tensors) && all(t -> real(eltype(t)) === Float32, tensors)
+    W = cx ? ComplexF64 : Float64
+    data = reduce(vcat, [vec(W.(t)) for t in tensors]; init=W[])
+    T = cx ? (f32 ? ComplexF32 : ComplexF64) : (f32 ? Float32 : Float64)
     nds, dims, reinterpret(Float64, data) |> collect, labels, T
 end
 

@@ -15,7 +15,8 @@ truncation against the oracle (edge sequence, maxdim_intermediate, Dmax-cut
 kept counts, lnZ to 1e-12, phase psi to 1e-10, discarded weight); complex
 double-layer networks (braket_network on complex PEPS, the bra built from
 conj(T)) against the dense <psi|psi> and the oracle; and open-leg readout of
-a complex network against the dense contraction.
+a complex network against the dense contraction. ComplexF32 storage is
+checked against the dense contraction and against ComplexF64.
 """
 import math
 
@@ -333,5 +334,38 @@ def test_device_complex_refusals(env):
     b = catn.tensor_network([t.real for t in ts], ixs, ctx=ctx, Dmax=-1, chi=10_000)
     with pytest.raises(catn.CatnError, match="real and complex"):
         catn.contraction_batch([a, b])
-    with pytest.raises(catn.CatnError, match="ComplexF32"):
-        catn.tensor_network(ts, ixs, ctx=ctx, dtype="f32x6")
+
+
+@pytest.mark.parametrize("sel", [0, 1, 2])
+def test_device_complexf32_generic(env, sel):
+    """ComplexF32 storage (TensorNetwork{ComplexF32}): FP32 (re, im) pairs in
+    the arena, every product in FP64; the generic networks of
+    test/test_complex.jl match the dense contraction to FP32 accuracy."""
+    catn, ctx = env
+    for ts, ixs in _generic_networks(np.random.default_rng(4)):
+        ts32 = [t.astype(np.complex64).astype(np.complex128) for t in ts]
+        ref = _dense(ts32, ixs)
+        tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=-1, chi=10_000, select=sel,
+                                 compress=True, dtype="f32")
+        lnz, err, psi = catn.contraction(tn)
+        assert isinstance(psi, complex)
+        assert abs(math.exp(lnz) * psi - ref) <= 2e-6 * abs(ref)
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(3, 3, 2, 10_000), (3, 3, 3, 8), (3, 4, 2, 12)])
+def test_device_complexf32_peps_norm(env, Lx, Ly, D, chi):
+    """A complex PEPS norm in ComplexF32 storage (config 5's element type)
+    against the same network in ComplexF64: |d lnZ / N| <= 1e-6 and the same
+    edge sequence."""
+    catn, ctx = env
+    ts, ixs = _lattice(np.random.default_rng(400 + Lx * Ly + D), Lx, Ly, D, 2)
+    ts = [t.astype(np.complex64) for t in ts]
+    a = catn.braket_network(ts, ixs, ctx=ctx, chi=chi, loops=True)
+    b = catn.braket_network(ts, ixs, ctx=ctx, chi=chi, loops=True, dtype="f32")
+    la, ea, pa = catn.contraction(a)
+    lb, eb, pb = catn.contraction(b)
+    assert a.edge_log() == b.edge_log()
+    assert abs(la - lb) / (Lx * Ly) <= 1e-6
+    assert abs(pb - pa) < 1e-5
+    if chi >= 10_000:
+        assert abs(lb - math.log(B.exact_norm([t.astype(np.complex128) for t in ts], ixs))) < 1e-5
