@@ -33,11 +33,12 @@ def _c(rng, *shape):
 
 
 def _dense(ts, ixs, out=()):
+    abc = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"
     letters = {}
     for ix in ixs:
         for l in ix:
             if l not in letters:
-                letters[l] = chr(ord("a") + len(letters))
+                letters[l] = abc[len(letters)]
     spec = ",".join("".join(letters[l] for l in ix) for ix in ixs)
     return np.einsum(spec + "->" + "".join(letters[l] for l in out), *ts)
 
@@ -369,3 +370,83 @@ def test_device_complexf32_peps_norm(env, Lx, Ly, D, chi):
     assert abs(pb - pa) < 1e-5
     if chi >= 10_000:
         assert abs(lb - math.log(B.exact_norm([t.astype(np.complex128) for t in ts], ixs))) < 1e-5
+
+
+def _spec(ket, ket_ix):
+    """einsum arguments of the single ket layer with its physical legs open
+    (the amplitude tensor psi)"""
+    abc = "abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"
+    letters = {}
+    for ix in ket_ix:
+        for l in ix:
+            letters.setdefault(l, abc[len(letters)])
+    out = "".join(letters[l] for ix in ket_ix for l in ix if l.startswith("p"))
+    return [",".join("".join(letters[l] for l in ix) for ix in ket_ix) + "->" + out] + list(ket)
+
+
+def _peps_fused(rng, L, D, d):
+    """examples/peps_norm.jl:67-110: a random complex PEPS (physical leg first,
+    then N, S, W, E) and its double-layer network of E tensors,
+    E[(l1 l1'), ...] = sum_p A[p, l1, ...] conj(A[p, l1', ...]), each bond of
+    dimension D^2 with the ket index fastest."""
+    ket, ket_ix, E, E_ix = [], [], [], []
+    for r in range(1, L + 1):
+        for c in range(1, L + 1):
+            bonds = []
+            if r > 1:
+                bonds.append(f"v{r - 1}_{c}")
+            if r < L:
+                bonds.append(f"v{r}_{c}")
+            if c > 1:
+                bonds.append(f"h{r}_{c - 1}")
+            if c < L:
+                bonds.append(f"h{r}_{c}")
+            deg = len(bonds)
+            A = _c(rng, d, *([D] * deg)) / math.sqrt(2)
+            ket.append(A)
+            ket_ix.append([f"p{r}_{c}"] + bonds)
+            Am = A.reshape((d, D ** deg), order="F")
+            M2 = Am.T @ Am.conj()                                  # (alpha, beta)
+            T = M2.reshape([D] * (2 * deg), order="F")             # (l1..ldeg, l1'..ldeg')
+            perm = [x for k in range(deg) for x in (k, deg + k)]   # (l1, l1', l2, l2', ...)
+            E.append(T.transpose(perm).reshape([D * D] * deg, order="F"))
+            E_ix.append(bonds)
+    return ket, ket_ix, E, E_ix
+
+
+@pytest.mark.parametrize("L,D,Dmax,chi,dtype", [(3, 2, -1, 10_000, "f64"), (4, 2, 8, 16, "f64"),
+                                                 (4, 3, 16, 32, "f64"), (4, 3, 128, 512, "f64"),
+                                                 (4, 3, 128, 512, "f32")])
+def test_device_fused_peps_norm(env, L, D, Dmax, chi, dtype):
+    """Config 5's route (examples/peps_norm.jl:112-116): the fused double-layer
+    network of a random complex PEPS as a complex TensorNetwork with Dmax and
+    chi, at config 5's D = 3, Dmax = 128, chi = 512 on a 4 x 4 lattice (no
+    bond reaches Dmax or chi there) in ComplexF64 and ComplexF32 storage.
+    Untruncated runs match sum |psi|^2 of the single ket layer (the amplitude
+    oracle of the example, 1e-9; 1e-6 per site for ComplexF32) and give a
+    real positive value; truncated runs match the oracle's schedule, lnZ and
+    phase."""
+    catn, ctx = env
+    ket, ket_ix, E, E_ix = _peps_fused(np.random.default_rng(500 + 10 * L + D), L, D, 2)
+    kw = dict(Dmax=Dmax, chi=chi)
+    tn = catn.tensor_network(E, E_ix, ctx=ctx, dtype=dtype, **kw)
+    lnz, err, psi = catn.contraction(tn)
+    if dtype == "f32":
+        amp = np.einsum(*_spec(ket, ket_ix), optimize=True)
+        assert abs(lnz - math.log(float(np.sum(np.abs(amp) ** 2)))) / (L * L) <= 1e-6
+        assert abs(psi - 1.0) < 1e-6
+        return
+    if Dmax < 0 or Dmax >= 128:
+        assert err == 0.0
+        amp = np.einsum(*_spec(ket, ket_ix), optimize=True)
+        ref = float(np.sum(np.abs(amp) ** 2))
+        assert abs(math.exp(lnz) * psi - ref) <= 1e-9 * ref
+        assert abs(psi - 1.0) < 1e-9
+        return
+    ref = O.tensor_network(E, E_ix, **kw)
+    s = O.Schedule()
+    lo, eo, po = O.contraction(ref, s)
+    assert tn.edge_log() == s.edges
+    assert tn.cut_kept() == [e[2] for e in s.events if e[0] == "cut_keep"]
+    assert abs(lnz - lo) <= 1e-12 * max(1.0, abs(lo))
+    assert abs(psi - complex(po)) < 1e-9
