@@ -431,6 +431,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--config3-steps", type=int, default=1,
                     help="timed steps of config 3's full-size batched forward (0: skip)")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="only config 3's batched forward (no FP64 reference), for profiling")
     ap.add_argument("--cut-free-batch", type=int, default=128,
                     help="instances per chosen-form step of the cut-free comparison (0: skip it)")
     ap.add_argument("--cut-free-only", action="store_true",
@@ -487,6 +489,10 @@ def main():
     from oracle import catn_oracle as O
     catn = G.load_package()
     ctx = catn.Context(0)
+    if args.config3_only:
+        print(json.dumps(run_config3(catn, ctx, max(args.config3_steps, 1), reference=False)),
+              flush=True)
+        return
     if args.cut_free_only:
         print(json.dumps(run_cut_free(catn, ctx, args.steps, args.warmup, args.cut_free_batch)),
               flush=True)
@@ -645,7 +651,7 @@ def config3_instances(seed=2025, count=64):
     return out
 
 
-def run_config3(catn, ctx, steps):
+def run_config3(catn, ctx, steps, reference=True):
     """Config 3's batched forward: all 64 lattices in one batch per step with FP32
     storage (six-pass split-BF16 swap products), lnZ per instance. Networks are
     built (device-side sites) before each timed step. Accuracy: the FP64 form of
@@ -668,6 +674,18 @@ def run_config3(catn, ctx, steps):
         log(f"config 3 step {k}: {times[-1]:.2f} s (64 lattices, FP32 storage)")
         for t in nets:
             t.free()
+    out = {
+        "workload": "config 3: 64 open LxL +-J lattices, L uniform in {8,...,24}, h=0, beta "
+                    "uniform in [0.2,1.2], Dmax=32, chi=128, FP32 storage with six-pass split-BF16 "
+                    "swap GEMMs, one batch (one workgroup per lattice); forward lnZ only",
+        "seconds_per_step": float(np.mean(times)),
+        "lnz_solves_per_s": 64 / float(np.mean(times)),
+        "steps": steps,
+        "chosen_form": "refused by the device (reverse sweep across a Dmax cut)",
+    }
+    if not reference:
+        ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
+        return out
     small = [q for q, (n, *_) in enumerate(inst) if n <= 18 * 18]
     ref = catn.ising_network_batch([inst[q][:4] for q in small],
                                    [(k, inst[q][4]) for k, q in enumerate(small)], ctx=ctx,
@@ -677,17 +695,9 @@ def run_config3(catn, ctx, steps):
         t.free()
     ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
     ns = np.array([inst[q][0] for q in small], dtype=float)
-    return {
-        "workload": "config 3: 64 open LxL +-J lattices, L uniform in {8,...,24}, h=0, beta "
-                    "uniform in [0.2,1.2], Dmax=32, chi=128, FP32 storage with six-pass split-BF16 "
-                    "swap GEMMs, one batch (one workgroup per lattice); forward lnZ only",
-        "seconds_per_step": float(np.mean(times)),
-        "lnz_solves_per_s": 64 / float(np.mean(times)),
-        "steps": steps,
-        "max_abs_dlnZ_per_N_vs_fp64_same_engine": float(np.max(np.abs(lz[small] - l64) / ns)),
-        "fp64_reference_instances": len(small),
-        "chosen_form": "refused by the device (reverse sweep across a Dmax cut)",
-    }
+    out["max_abs_dlnZ_per_N_vs_fp64_same_engine"] = float(np.max(np.abs(lz[small] - l64) / ns))
+    out["fp64_reference_instances"] = len(small)
+    return out
 
 
 def _prefetch(nets):
