@@ -173,26 +173,30 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
     """Both forms on a spin glass whose contraction never reaches a Dmax cut
     and that chi=256 does not truncate (n=90 random 3-regular, seed 1,
     Dmax=-1; lnZ equal at chi 256 and 1024): the device runs the chosen
-    form's reverse sweep only on networks without a cut. Chosen: `batch`
-    copies of the instance by recorded forward + reverse sweep per step.
-    Plain: the 1 + 2n + 4|E| = 721 pinned contractions of one solve per step.
-    The adjoint differentiates the executed schedule; where a coupling
-    perturbation changes a kept count (a singular value that is exactly zero
-    at w = +-1 and is dropped by the absolute cutoff) it parts from the
-    pinning route, which differentiates the exact lnZ; such entries are
-    counted."""
+    form's reverse sweep only on networks without a cut.
+    Chosen: `batch` lifted solves per step, each a recorded forward + reverse
+    sweep at w + eta and at w - eta (catn.lifted_pair, eta = 1e-6 of the
+    couplings' scale) combined by catn.combine_lifted. The lift makes the
+    adjoint exact where a split of the +-J instance is exactly rank-deficient
+    (four couplings of this instance; the unlifted executed-schedule gradient
+    is also reported). Plain: the 1 + 2n + 4|E| = 721 pinned contractions of
+    one solve per step."""
     from oracle import configs as Cf
     from oracle import catn_oracle as O
     n, edges, w, h = Cf.spin_glass_3reg(CUT_FREE["n"], seed=CUT_FREE["seed"])
     kw = dict(Dmax=CUT_FREE["Dmax"], chi=CUT_FREE["chi"], arena_elems=3 << 20,
               scratch_elems=1 << 21)
+    (wp, hp), (wm, hm), eta = catn.lifted_pair(w, h, catn.LIFT["f64"])
     ct, pt = [], []
     g_adj = g_pin = lz_c = None
     for step in range(warmup + steps):
-        nets = catn.ising_network_batch([(n, edges, w, h)], [(0, BETA)] * batch, ctx=ctx, **kw)
+        nets = catn.ising_network_batch([(n, edges, wp, hp), (n, edges, wm, hm)],
+                                        [(0, BETA), (1, BETA)] * batch, ctx=ctx, **kw)
         t0 = time.perf_counter()
         lz, _, _ = catn.gradient_batch(nets)
-        grads = [catn.ising_gradient(t, len(edges), n)[:2] for t in nets]
+        runs = [(float(lz[q]),) + catn.ising_gradient(t, len(edges), n)[:2]
+                for q, t in enumerate(nets)]
+        solves = [catn.combine_lifted(runs[2 * k], runs[2 * k + 1], eta) for k in range(batch)]
         t1 = time.perf_counter()
         del nets
         pn = pinned_batch(catn, ctx, n, edges, w, h, kw)
@@ -201,12 +205,18 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
         gp = gradient_from_lnz(plz, n, edges)
         t3 = time.perf_counter()
         del pn
-        log(f"cut-free step {step}: chosen {t1 - t0:.3f} s ({batch} solves), "
+        log(f"cut-free step {step}: chosen {t1 - t0:.3f} s ({batch} lifted solves), "
             f"plain {t3 - t2:.3f} s (1 solve)")
         if step >= warmup:
             ct.append(t1 - t0)
             pt.append(t3 - t2)
-            g_adj, g_pin, lz_c = grads[0], gp, float(lz[0])
+            lz_c, dw_c, dh_c = solves[0]
+            g_adj, g_pin = (dw_c, dh_c), gp
+    # the executed-schedule gradient at w itself (one unlifted run, not timed)
+    un = catn.ising_network_batch([(n, edges, w, h)], [(0, BETA)], ctx=ctx, **kw)
+    catn.gradient_batch(un)
+    g_un = catn.ising_gradient(un[0], len(edges), n)[:2]
+    del un
     ref = O.contraction(O.ising_network(n, edges, w, h, BETA, Dmax=CUT_FREE["Dmax"],
                                         chi=CUT_FREE["chi"]))[0]
     cv = batch * len(ct) / sum(ct)
@@ -215,14 +225,18 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
         "workload": f"spin glass n={n} random 3-regular (seed {CUT_FREE['seed']}), w=+-1, "
                     f"h~N(0,0.1^2), beta=1.0, Dmax=-1 (no cut), chi={CUT_FREE['chi']}, FP64",
         "chosen_solves_per_s": cv, "chosen_batch": batch,
+        "chosen_networks_per_solve": 2,
         "plain_solves_per_s": pv, "chosen_over_plain": cv / pv,
         "max_abs_dlnZ_per_N_vs_fp64_oracle": abs(lz_c - ref) / n,
         "adjoint_vs_pinning_max_abs_dw": float(np.max(np.abs(g_adj[0] - g_pin[0]))),
         "adjoint_vs_pinning_max_abs_dh": float(np.max(np.abs(g_adj[1] - g_pin[1]))),
-        # entries where the two routes part: edges whose coupling perturbation
-        # changes a kept count (an exactly zero singular value at w = +-1)
         "adjoint_vs_pinning_entries_over_1e-9": int(np.sum(np.abs(g_adj[0] - g_pin[0]) > 1e-9)
                                                      + np.sum(np.abs(g_adj[1] - g_pin[1]) > 1e-9)),
+        # without the lift: the executed schedule's derivative at w, which parts
+        # from the exact one on the couplings of an exactly rank-deficient split
+        "unlifted_vs_pinning_max_abs_dw": float(np.max(np.abs(g_un[0] - g_pin[0]))),
+        "unlifted_vs_pinning_entries_over_1e-9": int(np.sum(np.abs(g_un[0] - g_pin[0]) > 1e-9)
+                                                      + np.sum(np.abs(g_un[1] - g_pin[1]) > 1e-9)),
         "gradient_entries": int(len(g_adj[0]) + len(g_adj[1])),
     }
 

@@ -14,6 +14,7 @@ over the C ABI of `libcatn_hip.so` (include/catn_hip.h):
   gradient_batch([tn...])                   lnZ + hand-written adjoint per network
   ising_gradient(tn) -> (dw, dh, dbeta)     d lnZ / d(w, h, beta) after the sweep
   lnz_and_grad(n, edges, w, h, beta; ...)   one ising solve: lnZ and its gradient
+                                            (lifted: exact at rank-deficient splits)
   tsvd(A; cutoff, maxdim)                   src/linalg_utils.jl:15-28
   MPSNet: hand-built MPSNode primitives      src/mps_node.jl (swap!, cano_to!, ...)
 
@@ -38,7 +39,8 @@ __all__ = [
     "result_tensor",
     "contraction_batch", "free_energy", "magnetization", "correlation", "tsvd",
     "default_context", "LIB_PATH", "CatnError", "gradient_batch", "ising_gradient",
-    "lnz_and_grad", "lnz_and_grad_batch", "rsvd",
+    "lnz_and_grad", "lnz_and_grad_batch", "rsvd", "lift_direction", "lifted_pair",
+    "combine_lifted", "richardson", "lift_plan", "LIFT",
 ]
 
 
@@ -705,22 +707,126 @@ def ising_gradient(tn: TensorNetwork, nedges: int, n: int):
     return dw, dh, db.value
 
 
-def lnz_and_grad_batch(instances, *, ctx=None, tape_cap: int = 0, **kw):
+# Default lift per storage precision. A lifted singular value is of order
+# eta times an instance-dependent factor (down to about 1e-2 on +-J spin
+# glasses), and must clear the storage's noise floor (2^-22 of the largest
+# for six-pass FP32, 2^-18 for three-pass) by a margin: FP64 takes one
+# symmetric pair at 1e-6 (bias ~1e-12); FP32 storage takes pairs at eta and
+# 2 eta combined by a Richardson step, which cancels the O(eta^2) bias of the
+# larger lift.
+LIFT = {"f64": 1e-6, "f32x6": 2e-3, "f32": 1e-2}
+
+
+def lift_direction(nedges: int, n: int, seed: int = 0x5EED):
+    """The fixed perturbation direction of the lifted gradient: one value in
+    [-1, 1) per coupling, then one per field, from splitmix64 of the index
+    (the same sequence in every host binding)."""
+    m = (1 << 64) - 1
+    out = np.empty(nedges + n)
+    for i in range(nedges + n):
+        z = (seed + (i + 1) * 0x9E3779B97F4A7C15) & m
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        z ^= z >> 31
+        out[i] = (z >> 11) * 2.0 ** -52 - 1.0
+    return out[:nedges], out[nedges:]
+
+
+def lifted_pair(weights, fields, lift: float):
+    """(w + eta_w, h + eta_h), (w - eta_w, h - eta_h) and (eta_w, eta_h), with
+    eta = lift * max(1, |w|_inf, |h|_inf) * lift_direction."""
+    w = np.asarray(weights, float).reshape(-1)
+    h = np.asarray(fields, float).reshape(-1)
+    gw, gh = lift_direction(len(w), len(h))
+    s = lift * max(1.0, float(np.max(np.abs(w), initial=0.0)), float(np.max(np.abs(h), initial=0.0)))
+    ew, eh = s * gw, s * gh
+    return (w + ew, h + eh), (w - ew, h - eh), (ew, eh)
+
+
+def combine_lifted(plus, minus, eta):
+    """lnZ and gradient at w from the two runs at w +- eta:
+    grad = (g+ + g-)/2 and lnZ = (l+ + l-)/2 - eta.(g+ - g-)/4, both exact to
+    O(|eta|^2) (g+ - g- = 2 H eta + O(|eta|^3) removes the curvature term of
+    the average)."""
+    (lp, dwp, dhp), (lm, dwm, dhm), (ew, eh) = plus, minus, eta
+    lz = 0.5 * (lp + lm) - 0.25 * (float(ew @ (dwp - dwm)) + float(eh @ (dhp - dhm)))
+    return lz, 0.5 * (dwp + dwm), 0.5 * (dhp + dhm)
+
+
+def lift_plan(lift: Optional[float], dtype: str = "f64"):
+    """The lifts of one gradient solve: () for lift = 0, (eta,) for FP64
+    storage, (eta, 2 eta) for FP32 storage (Richardson step)."""
+    if lift is None:
+        lift = LIFT.get(dtype, LIFT["f64"])
+    if not lift >= 0:
+        raise CatnError(f"lift must be >= 0, got {lift!r}")
+    if lift == 0:
+        return ()
+    return (lift,) if dtype == "f64" else (lift, 2.0 * lift)
+
+
+def richardson(results):
+    """Combine the lifted solves at eta and 2 eta (each exact to
+    c eta^2 + O(eta^4)): gradient (4 g(eta) - g(2 eta)) / 3; lnZ from the
+    smaller lift, whose curvature correction already leaves O(eta^4)."""
+    if len(results) == 1:
+        return results[0]
+    (l1, w1, h1), (_, w2, h2) = results
+    return l1, (4.0 * w1 - w2) / 3.0, (4.0 * h1 - h2) / 3.0
+
+
+def lnz_and_grad_batch(instances, *, ctx=None, tape_cap: int = 0,
+                       lift: Optional[float] = None, **kw):
     """Many ising instances [(n, edges, w, h, beta), ...] (any sizes and betas)
-    in one forward + reverse sweep: [(lnZ, dlnZ/dw, dlnZ/dh), ...]."""
+    in one forward + reverse sweep: [(lnZ, dlnZ/dw, dlnZ/dh), ...].
+
+    The reverse sweep differentiates the executed schedule, with every split's
+    kept count held fixed. At an exactly rank-deficient split (a singular value
+    that is zero at w, such as a frustrated cycle of +-J couplings gives, and
+    that the cutoff drops) a perturbation of w restores that singular value,
+    and the exact lnZ has a derivative that the schedule at w does not carry.
+    With a lift every instance is run at w +- eta (lifted_pair), where a
+    generic eta lifts such zeros above the cutoff so that the schedule is
+    smooth there; the average gradient is then the exact one to O(|eta|^2)
+    (combine_lifted), and for FP32 storage a second pair at 2 eta removes that
+    term (richardson). All runs share the batch. lift=None takes the default
+    of the storage precision (LIFT); lift=0 gives the executed schedule's
+    derivative at w from one run per instance."""
     ctx = ctx or default_context()
-    nets = [ising_network(n, e, w, h, b, ctx=ctx, **kw) for (n, e, w, h, b) in instances]
+    lifts = lift_plan(lift, kw.get("dtype", "f64"))
+    runs = []      # per instance: (n, edges, [(w, h) per network], [eta per pair])
+    for (n, e, w, h, b) in instances:
+        if not lifts:
+            runs.append((n, e, b, [(w, h)], []))
+            continue
+        pts, etas = [], []
+        for s in lifts:
+            p, m, eta = lifted_pair(w, h, s)
+            pts += [p, m]
+            etas.append(eta)
+        runs.append((n, e, b, pts, etas))
+    nets = [ising_network(n, e, pw, ph, b, ctx=ctx, **kw)
+            for (n, e, b, pts, _) in runs for (pw, ph) in pts]
     lz, _, _ = gradient_batch(nets, tape_cap)
-    out = []
-    for t, (n, e, w, h, b), l in zip(nets, instances, lz):
-        dw, dh, _ = ising_gradient(t, len(e), n)
-        out.append((float(l), dw, dh))
+    out, q = [], 0
+    for (n, e, b, pts, etas) in runs:
+        res = []
+        for _ in pts:
+            dw, dh, _ = ising_gradient(nets[q], len(e), n)
+            res.append((float(lz[q]), dw, dh))
+            q += 1
+        if not etas:
+            out.append(res[0])
+            continue
+        pairs = [combine_lifted(res[2 * k], res[2 * k + 1], etas[k]) for k in range(len(etas))]
+        out.append(richardson(pairs))
     return out
 
 
 def lnz_and_grad(n, edges, weights, fields, beta, *, ctx=None, **kw):
-    """lnZ and (d lnZ/d w, d lnZ/d h) of one ising_network by one forward
-    contraction and one reverse sweep."""
+    """lnZ and (d lnZ/d w, d lnZ/d h) of one ising_network by forward
+    contraction and reverse sweep (lifted runs unless lift=0; see
+    lnz_and_grad_batch)."""
     (l, dw, dh), = lnz_and_grad_batch([(n, edges, weights, fields, beta)], ctx=ctx, **kw)
     return l, dw, dh
 

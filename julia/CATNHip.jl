@@ -200,19 +200,85 @@ function lnz_and_grad!(nets::Vector{HipNetwork})
     end
 end
 
+# ---------------------------------------------------------------------------
+# Lifted gradient. The reverse sweep differentiates the executed schedule (the
+# kept count of every split held fixed). A frustrated cycle of +-J couplings can
+# make a split exactly rank-deficient at w: the cutoff drops a zero singular
+# value that any perturbation of w restores, and the exact lnZ then has a
+# derivative the schedule at w does not carry. Each instance is run at
+# w +- eta for a fixed generic eta (splitmix64 of the index, the sequence of the
+# Python mirror's lift_direction), which lifts such zeros above the cutoff; the
+# average is the exact gradient to O(|eta|^2). FP32 storage needs a larger lift
+# to clear its noise floor and adds a pair at 2 eta (Richardson step).
+const LIFT = Dict(0 => 1e-6, 2 => 2e-3, 1 => 1e-2)     # by precision level
+
+function splitmix(seed::UInt64, i::Integer)
+    z = seed + UInt64(i) * 0x9E3779B97F4A7C15         # UInt64 arithmetic wraps
+    z = (z ⊻ (z >> 30)) * 0xBF58476D1CE4E5B9
+    z = (z ⊻ (z >> 27)) * 0x94D049BB133111EB
+    z ⊻ (z >> 31)
+end
+lift_direction(ne, n; seed=0x5EED) =
+    [Float64(splitmix(UInt64(seed), i) >> 11) * 2.0^-52 - 1.0 for i in 1:(ne + n)]
+
+# [(lnZ, dw, dh, dβ) per instance] for instances (n, edges, w, h, β); lift =
+# nothing takes the precision's default, 0 the executed schedule's derivative
+function lifted_grads(instances; lift=nothing, kw...)
+    prec = Int(params(; kw...).precision)
+    s0 = lift === nothing ? LIFT[prec] : Float64(lift)
+    s0 >= 0 || error("lift must be >= 0, got $lift")
+    s0 == 0 && return lnz_and_grad!([ising_network(i...; kw...) for i in instances])
+    lifts = prec == 0 ? [s0] : [s0, 2s0]
+    nets, etas = HipNetwork[], Tuple{Vector{Float64},Vector{Float64}}[]
+    for (n, edges, w, h, β) in instances
+        w, h = Float64.(collect(w)), Float64.(collect(h))
+        g = lift_direction(length(edges), n)
+        sc = max(1.0, maximum(abs, w; init=0.0), maximum(abs, h; init=0.0))
+        for s in lifts
+            ew, eh = (s * sc) .* g[1:length(edges)], (s * sc) .* g[length(edges)+1:end]
+            push!(nets, ising_network(n, edges, w .+ ew, h .+ eh, β; kw...))
+            push!(nets, ising_network(n, edges, w .- ew, h .- eh, β; kw...))
+            push!(etas, (ew, eh))
+        end
+    end
+    res = lnz_and_grad!(nets)
+    out = Tuple{Float64,Vector{Float64},Vector{Float64},Float64}[]
+    q = 0
+    for _ in instances
+        pairs = []
+        for _ in lifts
+            (lp, wp, hp, bp), (lm, wm, hm, bm) = res[q + 1], res[q + 2]
+            ew, eh = etas[q ÷ 2 + 1]
+            q += 2
+            lz = (lp + lm) / 2 - (sum(ew .* (wp .- wm)) + sum(eh .* (hp .- hm))) / 4
+            push!(pairs, (lz, (wp .+ wm) ./ 2, (hp .+ hm) ./ 2, (bp + bm) / 2))
+        end
+        if length(pairs) == 1
+            push!(out, pairs[1])
+        else
+            a, b = pairs
+            push!(out, (a[1], (4 .* a[2] .- b[2]) ./ 3, (4 .* a[3] .- b[3]) ./ 3, (4a[4] - b[4]) / 3))
+        end
+    end
+    out
+end
+
 """
-    ising_lnZ(n, edges, w, h, β; kw...) -> lnZ
+    ising_lnZ(n, edges, w, h, β; lift=nothing, kw...) -> lnZ
 
 lnZ of ising_network(n, edges, w, h, β; kw...) by contraction! on the device.
 Differentiable: the rrule's pullback is the library's reverse sweep, giving
-d lnZ/d w = β<s_i s_j>, d lnZ/d h = β<s_i> and d lnZ/d β = <E> at the executed
-schedule. The library refuses a gradient run whose schedule reaches a Dmax cut
-on the gfx950 device (an error is raised); use Dmax = -1 or the pinning route.
+d lnZ/d w = β<s_i s_j>, d lnZ/d h = β<s_i> and d lnZ/d β = <E>, lifted (see
+lifted_grads) so that exactly rank-deficient splits of ±J instances get the
+exact derivative. The library refuses a gradient run whose schedule reaches a
+Dmax cut on the gfx950 device (an error is raised); use Dmax = -1 or the
+pinning route.
 """
-ising_lnZ(n, edges, w, h, β; kw...) = contraction!(ising_network(n, edges, w, h, β; kw...))[1]
+ising_lnZ(n, edges, w, h, β; lift=nothing, kw...) =
+    contraction!(ising_network(n, edges, w, h, β; kw...))[1]
 
-function ChainRulesCore.rrule(::typeof(ising_lnZ), n, edges, w, h, β; kw...)
-    lz, dw, dh, db = lnz_and_grad!([ising_network(n, edges, w, h, β; kw...)])[1]
+function ChainRulesCore.rrule(::typeof(ising_lnZ), n, edges, w, h, β; lift=nothing, kw...)
+    lz, dw, dh, db = lifted_grads([(n, edges, w, h, β)]; lift=lift, kw...)[1]
     function ising_lnZ_pullback(Δ)
         δ = unthunk(Δ)
         (NoTangent(), NoTangent(), NoTangent(), δ .* dw, δ .* dh, δ * db)
@@ -221,16 +287,16 @@ function ChainRulesCore.rrule(::typeof(ising_lnZ), n, edges, w, h, β; kw...)
 end
 
 """
-    ising_lnZ_batch(instances; kw...) -> Vector of lnZ
+    ising_lnZ_batch(instances; lift=nothing, kw...) -> Vector of lnZ
 
 Batched form: `instances` is a vector of (n, edges, w, h, β); every instance's
-forward and reverse sweep run in the same launches.
+forward and reverse sweep (both lifted runs) run in the same launches.
 """
-ising_lnZ_batch(instances; kw...) =
+ising_lnZ_batch(instances; lift=nothing, kw...) =
     first.(contraction!([ising_network(i...; kw...) for i in instances]))
 
-function ChainRulesCore.rrule(::typeof(ising_lnZ_batch), instances; kw...)
-    res = lnz_and_grad!([ising_network(i...; kw...) for i in instances])
+function ChainRulesCore.rrule(::typeof(ising_lnZ_batch), instances; lift=nothing, kw...)
+    res = lifted_grads(instances; lift=lift, kw...)
     function ising_lnZ_batch_pullback(Δ)
         δ = unthunk(Δ)
         grads = [Tangent{Tuple}(NoTangent(), NoTangent(), δ[q] .* r[2], δ[q] .* r[3], δ[q] * r[4])
@@ -243,9 +309,9 @@ end
 # magnetization / correlation (src/ising.jl:263-339). method = :pinning runs the
 # reference's 1 + 2n / 1 + 4|E| contractions in one batch; method = :adjoint uses
 # one reverse sweep (exact isolated-spin values, tanh(β h)).
-function magnetization(n, edges, w, h, β; method=:pinning, kw...)
+function magnetization(n, edges, w, h, β; method=:pinning, lift=nothing, kw...)
     if method === :adjoint
-        _, _, dh, _ = lnz_and_grad!([ising_network(n, edges, w, h, β; kw...)])[1]
+        _, _, dh, _ = lifted_grads([(n, edges, w, h, β)]; lift=lift, kw...)[1]
         return dh ./ β
     end
     nets = [ising_network(n, edges, w, h, β; kw...)]
@@ -257,9 +323,9 @@ function magnetization(n, edges, w, h, β; method=:pinning, kw...)
     [exp(lz[2k + 1] - lz[1]) - exp(lz[2k] - lz[1]) for k in 1:n]
 end
 
-function correlation(n, edges, w, h, β; method=:pinning, kw...)
+function correlation(n, edges, w, h, β; method=:pinning, lift=nothing, kw...)
     if method === :adjoint
-        _, dw, _, _ = lnz_and_grad!([ising_network(n, edges, w, h, β; kw...)])[1]
+        _, dw, _, _ = lifted_grads([(n, edges, w, h, β)]; lift=lift, kw...)[1]
         return dw ./ β
     end
     nets = [ising_network(n, edges, w, h, β; kw...)]
