@@ -189,9 +189,22 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
     (wp, hp), (wm, hm), eta = catn.lifted_pair(w, h, catn.LIFT["f64"])
     ct, pt = [], []
     g_adj = g_pin = lz_c = None
-    for step in range(warmup + steps):
-        nets = catn.ising_network_batch([(n, edges, wp, hp), (n, edges, wm, hm)],
+    # each form at its own launch shape: the chosen batch (2 * batch networks)
+    # at the shape the autotuner picks for it, the plain batch at the shape
+    # already set on the context (the autotuned one in the default run)
+    plain_shape = ctx.launch()
+
+    def chosen_batch():
+        return catn.ising_network_batch([(n, edges, wp, hp), (n, edges, wm, hm)],
                                         [(0, BETA), (1, BETA)] * batch, ctx=ctx, **kw)
+    tuned = catn.autotune(ctx, chosen_batch, run=lambda nets: catn.gradient_batch(nets)[0])
+    chosen_shape = (tuned["threads"], tuned["lds_doubles"], tuned["panel"], tuned["jacobi_lanes"])
+    log("cut-free chosen autotune: " + ", ".join(
+        f"({r['threads']},{r['lds_doubles']},{r['panel']},{r['jacobi_lanes']}) {r['seconds']:.3f} s"
+        for r in tuned["table"]) + f" -> {chosen_shape}")
+    for step in range(warmup + steps):
+        nets = chosen_batch()
+        ctx.set_launch(*chosen_shape)
         t0 = time.perf_counter()
         lz, _, _ = catn.gradient_batch(nets)
         runs = [(float(lz[q]),) + catn.ising_gradient(t, len(edges), n)[:2]
@@ -199,6 +212,7 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
         solves = [catn.combine_lifted(runs[2 * k], runs[2 * k + 1], eta) for k in range(batch)]
         t1 = time.perf_counter()
         del nets
+        ctx.set_launch(plain_shape[0], plain_shape[1], plain_shape[3], plain_shape[4])
         pn = pinned_batch(catn, ctx, n, edges, w, h, kw)
         t2 = time.perf_counter()
         plz, _, _ = catn.contraction_batch(pn)
@@ -213,19 +227,28 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
             lz_c, dw_c, dh_c = solves[0]
             g_adj, g_pin = (dw_c, dh_c), gp
     # the executed-schedule gradient at w itself (one unlifted run, not timed)
+    ctx.set_launch(*chosen_shape)
     un = catn.ising_network_batch([(n, edges, w, h)], [(0, BETA)], ctx=ctx, **kw)
     catn.gradient_batch(un)
     g_un = catn.ising_gradient(un[0], len(edges), n)[:2]
     del un
+    ctx.set_launch(plain_shape[0], plain_shape[1], plain_shape[3], plain_shape[4])
     ref = O.contraction(O.ising_network(n, edges, w, h, BETA, Dmax=CUT_FREE["Dmax"],
                                         chi=CUT_FREE["chi"]))[0]
-    cv = batch * len(ct) / sum(ct)
-    pv = len(pt) / sum(pt)
+    # rates from the median step: a step of either form can stall on device
+    # memory allocation for its networks' tapes (seen once: 0.61 s against
+    # 0.16 s), and the per-step times are reported beside the rates
+    cv = batch / float(np.median(ct))
+    pv = 1.0 / float(np.median(pt))
     return {
         "workload": f"spin glass n={n} random 3-regular (seed {CUT_FREE['seed']}), w=+-1, "
                     f"h~N(0,0.1^2), beta=1.0, Dmax=-1 (no cut), chi={CUT_FREE['chi']}, FP64",
         "chosen_solves_per_s": cv, "chosen_batch": batch,
+        "chosen_step_seconds": [round(t, 4) for t in ct],
+        "plain_step_seconds": [round(t, 4) for t in pt],
         "chosen_networks_per_solve": 2,
+        "chosen_launch_shape": list(chosen_shape),
+        "plain_launch_shape": list(plain_shape[:2]) + list(plain_shape[3:]),
         "plain_solves_per_s": pv, "chosen_over_plain": cv / pv,
         "max_abs_dlnZ_per_N_vs_fp64_oracle": abs(lz_c - ref) / n,
         "adjoint_vs_pinning_max_abs_dw": float(np.max(np.abs(g_adj[0] - g_pin[0]))),
@@ -543,7 +566,7 @@ def main():
                           min(args.warmup, 1), args.op_profile, not args.no_autotune)
 
     if refused and args.form == "both" and args.cut_free_batch > 0:
-        out_cfg["cut_free_comparison"] = run_cut_free(catn, ctx, 2, 1, args.cut_free_batch)
+        out_cfg["cut_free_comparison"] = run_cut_free(catn, ctx, 3, 1, args.cut_free_batch)
     if args.form == "both" and args.config3_steps > 0:
         out_cfg["config3_forward"] = run_config3(catn, ctx, args.config3_steps)
 
@@ -675,7 +698,10 @@ def run_config3(catn, ctx, steps, reference=True):
     inst = config3_instances()
     kw = dict(Dmax=32, chi=128, arena_elems=1 << 26, scratch_elems=1 << 26)
     saved = ctx.launch()
-    ctx.set_launch(256, 0, 16, 16)        # automatic LDS budget for a 64-network batch
+    # wide workgroups, one per CU (1024 lanes, 20352 LDS doubles): the batch's
+    # time is set by its widest lattices' large swaps, which run 2.1x faster
+    # on 16 wavefronts than on 4 (26.7 s -> 12.6 s; 512 lanes: 16.5 s)
+    ctx.set_launch(1024, 20352, 16, 16)
     times, lz = [], None
     for k in range(steps):
         nets = catn.ising_network_batch([(n, e, w, h) for (n, e, w, h, _) in inst],
@@ -691,7 +717,7 @@ def run_config3(catn, ctx, steps, reference=True):
     out = {
         "workload": "config 3: 64 open LxL +-J lattices, L uniform in {8,...,24}, h=0, beta "
                     "uniform in [0.2,1.2], Dmax=32, chi=128, FP32 storage with six-pass split-BF16 "
-                    "swap GEMMs, one batch (one workgroup per lattice); forward lnZ only",
+                    "swap GEMMs, one batch (one 1024-lane workgroup per lattice); forward lnZ only",
         "seconds_per_step": float(np.mean(times)),
         "lnz_solves_per_s": 64 / float(np.mean(times)),
         "steps": steps,

@@ -40,7 +40,7 @@ __all__ = [
     "contraction_batch", "free_energy", "magnetization", "correlation", "tsvd",
     "default_context", "LIB_PATH", "CatnError", "gradient_batch", "ising_gradient",
     "lnz_and_grad", "lnz_and_grad_batch", "rsvd", "lift_direction", "lifted_pair",
-    "combine_lifted", "richardson", "lift_plan", "LIFT",
+    "combine_lifted", "richardson", "lift_plan", "LIFT", "autotune", "AUTOTUNE_WIDE",
 ]
 
 
@@ -205,7 +205,7 @@ class Context:
         return a.value, b.value
 
     def set_threads(self, threads: int):
-        """Workgroup size of the interpreter kernel (64, 128 or 256)."""
+        """Workgroup size of the interpreter kernel (64, 128, 256, 512 or 1024)."""
         self._check(self.lib.catn_set_threads(self.h, int(threads)))
 
     def set_launch(self, threads: int = 256, lds_doubles: int = 0, panel: int = 0,
@@ -642,19 +642,26 @@ def gradient_batch(tns: Sequence[TensorNetwork], tape_cap: int = 0):
 # let 1, 2, 3 and 4 workgroups share one CU's 160 KB of LDS.
 AUTOTUNE_SHAPES = ((256, 0), (256, 5056), (256, 6784), (256, 10176), (256, 20352),
                    (128, 5056), (128, 10176))
+# wide workgroups (one per CU): tried when the batch has at most WIDE_MAX_BATCH
+# networks. They pay where single ops are large (config 3's 64 lattices:
+# 26.7 s at 256 lanes, 16.5 s at 512, 12.6 s at 1024) and cost 10-40% on
+# networks of small ops.
+AUTOTUNE_WIDE = ((512, 20352), (1024, 20352))
+WIDE_MAX_BATCH = 256    # the MI355X's CUs
 # tile parameters tried at the fastest shape: (Householder panel width,
 # global-Jacobi lanes per column pair)
 AUTOTUNE_TILES = ((16, 16), (32, 16), (8, 16), (16, 32), (16, 8), (16, 64))
 
 
 def autotune(ctx: Context, build, run=None, shapes=AUTOTUNE_SHAPES, tiles=AUTOTUNE_TILES,
-             repeats=1, rtol=1e-10):
+             repeats=1, rtol=1e-10, wide=AUTOTUNE_WIDE):
     """Pick the interpreter's launch shape for a workload by timing it.
 
     build() returns a fresh batch of networks for ctx (a run consumes its
     batch); run(nets) executes one (default contraction_batch) and returns
     lnZ per network. Stage 1 times each (lanes, LDS budget) shape within the
-    device's LDS limit at the default tiles; stage 2 times each (panel width,
+    device's LDS limit at the default tiles, and the `wide` shapes when the
+    batch has at most WIDE_MAX_BATCH networks; stage 2 times each (panel width,
     Jacobi lanes) tile pair at the fastest shape. Times are best of
     `repeats`. A candidate counts only if its lnZ agree with the first
     candidate's to rtol (shape and tiles change rounding, never the
@@ -662,13 +669,14 @@ def autotune(ctx: Context, build, run=None, shapes=AUTOTUNE_SHAPES, tiles=AUTOTU
     "lds_doubles", "panel", "jacobi_lanes", "seconds", "table": [...]}."""
     run = run or (lambda nets: contraction_batch(nets)[0])
     limit = ctx.launch()[2]
-    table, ref = [], [None]
+    table, ref, size = [], [None], [0]
 
     def timed(threads, lds, panel, jl):
         ctx.set_launch(threads, lds, panel, jl)
         best, lz = math.inf, None
         for _ in range(max(1, int(repeats))):
             nets = build()
+            size[0] = len(nets)
             t0 = time.perf_counter()
             lz = np.asarray(run(nets), dtype=float)
             best = min(best, time.perf_counter() - t0)
@@ -686,6 +694,10 @@ def autotune(ctx: Context, build, run=None, shapes=AUTOTUNE_SHAPES, tiles=AUTOTU
     for threads, lds in shapes:
         if lds <= limit:
             timed(threads, lds, 16, 16)
+    if size[0] <= WIDE_MAX_BATCH:
+        for threads, lds in wide:
+            if lds <= limit:
+                timed(threads, lds, 16, 16)
     ok = [r for r in table if r["lnz_rel_dev"] <= rtol]
     if not ok:
         raise CatnError("autotune: no launch shape ran")

@@ -105,7 +105,8 @@ class Executor {
   double bwd_seconds = 0.0;
   // host-side time: planning, host<->device copies + launch overhead
   double plan_seconds = 0.0, run_seconds = 0.0;
-  // workgroup size of the interpreter (64, 128 or 256 lanes)
+  // workgroup size of the interpreter (64, 128 or 256 lanes; 512 or 1024 run
+  // the wide kernels, one workgroup per CU)
   int threads = 256;
   // LDS budget per workgroup in doubles; 0 = lds_doubles_for_batch(batch)
   int lds_override = 0;
@@ -1396,17 +1397,21 @@ int catn_host_stats(catn_ctx* ctx, double* plan_seconds, double* run_seconds) {
   return 0;
 }
 
+static bool lanes_ok(int32_t t) {
+  return t == 64 || t == 128 || t == 256 || t == 512 || t == 1024;
+}
+
 int catn_set_threads(catn_ctx* ctx, int32_t threads) {
-  if (threads != 64 && threads != 128 && threads != 256)
-    return catn::fail(-2, "catn_set_threads: threads must be 64, 128 or 256");
+  if (!lanes_ok(threads))
+    return catn::fail(-2, "catn_set_threads: threads must be 64, 128, 256, 512 or 1024");
   ctx->ex->threads = threads;
   return 0;
 }
 
 int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles, int32_t panel,
                     int32_t jacobi_lanes) {
-  if (threads != 64 && threads != 128 && threads != 256)
-    return catn::fail(-2, "catn_set_launch: threads must be 64, 128 or 256");
+  if (!lanes_ok(threads))
+    return catn::fail(-2, "catn_set_launch: threads must be 64, 128, 256, 512 or 1024");
   const int lim = ctx->ex->lds_limit();
   if (lds_doubles != 0 && (lds_doubles < catn::kLdsMinDoubles || lds_doubles > lim))
     return catn::fail(-2, "catn_set_launch: lds_doubles must be 0 (automatic) or in [" +

@@ -28,22 +28,24 @@ static __device__ __align__(256) double scratch[kFixedDoubles];
 namespace catn {
 
 constexpr int kThreads = 256;
+constexpr int kWideThreads = 1024;
 constexpr int kMaxDevices = 64;
 // device address of `scratch` per device (null until the device's first executor)
 static void* g_fixed_ptr[kMaxDevices] = {};
-// static LDS of both kernels: red[16] doubles + flag, padded
-constexpr int kStaticLdsBytes = 256;
+// static LDS of the kernels: red[32] doubles + flag, padded
+constexpr int kStaticLdsBytes = 320;
 
-__global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, const Op* ops,
-                                                             const int* ranges, int lds_cap,
-                                                             int qb, int jac_lanes) {
-  __shared__ double red[16];
-  __shared__ int flag;
+// One network per workgroup: workgroup b runs program range b of `ops` on
+// instance b. `red` holds 32 reduction doubles (two halves of one slot per
+// wavefront, up to 16 wavefronts).
+__device__ __forceinline__ void programs_wg(InstState* states, const Op* ops, const int* ranges,
+                                            int lds_cap, int qb, int jac_lanes, double* red,
+                                            int* flag) {
   WG g;
   g.tid = threadIdx.x;
   g.nthr = blockDim.x;
   g.red = red;
-  g.flag = &flag;
+  g.flag = flag;
   g.lds_cap = lds_cap;
   g.qb = qb;
   g.jac_lanes = jac_lanes;
@@ -54,21 +56,56 @@ __global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, 
 }
 
 // Reverse sweep: workgroup b walks the tape of instance b backwards.
-__global__ __launch_bounds__(kThreads, 4) void exec_backward(InstState* states, int lds_cap, int qb,
-                                                             int jac_lanes) {
-  __shared__ double red[16];
-  __shared__ int flag;
+__device__ __forceinline__ void backward_wg(InstState* states, int lds_cap, int qb, int jac_lanes,
+                                            double* red, int* flag) {
   WG g;
   g.tid = threadIdx.x;
   g.nthr = blockDim.x;
   g.red = red;
-  g.flag = &flag;
+  g.flag = flag;
   g.lds_cap = lds_cap;
   g.qb = qb;
   g.jac_lanes = jac_lanes;
   InstState* st = states + blockIdx.x;
   g.sub = st->sub_ticks;
   run_backward(g, st);
+}
+
+// Up to 256 lanes per workgroup, several workgroups per CU.
+__global__ __launch_bounds__(kThreads, 4) void exec_programs(InstState* states, const Op* ops,
+                                                             const int* ranges, int lds_cap,
+                                                             int qb, int jac_lanes) {
+  __shared__ double red[32];
+  __shared__ int flag;
+  programs_wg(states, ops, ranges, lds_cap, qb, jac_lanes, red, &flag);
+}
+
+__global__ __launch_bounds__(kThreads, 4) void exec_backward(InstState* states, int lds_cap, int qb,
+                                                             int jac_lanes) {
+  __shared__ double red[32];
+  __shared__ int flag;
+  backward_wg(states, lds_cap, qb, jac_lanes, red, &flag);
+}
+
+// Wide workgroups (512 or 1024 lanes, up to 16 wavefronts, 4 per SIMD at
+// 1024): for batches of fewer networks than CUs, where each network's ops get
+// a whole CU.
+__global__ __launch_bounds__(kWideThreads, 1) void exec_programs_wide(InstState* states,
+                                                                      const Op* ops,
+                                                                      const int* ranges,
+                                                                      int lds_cap, int qb,
+                                                                      int jac_lanes) {
+  __shared__ double red[32];
+  __shared__ int flag;
+  programs_wg(states, ops, ranges, lds_cap, qb, jac_lanes, red, &flag);
+}
+
+__global__ __launch_bounds__(kWideThreads, 1) void exec_backward_wide(InstState* states,
+                                                                      int lds_cap, int qb,
+                                                                      int jac_lanes) {
+  __shared__ double red[32];
+  __shared__ int flag;
+  backward_wg(states, lds_cap, qb, jac_lanes, red, &flag);
 }
 
 // Device builder of ising_network sites: block (x, y) handles sites
@@ -127,6 +164,14 @@ class HipExecutor : public Executor {
                                  (int)(sizeof(double) * lds_limit_)),
              "hipFuncSetAttribute");
     hipcheck(hipFuncSetAttribute((const void*)exec_backward,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * lds_limit_)),
+             "hipFuncSetAttribute");
+    hipcheck(hipFuncSetAttribute((const void*)exec_programs_wide,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * lds_limit_)),
+             "hipFuncSetAttribute");
+    hipcheck(hipFuncSetAttribute((const void*)exec_backward_wide,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(sizeof(double) * lds_limit_)),
              "hipFuncSetAttribute");
@@ -219,8 +264,8 @@ class HipExecutor : public Executor {
     hipcheck(hipMemcpyAsync(d_state, hs, sizeof(InstState), hipMemcpyHostToDevice, stream_), "state");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
     hipcheck(hipGetLastError(), "before launch exec_programs");
-    hipLaunchKernelGGL(exec_programs, dim3(1), dim3(threads), sizeof(double) * lds_cap, stream_,
-                       (InstState*)d_state, (const Op*)d_op, (const int*)d_ranges, lds_cap, qb,
+    hipLaunchKernelGGL(threads > kThreads ? exec_programs_wide : exec_programs, dim3(1),
+                       dim3(threads), sizeof(double) * lds_cap, stream_, (InstState*)d_state, (const Op*)d_op, (const int*)d_ranges, lds_cap, qb,
                        jac_lanes);
     finish_launch("exec_programs (fixed region)");
     hipcheck(hipMemcpyAsync(hs, d_state, sizeof(InstState), hipMemcpyDeviceToHost, stream_),
@@ -259,8 +304,9 @@ class HipExecutor : public Executor {
     hipcheck(hipMemcpyAsync(d_states_, packed.data(), sizeof(InstState) * nb, hipMemcpyHostToDevice, stream_), "states");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
     hipcheck(hipGetLastError(), "before launch exec_programs");
-    hipLaunchKernelGGL(exec_programs, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
-                       d_states_, d_ops_, d_ranges_, lds_cap, qb, jac_lanes);
+    hipLaunchKernelGGL(threads > kThreads ? exec_programs_wide : exec_programs, dim3(nb),
+                       dim3(threads), sizeof(double) * lds_cap, stream_, d_states_, d_ops_,
+                       d_ranges_, lds_cap, qb, jac_lanes);
     finish_launch("exec_programs");
     float ms = 0.0f;
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
@@ -280,8 +326,9 @@ class HipExecutor : public Executor {
     hipcheck(hipMemcpyAsync(d_states_, packed.data(), sizeof(InstState) * nb, hipMemcpyHostToDevice, stream_), "states");
     hipcheck(hipEventRecord(ev0_, stream_), "event record");
     hipcheck(hipGetLastError(), "before launch exec_backward");
-    hipLaunchKernelGGL(exec_backward, dim3(nb), dim3(threads), sizeof(double) * lds_cap, stream_,
-                       d_states_, lds_cap, qb, jac_lanes);
+    hipLaunchKernelGGL(threads > kThreads ? exec_backward_wide : exec_backward, dim3(nb),
+                       dim3(threads), sizeof(double) * lds_cap, stream_, d_states_, lds_cap, qb,
+                       jac_lanes);
     finish_launch("exec_backward");
     float ms = 0.0f;
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");

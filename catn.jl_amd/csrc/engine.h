@@ -108,7 +108,7 @@ constexpr int kJacLanesDefault = 16;
 
 struct WG {
   int tid, nthr;
-  double* red;     // LDS, >= 16 doubles (one per wavefront)
+  double* red;     // LDS, 32 doubles (two halves of one per wavefront, <= 16 wavefronts)
   int* flag;       // LDS int
   int lds_cap;     // doubles available in the dynamic LDS region
   int64_t* sub;    // sub-step tick accumulators (instance state), or null
@@ -150,13 +150,14 @@ CATN_D double wg_sum(const WG& g, double v) {
 }
 
 // wg_sum with one barrier: the wavefront sums go to half `parity` of the
-// reduction slots (nthr <= 512). The caller alternates the parity and keeps a
-// workgroup barrier between the reads of one half and its next write.
+// reduction slots (16 per half, nthr <= 1024). The caller alternates the
+// parity and keeps a workgroup barrier between the reads of one half and its
+// next write.
 CATN_D double wg_sum_db(const WG& g, double v, int parity) {
 #if CATN_DEV
   for (int o = 32; o > 0; o >>= 1) v += shfl_xor(v, o);
   const int w = g.tid >> 6, l = g.tid & 63;
-  double* r = g.red + 8 * (parity & 1);
+  double* r = g.red + 16 * (parity & 1);
   if (l == 0) r[w] = v;
   g.sync();
   double s = 0.0;
@@ -745,8 +746,9 @@ CATN_D double refl_at(const double* V, int64_t ldv, bool unit, int k, int q) {
 // a unit-lower reflector panel read in place (vunit / cunit): one wavefront
 // per 16 x 16 tile of W walking the rows 4 at a time, so no workgroup barrier
 // is needed inside. When the tiles are fewer than the wavefronts the rows are
-// split among them and the partial tiles are summed through LDS (the top
-// 256 doubles per wavefront of the free LDS region).
+// split among them and the partial tiles are summed through LDS (one per
+// wavefront up to 4 partial tiles: the top 256 min(wavefronts, 4) doubles of
+// the free LDS region, which the callers leave free).
 // With T given (jb <= 16, upper triangular, ld kQBMax) and the rows not split,
 // W = op(T) V^T C is formed in registers (op(T) = T^T when tT): the
 // accumulator layout of a tile (lane l, element v: row l/16 + 4v, column l%16)
@@ -758,10 +760,12 @@ CATN_D bool wg_vtc_mfma(const WG& g, int jb, int N, int rows, const double* V, i
   const int nw = g.nthr >> 6, wv = g.tid >> 6, l = g.tid & 63;
   const int li = l & 15, lk = l >> 4;
   const int qt = (jb + 15) >> 4, ct = (N + 15) >> 4, tasks = qt * ct;
-  const int ks = tasks >= nw ? 1 : nw / tasks;
+  constexpr int kPartials = 4;
+  int ks = tasks >= nw ? 1 : nw / tasks;
+  if (ks > 1 && tasks * ks > kPartials) ks = kPartials / tasks > 1 ? kPartials / tasks : 1;
   const bool fuse = T != nullptr && ks == 1 && jb <= 16;
   const int chunk = ((rows + ks - 1) / ks + 3) & ~3;
-  double* P = lds_base() + (g.lds_cap - 256 * nw);
+  double* P = lds_base() + (g.lds_cap - 256 * (nw < kPartials ? nw : kPartials));
   for (int t = wv; t < tasks * ks; t += nw) {
     const int tile = t % tasks, part = t / tasks;
     const int q0 = (tile % qt) * 16, c0 = (tile / qt) * 16;

@@ -230,10 +230,13 @@ int catn_net_svd_profile(catn_net* net, int64_t* sub_ticks);
 int catn_host_stats(catn_ctx* ctx, double* plan_seconds, double* run_seconds);
 /* Launches of the reverse-sweep kernel and their summed duration (HIP events). */
 int catn_bwd_kernel_stats(catn_ctx* ctx, int64_t* launches, double* seconds, int32_t reset);
-/* Workgroup size of the interpreter kernel: 64, 128 or 256 lanes (default 256). */
+/* Workgroup size of the interpreter kernel: 64, 128, 256 (default), 512 or
+   1024 lanes. 512 and 1024 run the wide kernels (up to 16 wavefronts, one
+   workgroup per CU), for batches of fewer networks than CUs whose single ops
+   are large. */
 int catn_set_threads(catn_ctx* ctx, int32_t threads);
 /* Launch shape of the interpreter and reverse-sweep kernels: lanes per
-   workgroup (64, 128, 256); LDS budget per workgroup in doubles (0 =
+   workgroup (64, 128, 256, 512, 1024); LDS budget per workgroup in doubles (0 =
    automatic: 10176 for batches up to 512 networks, else 5056; otherwise
    within [2048, lds_limit]); Householder panel width (8, 16, 32; 0 = 16);
    lanes per column pair of the global-memory Jacobi (8..64; 0 = 16). The

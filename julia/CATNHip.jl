@@ -339,7 +339,7 @@ function correlation(n, edges, w, h, β; method=:pinning, lift=nothing, kw...)
     end
 end
 
-# launch shape: lanes per workgroup, LDS doubles per workgroup, Householder panel
+# launch shape: lanes per workgroup (64..1024), LDS doubles per workgroup, Householder panel
 # width, Jacobi lanes per pair (0 = automatic / default)
 set_launch!(threads=256, lds=0, panel=0, jacobi_lanes=0) =
     check(ccall((:catn_set_launch, lib), Cint, (Ptr{Cvoid}, Int32, Int32, Int32, Int32), ctx(),

@@ -56,7 +56,7 @@ class CpuExecutor : public Executor {
   }
   void run(const std::vector<InstState*>& states,
            const std::vector<const std::vector<Op>*>& programs) override {
-    double red[16];
+    double red[32];
     int flag = 0;
     WG g;
     g.tid = 0;
@@ -74,7 +74,7 @@ class CpuExecutor : public Executor {
     }
   }
   void run_backward(const std::vector<InstState*>& states, int lds_cap) override {
-    double red[16];
+    double red[32];
     int flag = 0;
     WG g;
     g.tid = 0;

@@ -440,7 +440,7 @@ class MtExecutor : public Executor {
   template <class F>
   void launch(int lds_cap, InstState* st, F body) {
     EmuShared sh(threads, lds_cap);
-    double red[16];
+    double red[32];
     int flag = 0;
     std::vector<std::thread> th;
     std::vector<EmuLane*> lanes(threads, nullptr);

@@ -35,8 +35,8 @@ def test_launch_shape_round_trip_and_limits(env):
     try:
         ctx.set_launch(128, 6784, 32, 8)
         assert ctx.launch() == (128, 6784, lim, 32, 8)
-        for bad in ((96, 0), (256, 100), (256, lim + 1), (256, 0, 12, 16), (256, 0, 16, 24),
-                    (64, 0, 16, 128)):
+        for bad in ((96, 0), (768, 0), (2048, 0), (256, 100), (256, lim + 1), (256, 0, 12, 16),
+                    (256, 0, 16, 24), (64, 0, 16, 128)):
             with pytest.raises(catn.CatnError):
                 ctx.set_launch(*bad)
         assert ctx.launch() == (128, 6784, lim, 32, 8)   # a rejected shape changes nothing
@@ -50,7 +50,7 @@ def test_every_shape_gives_the_same_contraction(env):
     kw = dict(Dmax=6, chi=12)
     t0, l0, lim, q0, j0 = ctx.launch()
     ref = None
-    cands = [(t, l, 16, 16) for (t, l) in catn.AUTOTUNE_SHAPES if l <= lim]
+    cands = [(t, l, 16, 16) for (t, l) in catn.AUTOTUNE_SHAPES + catn.AUTOTUNE_WIDE if l <= lim]
     cands += [(256, 5056, q, j) for (q, j) in catn.AUTOTUNE_TILES]
     try:
         for threads, lds, qb, jl in cands:
@@ -81,6 +81,8 @@ def test_autotune_picks_a_timed_shape(env):
         assert ctx.launch() == (res["threads"], res["lds_doubles"], lim, res["panel"],
                                 res["jacobi_lanes"])
         assert all(r["lnz_rel_dev"] <= 1e-10 for r in res["table"])
+        # a batch of 129 networks also tries the wide workgroups
+        assert {(r["threads"], r["lds_doubles"]) for r in res["table"]} >= set(catn.AUTOTUNE_WIDE)
         assert all(r["lds_doubles"] <= lim for r in res["table"])
         assert {(r["panel"], r["jacobi_lanes"]) for r in res["table"]} >= set(catn.AUTOTUNE_TILES)
         best = min(r["seconds"] for r in res["table"])

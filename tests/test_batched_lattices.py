@@ -80,7 +80,7 @@ BIG = dict(arena_elems=1 << 26, scratch_elems=1 << 26)
 @pytest.mark.gpu
 def test_config3_full_size_batch_fp32_storage(catn, gpu_ctx):
     """All 64 instances of config 3 in one batch with FP32 storage (six-pass
-    split-BF16 swap products): every lnZ finite and within the FP64 form of
+    split-BF16 swap products) on 1024-lane workgroups: every lnZ finite and within the FP64 form of
     the same engine at the same (Dmax, chi) to 1e-6 per spin on the
     instances up to 18 x 18 (the FP64 form of the 20..24-wide ones takes
     minutes per instance). Measured over 62 of the 64 instances: within
@@ -92,7 +92,14 @@ def test_config3_full_size_batch_fp32_storage(catn, gpu_ctx):
     nets = catn.ising_network_batch([(n, e, w, h) for (n, e, w, h, _) in inst],
                                     [(q, b) for q, (*_, b) in enumerate(inst)], ctx=gpu_ctx,
                                     dtype="f32x6", **KW, **BIG)
-    l32, err, psi = catn.contraction_batch(nets)
+    # the FP32 batch on wide workgroups (1024 lanes, one per CU), the FP64
+    # reference at the default 256 lanes
+    saved = gpu_ctx.launch()
+    gpu_ctx.set_launch(1024, 20352, 16, 16)
+    try:
+        l32, err, psi = catn.contraction_batch(nets)
+    finally:
+        gpu_ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
     assert np.all(np.isfinite(l32)) and np.all(psi == 1.0)
     assert max(t.maxdim_intermediate for t in nets) <= KW["Dmax"]
     small = [q for q, (n, *_) in enumerate(inst) if n <= 18 * 18]

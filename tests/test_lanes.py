@@ -255,3 +255,42 @@ def test_device_paths_matrix_cores(catn, devctx):
     assert np.allclose(S0, S1, rtol=1e-13, atol=1e-13 * S1[0])
     assert np.allclose(U0 @ np.diag(S0) @ V0.T, A, atol=1e-11 * S1[0])
     assert np.allclose(U0.T @ U0, np.eye(100), atol=1e-12)
+
+
+def test_wide_workgroups_lane_parallel(catn, ctxs, devctx):
+    """1024 lanes per workgroup (the wide kernel: 16 wavefronts, 16 reduction
+    slots per half of the one-barrier sums): a truncated forward and reverse
+    sweep on the lane emulator equal to the one-thread build with no
+    divergent barrier, and a 64 x 36 SVD at the minimum LDS budget through the
+    device paths (emulated shuffles and matrix cores; the Householder T
+    factors split their rows over partial tiles summed in LDS) equal to it to
+    rounding."""
+    mt, st = ctxs
+    dev, _ = devctx
+    lanes = 1024
+    n, edges, w, h = Cf.spin_glass_3reg(12, seed=2)
+    kw = dict(Dmax=4, chi=8)
+    b = _run(catn, st, n, edges, w, h, 1.0, **kw)
+    A = np.random.default_rng(9).standard_normal((64, 36))
+    S_ref = catn.tsvd(A, cutoff=0.0, ctx=st)[1]
+    before, dbefore = _divergent(), _dev_divergent()
+    mt.set_launch(lanes, 0, 0, 0)
+    try:
+        a = _run(catn, mt, n, edges, w, h, 1.0, **kw)
+    finally:
+        mt.set_launch(256, 0, 0, 0)
+    assert a[4] == b[4]
+    assert abs(a[0] - b[0]) / n < 1e-13
+    scale = max(1.0, np.abs(b[2]).max(), np.abs(b[3]).max())
+    assert np.abs(a[2] - b[2]).max() < 1e-9 * scale
+    assert np.abs(a[3] - b[3]).max() < 1e-9 * scale
+    _wave_ops()
+    dev.set_launch(lanes, 2048, 0, 0)
+    try:
+        U, S, V, _, _ = catn.tsvd(A, cutoff=0.0, ctx=dev)
+    finally:
+        dev.set_launch(256, 0, 0, 0)
+    assert _wave_ops()[0] > 0                 # FP64 matrix-core products ran
+    assert np.allclose(S, S_ref, rtol=1e-13, atol=1e-13 * S_ref[0])
+    assert np.allclose(U @ np.diag(S) @ V.T, A, atol=1e-11 * S_ref[0])
+    assert _divergent() == before and _dev_divergent() == dbefore
