@@ -106,6 +106,7 @@ _SIGS = {
     "catn_braket_network": (C.c_int, [_P, C.c_int32, _I32P, _I64P, _DP, _I64P, _P, _P]),
     "catn_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "catn_get_launch": (C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P]),
+    "catn_set_batch_streams": (C.c_int, [_P, C.c_int32]),
     "catn_node_swap": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _DP]),
     "catn_node_cano_to": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "catn_node_left_canonical": (C.c_int, [_P, C.c_int32]),
@@ -215,6 +216,12 @@ class Context:
         width (0 = 16) and global-Jacobi lanes per column pair (0 = 16)."""
         self._check(self.lib.catn_set_launch(self.h, int(threads), int(lds_doubles), int(panel),
                                              int(jacobi_lanes)))
+
+    def set_batch_streams(self, streams: int):
+        """Concurrent groups (streams) of a forward contraction_batch of
+        distinct networks: 1 to 4 (default 4); 1 makes every launch
+        batch-wide."""
+        self._check(self.lib.catn_set_batch_streams(self.h, int(streams)))
 
     def launch(self):
         """(threads, lds_doubles, lds_limit, panel, jacobi_lanes) of the current

@@ -91,6 +91,17 @@ class Executor {
   virtual void run_backward(const std::vector<InstState*>& states, int lds_cap) = 0;
   // frequency of the device wall clock read by the op timers (Hz)
   virtual double tick_hz() const { return 1.0; }
+  // A sibling on the same device with its own stream and staging buffers,
+  // used by one host thread while others use theirs (concurrent groups of a
+  // batch); null when this executor has none. The caller owns it and copies
+  // the launch shape into it.
+  virtual Executor* sibling() { return nullptr; }
+  // Make the calling host thread use this executor's device.
+  virtual void bind_thread() {}
+  // The executor that owns the networks' memory when this one is a sibling.
+  Executor* parent = nullptr;
+  // streams (concurrent groups) of a forward batch of distinct networks, 1..4
+  int batch_streams = 4;
   // Whether the reverse sweep may run over a Dmax cut (cut_bondim_opt!) on
   // this executor. The gfx950 executor refuses it: gradient runs of
   // networks with a Dmax cut faulted the GPU and the cause is not yet known,
@@ -418,12 +429,16 @@ inline Executor* exec_of(const catn_net* n) {
 
 inline void run_one(catn_net* n) { run_nets(exec_of(n), {n}); }
 
+// host threads parallel_for may use on the calling thread (0: up to 16); the
+// concurrent groups of a batch split them
+inline thread_local int g_plan_threads = 0;
+
 // f(i) for i in [0, n) on up to 16 host threads (planners of different
 // networks share no state); the first exception is rethrown.
 template <class F>
 void parallel_for(int n, F f) {
   int T = (int)std::thread::hardware_concurrency();
-  T = std::max(1, std::min(T, 16));
+  T = std::max(1, std::min(T, g_plan_threads > 0 ? g_plan_threads : 16));
   if (n < 16 || T == 1) {
     for (int i = 0; i < n; ++i) f(i);
     return;
@@ -628,7 +643,8 @@ inline void add_tensor_node(catn_net* net, int id, int nd, const int64_t* dims, 
 inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
   for (catn_net* n : nets) {
     if (n->consumed) throw Error(-5, "contraction!: network already contracted");
-    if (exec_of(n) != ex) throw Error(-2, "contraction: networks of another context");
+    if (exec_of(n) != (ex->parent ? ex->parent : ex))
+      throw Error(-2, "contraction: networks of another context");
     n->ensure_device();
     n->reset_acc();
   }
@@ -744,6 +760,89 @@ inline void contract_nets(Executor* ex, const std::vector<catn_net*>& nets) {
     active.swap(still);
     att.swap(next_att);
   }
+}
+
+// Forward contraction of a batch of distinct networks in concurrent groups.
+// contract_nets advances a batch one planning segment at a time: every launch
+// waits for the slowest network of the batch before any network starts its
+// next segment, so a batch whose networks differ in size takes the sum over
+// segments of the slowest segment. Here the networks are dealt into up to
+// kGroupStreams groups, each contracted by its own host thread on a sibling
+// executor (its own stream), so a group waits only for its own networks.
+// They are dealt in decreasing order of node count, round robin, so the
+// largest networks land in different groups; within a group, networks of one
+// structure still share plans. A batch of fewer than 3 structures (such as
+// the pinned networks of one or two instances) keeps the one-stream path,
+// where all of them share plans. Returns false (nothing run) when the batch
+// does not qualify or the executor has no siblings.
+constexpr int kGroupStreams = 16;     // streams of a batch at most
+constexpr int kGroupMinNets = 4;      // networks per group at least
+
+inline bool contract_nets_grouped(Executor* ex, const std::vector<catn_net*>& nets) {
+  const int nn = (int)nets.size();
+  const int G = std::min(std::min(kGroupStreams, ex->batch_streams), nn / kGroupMinNets);
+  if (G < 2 || ex->parent) return false;
+  for (catn_net* n : nets) {
+    if (n->consumed) throw Error(-5, "contraction!: network already contracted");
+    if (exec_of(n) != ex) throw Error(-2, "contraction: networks of another context");
+    if (n->hs.record) return false;
+  }
+  {
+    // planner-state classes (networks of one structure: the pinned networks of
+    // an instance, lattices of one size); a batch of few classes shares its
+    // plans best on one stream
+    std::vector<uint64_t> fp(nn);
+    parallel_for(nn, [&](int k) { fp[k] = nets[k]->pl.fingerprint(); });
+    std::unordered_map<uint64_t, int> classes;
+    for (int k = 0; k < nn; ++k) classes.emplace(fp[k], k);
+    if ((int)classes.size() < 3) return false;
+  }
+  // group 0 runs on this executor's own stream, groups 1.. on siblings: G
+  // streams in all, one per hardware queue (a fifth stream would share a
+  // queue and serialise two groups)
+  std::vector<std::unique_ptr<Executor>> sib;
+  for (int g = 1; g < G; ++g) {
+    sib.emplace_back(ex->sibling());
+    if (!sib.back()) return false;
+    Executor* s = sib.back().get();
+    s->parent = ex;
+    s->threads = ex->threads;
+    s->lds_override = ex->lds_override;
+    s->qb = ex->qb;
+    s->jac_lanes = ex->jac_lanes;
+  }
+  // uploads through the owning executor, before any group starts
+  for (catn_net* n : nets) n->ensure_device();
+  std::vector<int> order(nn);
+  for (int k = 0; k < nn; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return nets[a]->pl.nodes.size() > nets[b]->pl.nodes.size();
+  });
+  std::vector<std::vector<catn_net*>> groups(G);
+  for (int k = 0; k < nn; ++k) groups[k % G].push_back(nets[order[k]]);
+  std::vector<std::exception_ptr> errs(G);
+  std::vector<std::thread> th;
+  for (int g = 0; g < G; ++g)
+    th.emplace_back([&, g] {
+      try {
+        Executor* e = g == 0 ? ex : sib[g - 1].get();
+        e->bind_thread();
+        g_plan_threads = std::max(1, 16 / G);
+        contract_nets(e, groups[g]);
+      } catch (...) {
+        errs[g] = std::current_exception();
+      }
+    });
+  for (auto& t : th) t.join();
+  for (auto& s : sib) {   // groups 1..: launches and times (kernel time summed over streams)
+    ex->launches += s->launches;
+    ex->kernel_seconds += s->kernel_seconds;
+    ex->plan_seconds += s->plan_seconds;
+    ex->run_seconds += s->run_seconds;
+  }
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
+  return true;
 }
 
 // contraction! with the tape on, then one reverse-sweep launch for all
@@ -1215,7 +1314,7 @@ int catn_contraction_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, 
   for (int k = 1; k < count; ++k)
     if (v[k]->cplx != v[0]->cplx)
       throw catn::Error(-2, "contraction_batch: real and complex networks in one batch");
-  catn::contract_nets(ctx->ex.get(), v);
+  if (!catn::contract_nets_grouped(ctx->ex.get(), v)) catn::contract_nets(ctx->ex.get(), v);
   const bool cx = count > 0 && v[0]->cplx;
   for (int k = 0; k < count; ++k) {
     if (lnZ) lnZ[k] = v[k]->hs.lnZ + v[k]->hs.lntail;
@@ -1428,6 +1527,14 @@ int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles, int32_t
   ctx->ex->lds_override = lds_doubles;
   ctx->ex->qb = panel ? panel : catn::kQBDefault;
   ctx->ex->jac_lanes = jacobi_lanes ? jacobi_lanes : catn::kJacLanesDefault;
+  return 0;
+}
+
+int catn_set_batch_streams(catn_ctx* ctx, int32_t streams) {
+  if (streams < 1 || streams > catn::kGroupStreams)
+    return catn::fail(-2, "catn_set_batch_streams: streams must be in [1, " +
+                              std::to_string(catn::kGroupStreams) + "]");
+  ctx->ex->batch_streams = streams;
   return 0;
 }
 

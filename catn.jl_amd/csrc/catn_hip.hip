@@ -248,6 +248,10 @@ class HipExecutor : public Executor {
     }
     dfree(d);
   }
+  // a second executor on the same device: its own stream, events and
+  // staging buffers (the networks' memory stays with this one)
+  Executor* sibling() override { return new HipExecutor(device_); }
+  void bind_thread() override { hipcheck(hipSetDevice(device_), "hipSetDevice"); }
   void* fixed_region(size_t* bytes) override {
     if (bytes) *bytes = sizeof(double) * kFixedDoubles;
     return g_fixed_ptr[device_];
@@ -312,7 +316,11 @@ class HipExecutor : public Executor {
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
     launches += 1;
     kernel_seconds += 1e-3 * ms;
-    hipcheck(hipMemcpy(packed.data(), d_states_, sizeof(InstState) * nb, hipMemcpyDeviceToHost), "states back");
+    // on this executor's stream: a null-stream copy would wait for the
+    // kernels of sibling executors' streams too
+    hipcheck(hipMemcpyAsync(packed.data(), d_states_, sizeof(InstState) * nb,
+                            hipMemcpyDeviceToHost, stream_), "states back");
+    hipcheck(hipStreamSynchronize(stream_), "states back sync");
     for (int b = 0; b < nb; ++b) *states[b] = packed[b];
   }
 
@@ -334,7 +342,11 @@ class HipExecutor : public Executor {
     hipcheck(hipEventElapsedTime(&ms, ev0_, ev1_), "event elapsed");
     bwd_launches += 1;
     bwd_seconds += 1e-3 * ms;
-    hipcheck(hipMemcpy(packed.data(), d_states_, sizeof(InstState) * nb, hipMemcpyDeviceToHost), "states back");
+    // on this executor's stream: a null-stream copy would wait for the
+    // kernels of sibling executors' streams too
+    hipcheck(hipMemcpyAsync(packed.data(), d_states_, sizeof(InstState) * nb,
+                            hipMemcpyDeviceToHost, stream_), "states back");
+    hipcheck(hipStreamSynchronize(stream_), "states back sync");
     for (int b = 0; b < nb; ++b) *states[b] = packed[b];
   }
 

@@ -247,6 +247,16 @@ int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles, int32_t
                     int32_t jacobi_lanes);
 int catn_get_launch(catn_ctx* ctx, int32_t* threads, int32_t* lds_doubles, int32_t* lds_limit,
                     int32_t* panel, int32_t* jacobi_lanes);
+/* Streams of a forward catn_contraction_batch of distinct networks (1..4,
+   default 4). With s > 1 and at least 4 networks per stream, the networks are
+   dealt (largest first, round robin) into s groups that run concurrently,
+   each on its own stream and host thread, so that a group's planning segments
+   wait only for its own networks; 1 keeps every launch batch-wide. Batches
+   of fewer than 3 network structures (the pinned networks of one or two
+   instances) and recorded runs take one stream. No counterpart in the
+   reference; results are those of the one-stream path up to the rounding of
+   the launch shape. */
+int catn_set_batch_streams(catn_ctx* ctx, int32_t streams);
 /* Device memory statistics: arena high-water mark and capacity (elements). */
 int catn_net_mem(catn_net* net, int64_t* arena_used, int64_t* arena_cap);
 

@@ -16,7 +16,8 @@ module CATNHip
 using ChainRulesCore
 
 export ising_network, tensor_network, braket_network, contraction!, free_energy,
-       magnetization, correlation, ising_lnZ, ising_lnZ_batch, result_tensor, set_launch!
+       magnetization, correlation, ising_lnZ, ising_lnZ_batch, result_tensor, set_launch!,
+       set_batch_streams!
 
 const lib = get(ENV, "CATN_HIP_LIB", joinpath(@__DIR__, "..", "catn.jl_amd", "lib", "libcatn_hip.so"))
 
@@ -344,5 +345,9 @@ end
 set_launch!(threads=256, lds=0, panel=0, jacobi_lanes=0) =
     check(ccall((:catn_set_launch, lib), Cint, (Ptr{Cvoid}, Int32, Int32, Int32, Int32), ctx(),
                 threads, lds, panel, jacobi_lanes))
+
+# concurrent groups (streams) of a forward batch of distinct networks, 1..4
+set_batch_streams!(streams=4) =
+    check(ccall((:catn_set_batch_streams, lib), Cint, (Ptr{Cvoid}, Int32), ctx(), streams))
 
 end # module

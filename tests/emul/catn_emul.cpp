@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <vector>
 
 #include "../../catn.jl_amd/csrc/engine.h"
@@ -15,9 +16,14 @@
 
 namespace catn {
 
+// launches run by sibling executors (the concurrent groups of a batch)
+static std::atomic<long> g_sibling_runs{0};
+
 class CpuExecutor : public Executor {
  public:
   const char* name() const override { return "cpu-emulator"; }
+  // concurrent batch groups run on host threads (the LDS stand-in is per thread)
+  Executor* sibling() override { return new CpuExecutor(); }
   // CATN_EMU_POISON=1 fills new device blocks with 0xFF bytes (NaN doubles,
   // -1 integers), as hipMalloc gives no zeroed memory: a read of memory the
   // engine never wrote then shows up in the results.
@@ -56,6 +62,7 @@ class CpuExecutor : public Executor {
   }
   void run(const std::vector<InstState*>& states,
            const std::vector<const std::vector<Op>*>& programs) override {
+    if (parent) g_sibling_runs.fetch_add(1);
     double red[32];
     int flag = 0;
     WG g;
@@ -94,3 +101,5 @@ class CpuExecutor : public Executor {
 Executor* make_executor(int) { return new CpuExecutor(); }
 
 }  // namespace catn
+
+extern "C" long catn_emu_sibling_runs() { return catn::g_sibling_runs.load(); }

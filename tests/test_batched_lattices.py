@@ -8,6 +8,8 @@ engine at the same (Dmax, chi), which isolates precision, and the FP64
 form with the FP64 oracle on the smaller lattices. Tolerances: f32x6
 (6-pass) 1e-6 per spin, f32 (3-pass) 5e-6, FP64 1e-12.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -110,3 +112,59 @@ def test_config3_full_size_batch_fp32_storage(catn, gpu_ctx):
     l64 = catn.contraction_batch(ref)[0]
     d = np.abs(l32[small] - l64) / ns[small]
     assert d.max() <= 1e-6, d
+
+
+def _sibling_runs(ctx):
+    fn = getattr(ctx.lib, "catn_emu_sibling_runs", None)
+    if fn is None:
+        return None
+    fn.restype = ctypes.c_long
+    return fn()
+
+
+def test_grouped_batch_equals_one_stream(env):
+    """A forward batch of distinct networks runs in concurrent groups (one
+    stream and host thread each, largest networks dealt first): per network
+    the same ops on the same launch shape, so lnZ, error, psi, edge sequence
+    and Dmax-cut kept counts equal the one-stream batch bit for bit."""
+    catn, ctx, sizes, count = env
+    emu = count < 12
+    sizes, count = (tuple(range(4, 10)), 10) if emu else (tuple(range(6, 17)), 24)
+    groups = min(4, count // 4)
+    inst = _instances(sizes, count, seed=77)
+    assert len({n for (n, *_) in inst}) >= 2 * groups     # the batch qualifies for groups
+    out = {}
+    try:
+        for streams in (1, 4):
+            ctx.set_batch_streams(streams)
+            nets = [catn.ising_network(n, e, w, h, b, ctx=ctx, **KW) for (n, e, w, h, b) in inst]
+            before = _sibling_runs(ctx)
+            lz, err, psi = catn.contraction_batch(nets)
+            if before is not None:    # the emulator counts the groups' launches
+                assert (_sibling_runs(ctx) > before) == (streams > 1)
+            out[streams] = (lz, err, psi, [t.edge_log() for t in nets], [t.cut_kept() for t in nets])
+    finally:
+        ctx.set_batch_streams(4)
+    a, b = out[1], out[4]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert a[3] == b[3]
+    assert all(np.array_equal(x, y) for x, y in zip(a[4], b[4]))
+
+
+def test_grouped_batch_reports_a_failing_group(catn, emu_ctx):
+    """A network that fails (scratch exhausted) in one group raises after every
+    group has finished; the context keeps working."""
+    inst = _instances((4, 5, 6, 7, 8), 8, seed=5)
+    nets = [catn.ising_network(n, e, w, h, b, ctx=emu_ctx, **KW) for (n, e, w, h, b) in inst[:7]]
+    n, e, w, h, b = inst[7]
+    nets.append(catn.ising_network(n, e, w, h, b, ctx=emu_ctx, scratch_elems=64, **KW))
+    assert len({n for (n, *_) in inst}) >= 4
+    before = _sibling_runs(emu_ctx)
+    with pytest.raises(catn.CatnError, match="scratch full"):
+        catn.contraction_batch(nets)
+    assert _sibling_runs(emu_ctx) > before
+    ok = [catn.ising_network(n, e, w, h, b, ctx=emu_ctx, **KW) for (n, e, w, h, b) in inst]
+    lz, _, _ = catn.contraction_batch(ok)
+    assert np.all(np.isfinite(lz))
+    with pytest.raises(catn.CatnError, match="streams"):
+        emu_ctx.set_batch_streams(5)
