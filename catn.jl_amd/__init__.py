@@ -219,8 +219,9 @@ class Context:
 
     def set_batch_streams(self, streams: int):
         """Concurrent groups (streams) of a forward contraction_batch of
-        distinct networks: 1 to 4 (default 4); 1 makes every launch
-        batch-wide."""
+        distinct networks: 1 to 16 (default 4, the HIP runtime's default
+        hardware queues; keep it at most GPU_MAX_HW_QUEUES); 1 makes every
+        launch batch-wide."""
         self._check(self.lib.catn_set_batch_streams(self.h, int(streams)))
 
     def launch(self):

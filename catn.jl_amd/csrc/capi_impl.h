@@ -100,7 +100,8 @@ class Executor {
   virtual void bind_thread() {}
   // The executor that owns the networks' memory when this one is a sibling.
   Executor* parent = nullptr;
-  // streams (concurrent groups) of a forward batch of distinct networks, 1..4
+  // streams (concurrent groups) of a forward batch of distinct networks
+  // (1..kGroupStreams; 4 = the HIP runtime's default hardware queues)
   int batch_streams = 4;
   // Whether the reverse sweep may run over a Dmax cut (cut_bondim_opt!) on
   // this executor. The gfx950 executor refuses it: gradient runs of

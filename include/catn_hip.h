@@ -247,8 +247,10 @@ int catn_set_launch(catn_ctx* ctx, int32_t threads, int32_t lds_doubles, int32_t
                     int32_t jacobi_lanes);
 int catn_get_launch(catn_ctx* ctx, int32_t* threads, int32_t* lds_doubles, int32_t* lds_limit,
                     int32_t* panel, int32_t* jacobi_lanes);
-/* Streams of a forward catn_contraction_batch of distinct networks (1..4,
-   default 4). With s > 1 and at least 4 networks per stream, the networks are
+/* Streams of a forward catn_contraction_batch of distinct networks (1..16,
+   default 4, the HIP runtime's default hardware queues per process; more
+   streams than GPU_MAX_HW_QUEUES share queues and serialise). With s > 1 and
+   at least 4 networks per stream, the networks are
    dealt (largest first, round robin) into s groups that run concurrently,
    each on its own stream and host thread, so that a group's planning segments
    wait only for its own networks; 1 keeps every launch batch-wide. Batches

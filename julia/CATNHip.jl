@@ -346,7 +346,7 @@ set_launch!(threads=256, lds=0, panel=0, jacobi_lanes=0) =
     check(ccall((:catn_set_launch, lib), Cint, (Ptr{Cvoid}, Int32, Int32, Int32, Int32), ctx(),
                 threads, lds, panel, jacobi_lanes))
 
-# concurrent groups (streams) of a forward batch of distinct networks, 1..4
+# concurrent groups (streams) of a forward batch of distinct networks, 1..16 (default 4)
 set_batch_streams!(streams=4) =
     check(ccall((:catn_set_batch_streams, lib), Cint, (Ptr{Cvoid}, Int32), ctx(), streams))
 

@@ -166,5 +166,6 @@ def test_grouped_batch_reports_a_failing_group(catn, emu_ctx):
     ok = [catn.ising_network(n, e, w, h, b, ctx=emu_ctx, **KW) for (n, e, w, h, b) in inst]
     lz, _, _ = catn.contraction_batch(ok)
     assert np.all(np.isfinite(lz))
-    with pytest.raises(catn.CatnError, match="streams"):
-        emu_ctx.set_batch_streams(5)
+    for bad in (0, 17):
+        with pytest.raises(catn.CatnError, match="streams"):
+            emu_ctx.set_batch_streams(bad)
