@@ -450,3 +450,56 @@ def test_device_fused_peps_norm(env, L, D, Dmax, chi, dtype):
     assert tn.cut_kept() == [e[2] for e in s.events if e[0] == "cut_keep"]
     assert abs(lnz - lo) <= 1e-12 * max(1.0, abs(lo))
     assert abs(psi - complex(po)) < 1e-9
+
+
+def _ising_factor_graph(L, beta, seed):
+    """+-J Ising model on an open L x L lattice as a generic network of one
+    tensor per spin: T_k[s_1, ...] = sum_sigma prod_legs F[sigma, s_leg], with
+    F the Boltzmann matrix B_e = exp(beta w_e sigma s) on one end of each
+    coupling and the identity on the other (real entries; its splits have
+    exactly degenerate singular values)."""
+    from oracle import exact as X
+    r = np.random.default_rng(seed)
+    edges = X.square_edges(L)
+    w = r.choice([-1.0, 1.0], len(edges))
+    spin = np.array([1.0, -1.0])
+    legs = {k: [] for k in range(1, L * L + 1)}
+    for e, (a, b) in enumerate(edges):
+        legs[a].append((f"e{e}", np.exp(beta * w[e] * np.outer(spin, spin))))
+        legs[b].append((f"e{e}", np.eye(2)))
+    ts, ixs = [], []
+    for k in range(1, L * L + 1):
+        t = 0.0
+        for sig in range(2):
+            f = np.ones(())
+            for (_, F) in legs[k]:
+                f = np.multiply.outer(f, F[sig])
+            t = t + f
+        ts.append(np.asarray(t, dtype=float))
+        ixs.append([lab for (lab, _) in legs[k]])
+    return L * L, edges, w, ts, ixs
+
+
+@pytest.mark.parametrize("L,beta,Dmax,chi", [(4, 1.0, -1, 10_000), (8, 0.6, 32, 128),
+                                          (10, 0.6, 32, 128)])
+def test_device_complex_type_on_real_data_is_the_real_engine(env, L, beta, Dmax, chi):
+    """The complex engine (complex one-sided Jacobi, complex rotations) on
+    real data with exactly degenerate spectra (+-J couplings): lnZ, the phase
+    (+1), the edge sequence and the Dmax-cut kept counts equal the real
+    engine's (QR-preconditioned real Jacobi), and the exact lnZ at 4 x 4."""
+    from oracle import exact as X
+    catn, ctx = env
+    n, edges, w, ts, ixs = _ising_factor_graph(L, beta, 10 + L)
+    kw = dict(Dmax=Dmax, chi=chi, arena_elems=1 << 24, scratch_elems=1 << 24)
+    real = catn.tensor_network(ts, ixs, ctx=ctx, **kw)
+    cplx = catn.tensor_network([t.astype(complex) for t in ts], ixs, ctx=ctx, **kw)
+    lr, er, pr = catn.contraction(real)
+    lc, ec, pc = catn.contraction(cplx)
+    assert abs(lc - lr) <= 1e-12 * max(1.0, abs(lr))
+    assert abs(pc - pr) <= 1e-12 and abs(ec - er) <= 1e-9 * max(1.0, er)
+    assert cplx.edge_log() == real.edge_log()
+    assert cplx.cut_kept() == real.cut_kept()
+    if L == 4:
+        assert abs(lr - X.brute_lnZ(n, edges, w, np.zeros(n), beta)) <= 1e-12 * abs(lr)
+    if L == 10:   # splits of min(m, n) in 17..32 ran
+        assert cplx.profile()[1].get("<=32", (0,))[0] > 0
