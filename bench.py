@@ -683,25 +683,22 @@ def main():
 
 def config3_instances(seed=2025, count=64):
     """Config 3: 64 open L x L lattices, L uniform in {8, 10, ..., 24}, w = +-1,
-    h = 0, beta uniform in [0.2, 1.2] (the same generator as the GPU test)."""
-    from oracle import exact as X
-    r = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        L = int(r.choice(np.arange(8, 25, 2)))
-        edges = X.square_edges(L)
-        out.append((L * L, edges, r.choice([-1.0, 1.0], len(edges)), np.zeros(L * L),
-                    float(r.uniform(0.2, 1.2))))
-    return out
+    h = 0, beta uniform in [0.2, 1.2] (oracle/configs.py batch_instances, the
+    generator of the GPU test and of tests/golden/config3_oracle.json)."""
+    from oracle import configs as Cf
+    return [(n, e, w, h, b) for (_, n, e, w, h, b) in Cf.batch_instances(count, seed)]
+
+
+C3_ORACLE = os.path.join(ROOT, "tests", "golden", "config3_oracle.json")
 
 
 def run_config3(catn, ctx, steps, reference=True):
     """Config 3's batched forward: all 64 lattices in one batch per step with FP32
     storage (six-pass split-BF16 swap products), lnZ per instance. Networks are
-    built (device-side sites) before each timed step. Accuracy: the FP64 form of
-    the same engine at the same (Dmax, chi) on the instances up to 18 x 18 (the
-    FP64 form of the wider ones takes minutes per instance). The chosen form's
-    gradient is refused there as at config 2 (every lattice reaches a Dmax cut)."""
+    built (device-side sites) before each timed step. Accuracy: |dlnZ/N| of
+    every instance against the FP64 oracle at the same (Dmax, chi), from the
+    committed fixture. The chosen form's gradient is refused there as at
+    config 2 (every lattice reaches a Dmax cut)."""
     inst = config3_instances()
     kw = dict(Dmax=32, chi=128, arena_elems=1 << 26, scratch_elems=1 << 26)
     saved = ctx.launch()
@@ -736,18 +733,18 @@ def run_config3(catn, ctx, steps, reference=True):
         ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
         ctx.set_batch_streams(4)
         return out
-    small = [q for q, (n, *_) in enumerate(inst) if n <= 18 * 18]
-    ref = catn.ising_network_batch([inst[q][:4] for q in small],
-                                   [(k, inst[q][4]) for k, q in enumerate(small)], ctx=ctx,
-                                   dtype="f64", **kw)
-    l64 = catn.contraction_batch(ref)[0]
-    for t in ref:
-        t.free()
     ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
     ctx.set_batch_streams(4)
-    ns = np.array([inst[q][0] for q in small], dtype=float)
-    out["max_abs_dlnZ_per_N_vs_fp64_same_engine"] = float(np.max(np.abs(lz[small] - l64) / ns))
-    out["fp64_reference_instances"] = len(small)
+    # against the FP64 oracle's lnZ of every instance at the same (Dmax, chi)
+    # (tests/golden/config3_oracle.json, written by tests/golden/make_config3.py)
+    with open(C3_ORACLE) as f:
+        gold = json.load(f)
+    ref = np.array([g["lnZ"] for g in gold["instances"]])
+    ns = np.array([n for (n, *_) in inst], dtype=float)
+    d = np.abs(lz - ref) / ns
+    out["abs_dlnZ_per_N_vs_fp64_oracle"] = [float(x) for x in d]
+    out["max_abs_dlnZ_per_N_vs_fp64_oracle"] = float(d.max())
+    out["instances_within_1e-6_of_fp64_oracle"] = int(np.sum(d <= 1e-6))
     return out
 
 
