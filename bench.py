@@ -476,7 +476,7 @@ def main():
     ap.add_argument("--config3-steps", type=int, default=1,
                     help="timed steps of config 3's full-size batched forward (0: skip)")
     ap.add_argument("--config3-only", action="store_true",
-                    help="only config 3's batched forward (no FP64 reference), for profiling")
+                    help="only config 3's batched forward and its oracle comparison, for profiling")
     ap.add_argument("--cut-free-batch", type=int, default=128,
                     help="instances per chosen-form step of the cut-free comparison (0: skip it)")
     ap.add_argument("--cut-free-only", action="store_true",
@@ -534,7 +534,7 @@ def main():
     catn = G.load_package()
     ctx = catn.Context(0)
     if args.config3_only:
-        print(json.dumps(run_config3(catn, ctx, max(args.config3_steps, 1), reference=False)),
+        print(json.dumps(run_config3(catn, ctx, max(args.config3_steps, 1))),
               flush=True)
         return
     if args.cut_free_only:
@@ -692,7 +692,7 @@ def config3_instances(seed=2025, count=64):
 C3_ORACLE = os.path.join(ROOT, "tests", "golden", "config3_oracle.json")
 
 
-def run_config3(catn, ctx, steps, reference=True):
+def run_config3(catn, ctx, steps):
     """Config 3's batched forward: all 64 lattices in one batch per step with FP32
     storage (six-pass split-BF16 swap products), lnZ per instance. Networks are
     built (device-side sites) before each timed step. Accuracy: |dlnZ/N| of
@@ -729,10 +729,6 @@ def run_config3(catn, ctx, steps, reference=True):
         "steps": steps,
         "chosen_form": "refused by the device (reverse sweep across a Dmax cut)",
     }
-    if not reference:
-        ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
-        ctx.set_batch_streams(4)
-        return out
     ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
     ctx.set_batch_streams(4)
     # against the FP64 oracle's lnZ of every instance at the same (Dmax, chi)
