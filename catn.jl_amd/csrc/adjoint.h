@@ -761,6 +761,7 @@ CATN_DN int run_backward(const WG& g, InstState* st) {
   E.tape_len = st->tape_len;
   E.rp = nullptr;
   E.g.svd_fault = st->fault & 1;   // replays take the forward's SVD path
+  E.g.jac_tol = st->f32 ? 0x1p-26 : 0.0;
   wg_fill(g, st->adj, 0.0, st->arena_top);
   if (g.tid == 0) st->replay_dev = 0.0;
   g.sync();

@@ -115,6 +115,9 @@ struct WG {
   int qb = kQBDefault;               // Householder panel width (8, 16 or 32)
   int jac_lanes = kJacLanesDefault;  // global Jacobi lanes per pair (8..64)
   int svd_fault = 0;                 // 1: every SVD's primary path counts as not converged
+  // Jacobi pair test floor (cosine): 0 for FP64 storage (4 eps sqrt(m) only);
+  // FP32 storage stops rotating at 2^-26, below the rounding of its outputs
+  double jac_tol = 0.0;
 #if defined(CATN_EMU_MT)
   // the emulator checks that every lane arrives from the same call site
   void sync(const char* file = __builtin_FILE(), int line = __builtin_LINE()) const {
@@ -1096,7 +1099,7 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
   const int ng = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
   // pair convergence test on the cosine |a_p.a_q| / (|a_p||a_q|); a rotation
   // below that level is below working precision and can repeat forever
-  const double tol = 4.0 * 2.220446049250313e-16 * sqrt((double)(m > 1 ? m : 1));
+  const double tol = fmax(4.0 * 2.220446049250313e-16 * sqrt((double)(m > 1 ? m : 1)), g.jac_tol);
   // columns below 1e-18 ||A||_F are rounding noise of a rank-deficient input:
   // they cannot be made orthogonal (the column space is full) and only shrink
   // towards denormals, so they are treated as zero columns
@@ -1372,7 +1375,8 @@ CATN_DN int wg_svd_tiny(const WG& g, Scratch& sc, const double* A, int m, int n,
     for (int j = 0; j < tn; ++j)
       for (int i = 0; i < tn; ++i) V[i + tn * j] = i == j ? 1.0 : 0.0;
     const double thr = 1e-36 * fro2;
-    const double tol = 4.0 * 2.220446049250313e-16 * sqrt((double)(tm > 1 ? tm : 1));
+    const double tol = fmax(4.0 * 2.220446049250313e-16 * sqrt((double)(tm > 1 ? tm : 1)),
+                            g.jac_tol);
     const int N = tn + (tn & 1), npairs = N / 2;
     int sw = tn < 2 ? 0 : -1;
     for (int sweep = 1; sweep <= 60 && sw < 0; ++sweep) {
@@ -3145,6 +3149,7 @@ CATN_DN int run_program(const WG& g, InstState* st, const Op* ops, int begin, in
   E.bytes = 0.0;
   E.tape_len = st->tape_len;
   E.g.svd_fault = st->fault & 1;
+  E.g.jac_tol = st->f32 ? 0x1p-26 : 0.0;
   if (g.tid == 0 && g.lds_cap > st->lds_max) st->lds_max = g.lds_cap;
   int64_t shwm = 0;
   for (int i = begin; i < end && E.status == STATUS_OK; ++i) {
