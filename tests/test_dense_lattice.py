@@ -1,13 +1,18 @@
-"""Config 4's dense-regime network at a size the one-workgroup ops reach: an
-open 4 x 4 square-lattice tensor network with bond dimension 4 on every edge,
-entries i.i.d. uniform[0, 1) (positive variant) and N(0, 1) (signed variant),
-Dmax = 256, chi = 1024, in FP64 and in FP32 storage with split-BF16 swap
-products (3 and 6 passes), against the exact contraction (numpy einsum of the
-16 tensors). Tolerances on lnZ: FP64 1e-12, six-pass 2e-6, three-pass 1e-4
-(its 2^-16 products meet the sign cancellations of the signed variant: 2.9e-5
-measured). At 5 x 5 the swaps' thetas pass a few thousand rows and columns
-and one network no longer finishes in minutes on one workgroup, so config 4's
-12 x 12 size is not run.
+"""Config 4's dense-regime network: an open L x L square-lattice tensor
+network with bond dimension 4 on every edge, entries i.i.d. uniform[0, 1)
+(positive variant) and N(0, 1) (signed variant), Dmax = 256, chi = 1024, in
+FP64 and in FP32 storage with split-BF16 swap products (3 and 6 passes),
+against the exact contraction: numpy einsum of the tensors at 4 x 4, and a
+row-by-row transfer contraction (exact, 4^(L+1) entries) that this file pins
+against einsum at 3 x 3 and 4 x 4. Tolerances on lnZ at 4 x 4: FP64 1e-12,
+six-pass 2e-6, three-pass 1e-4 (its 2^-16 products meet the sign
+cancellations of the signed variant: 2.9e-5 measured). Measured on the
+MI355X (one 1024-lane workgroup): 5 x 5 in 0.03 s, 6 x 6 in 0.1 s, 8 x 8 in
+9.3 s (one split of min(m, n) > 256 takes 8.5 s of it). From 10 x 10 a swap
+reaches swap!'s randomized-SVD switch with a sketch of min(n, 10 chi) = 10240
+columns wider than theta's rows, which the device refuses as the reference's
+rsvd cannot form that sketch's thin Q either, so config 4's 12 x 12 size is
+not run.
 """
 import math
 
@@ -41,6 +46,46 @@ def _grid(L, D, rng, signed):
     return ts, ixs
 
 
+def _row_exact(ts, L):
+    """Exact contraction of the _grid network by row transfer: the state
+    carries the open vertical bonds below the finished rows (and one
+    horizontal bond inside the current row); rescaled per row. Returns
+    (ln|Z|, sign Z)."""
+    psi, lnscale = np.ones(()), 0.0
+    for r in range(L):
+        for c in range(L):
+            ax = [a for a, on in (("u", r > 0), ("d", r < L - 1), ("l", c > 0), ("r", c < L - 1)) if on]
+            nd = c if r < L - 1 else 0          # finished down legs of this row
+            pa, ta = [], []
+            if c > 0:
+                pa.append(nd)
+                ta.append(ax.index("l"))
+            if r > 0:
+                pa.append(nd + (1 if c > 0 else 0))
+                ta.append(ax.index("u"))
+            out = np.tensordot(psi, ts[r * L + c], axes=(pa, ta))
+            rest = [a for i, a in enumerate(ax) if i not in ta]
+            nu = (L - c - 1) if r > 0 else 0    # up legs still open
+            order = list(range(nd))
+            order += [nd + nu + rest.index(a) for a in ("d", "r") if a in rest]
+            order += list(range(nd, nd + nu))
+            psi = np.transpose(out, order)
+        s = float(np.max(np.abs(psi)))
+        psi = psi / s
+        lnscale += math.log(s)
+    v = float(psi)
+    return lnscale + math.log(abs(v)), math.copysign(1.0, v)
+
+
+@pytest.mark.parametrize("L", [3, 4])
+@pytest.mark.parametrize("signed", [False, True])
+def test_row_transfer_is_the_einsum_contraction(L, signed):
+    ts, ixs = _grid(L, 4, np.random.default_rng(L + signed), signed)
+    ref = _exact(ts, ixs)
+    lz, sg = _row_exact(ts, L)
+    assert abs(lz - math.log(abs(ref))) <= 1e-12 * abs(lz) and sg == math.copysign(1.0, ref)
+
+
 def _exact(ts, ixs):
     ids = {}
     args = []
@@ -59,3 +104,38 @@ def test_dense_lattice_4x4(env, signed, dtype, tol):
     lnz, err, psi = catn.contraction(tn)
     assert abs(lnz - math.log(abs(ref))) <= tol
     assert psi == math.copysign(1.0, ref)
+
+
+def _run_grid(catn, ctx, L, dtype, signed, lanes=0):
+    ts, ixs = _grid(L, 4, np.random.default_rng(L + signed), signed)
+    saved = ctx.launch()
+    if lanes:
+        ctx.set_launch(lanes, 20352, 16, 16)
+    try:
+        tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=256, chi=1024, dtype=dtype,
+                                 arena_elems=1 << 26, scratch_elems=1 << 26)
+        lnz, err, psi = catn.contraction(tn)
+    finally:
+        ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
+    return (lnz, psi), _row_exact(ts, L)
+
+
+@pytest.mark.parametrize("signed", [False, True])
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32x6", 2e-6)])
+def test_dense_lattice_6x6(env, dtype, tol, signed):
+    """6 x 6 against the exact row-transfer contraction (absolute lnZ
+    tolerance; lnZ is 40..60). Measured with six passes: 1.5e-7 (positive),
+    2e-7 (signed)."""
+    catn, ctx = env
+    (lnz, psi), (ref, sg) = _run_grid(catn, ctx, 6, dtype, signed)
+    assert abs(lnz - ref) <= tol and psi == sg
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("signed", [False, True])
+def test_dense_lattice_8x8_gpu(catn, gpu_ctx, signed):
+    """8 x 8, six-pass FP32 storage, one 1024-lane workgroup (9.3 s: one split
+    of min(m, n) > 256 takes 8.5 s), against the exact row-transfer
+    contraction (lnZ 78..111). Measured: 4e-7 (positive), 1.2e-6 (signed)."""
+    (lnz, psi), (ref, sg) = _run_grid(catn, gpu_ctx, 8, "f32x6", signed, lanes=1024)
+    assert abs(lnz - ref) <= 5e-6 and psi == sg
