@@ -10,9 +10,8 @@ cancellations of the signed variant: 2.9e-5 measured). Measured on the
 MI355X (one 1024-lane workgroup): 5 x 5 in 0.03 s, 6 x 6 in 0.1 s, 8 x 8 in
 9.3 s (one split of min(m, n) > 256 takes 8.5 s of it). From 10 x 10 a swap
 reaches swap!'s randomized-SVD switch with a sketch of min(n, 10 chi) = 10240
-columns wider than theta's rows, which the device refuses as the reference's
-rsvd cannot form that sketch's thin Q either, so config 4's 12 x 12 size is
-not run.
+columns wider than theta's rows, which the device's range finder refuses (it
+needs p <= m), so config 4's 12 x 12 size is not run.
 """
 import math
 
