@@ -96,8 +96,9 @@ struct ZSvd {
 //   w_p' = c w_p - s e^{-i phi} w_q,  w_q' = s w_p + c e^{-i phi} w_q,
 // the same unitary 2 x 2 transform applied to V's columns, so W V^H = A
 // throughout. Skip tests as in the real Jacobi (|gamma| <= 4 eps sqrt(m)
-// sqrt(alpha beta), columns below 1e-36 ||A||_F^2). Returns r = min(m, n), or
-// -1 (scratch exhausted, or 60 sweeps without convergence).
+// sqrt(alpha beta), or the FP32-storage floor g.jac_tol; columns below
+// 1e-36 ||A||_F^2). Returns r = min(m, n), or -1 (scratch exhausted, or 60
+// sweeps without convergence).
 CATN_DN int zsvd(const WG& g, Scratch& sc, const zd* A, int m, int n, int64_t lda, ZSvd& out) {
   const bool tr = m < n;
   const int tm = tr ? n : m, tn = tr ? m : n;
@@ -120,9 +121,13 @@ CATN_DN int zsvd(const WG& g, Scratch& sc, const zd* A, int m, int n, int64_t ld
     V[idx] = zmk((idx % tn) == (idx / tn) ? 1.0 : 0.0, 0.0);
   const double fro2 = wg_sum(g, part);   // its barrier also publishes W and V
   const double thr = 1e-36 * fro2;
-  const double tol = 4.0 * 2.220446049250313e-16 * sqrt((double)(tm > 1 ? tm : 1));
+  const double tol = fmax(4.0 * 2.220446049250313e-16 * sqrt((double)(tm > 1 ? tm : 1)), g.jac_tol);
   const int N = tn + (tn & 1), npairs = N / 2;
+  // lanes per column pair: the workgroup split over the pairs, and at least
+  // the global Jacobi's lanes per pair (g.jac_lanes) so that a long pair is
+  // read by consecutive lanes (coalesced) and the round's pairs go in passes
   int L = pow2floor(g.nthr / (npairs > 0 ? npairs : 1));
+  if (L < g.jac_lanes) L = pow2floor(g.jac_lanes < g.nthr ? g.jac_lanes : g.nthr);
   if (L > 64) L = 64;
   const int ngr = g.nthr / L, grp = g.tid / L, lane = g.tid % L;
   int sw = tn < 2 ? 0 : -1;
