@@ -151,3 +151,22 @@ def test_lift_zero_is_one_run_and_negative_is_refused(catn, emu_ctx):
     lz, _, _ = catn.gradient_batch([tn])
     dw, dh, _ = catn.ising_gradient(tn, len(edges), n)
     assert a[0] == lz[0] and np.array_equal(a[1], dw) and np.array_equal(a[2], dh)
+
+
+@pytest.mark.gpu
+def test_reverse_sweep_on_wide_workgroups(catn, gpu_ctx):
+    """The reverse sweep on the 1024-lane kernel (exec_backward_wide) gives the
+    256-lane kernel's gradient to rounding (n = 40 spin glass, exact
+    contraction, lifted)."""
+    n, edges, w, h = Cf.spin_glass_3reg(40, seed=3)
+    out = {}
+    saved = gpu_ctx.launch()
+    try:
+        for lanes in (256, 1024):
+            gpu_ctx.set_launch(lanes, 20352 if lanes > 256 else 0, 16, 16)
+            out[lanes] = catn.lnz_and_grad(n, edges, w, h, 1.0, ctx=gpu_ctx, **EXACT)
+    finally:
+        gpu_ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
+    (l0, w0, h0), (l1, w1, h1) = out[256], out[1024]
+    assert abs(l1 - l0) <= 1e-12 * abs(l0)
+    assert np.abs(w1 - w0).max() < 1e-10 and np.abs(h1 - h0).max() < 1e-10
