@@ -803,7 +803,11 @@ inline bool contract_nets_grouped(Executor* ex, const std::vector<catn_net*>& ne
   // queue and serialise two groups)
   std::vector<std::unique_ptr<Executor>> sib;
   for (int g = 1; g < G; ++g) {
-    sib.emplace_back(ex->sibling());
+    try {
+      sib.emplace_back(ex->sibling());
+    } catch (const Error&) {
+      return false;   // no second stream on this device: the one-stream path
+    }
     if (!sib.back()) return false;
     Executor* s = sib.back().get();
     s->parent = ex;
