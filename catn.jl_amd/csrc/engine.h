@@ -1163,6 +1163,9 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
 #undef CATN_JR8
           continue;
         }
+        // streamed pair (rows beyond the registers): unrolled by 4 so that
+        // four rows' loads are in flight per lane (the sums keep their order)
+#pragma unroll 4
         for (int i = lane; i < m; i += L) {
           const double x = ap[i], y = aq[i];
           al += x * x; be += y * y; ga += x * y;
@@ -1174,11 +1177,13 @@ CATN_D int jacobi_core(const WG& g, double* A, int m, int n, int64_t lda, double
         const double zeta = (be - al) / (2.0 * ga);
         const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+#pragma unroll 4
         for (int i = lane; i < m; i += L) {
           const double x = ap[i], y = aq[i];
           ap[i] = c * x - s * y;
           aq[i] = s * x + c * y;
         }
+#pragma unroll 4
         for (int i = lane; i < n; i += L) {
           const double x = vp[i], y = vq[i];
           vp[i] = c * x - s * y;
