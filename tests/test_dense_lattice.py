@@ -8,7 +8,7 @@ against einsum at 3 x 3 and 4 x 4. Tolerances on lnZ at 4 x 4: FP64 1e-12,
 six-pass 2e-6, three-pass 1e-4 (its 2^-16 products meet the sign
 cancellations of the signed variant: 2.9e-5 measured). Measured on the
 MI355X (one 1024-lane workgroup): 5 x 5 in 0.03 s, 6 x 6 in 0.1 s, 8 x 8 in
-9.3 s (one split of min(m, n) > 256 takes 8.5 s of it). From 10 x 10 a swap
+7.9 s (mostly one split of min(m, n) > 256). From 10 x 10 a swap
 reaches swap!'s randomized-SVD switch with a sketch of min(n, 10 chi) = 10240
 columns wider than theta's rows, which the device's range finder refuses (it
 needs p <= m), so config 4's 12 x 12 size is not run.
@@ -133,8 +133,8 @@ def test_dense_lattice_6x6(env, dtype, tol, signed):
 @pytest.mark.gpu
 @pytest.mark.parametrize("signed", [False, True])
 def test_dense_lattice_8x8_gpu(catn, gpu_ctx, signed):
-    """8 x 8, six-pass FP32 storage, one 1024-lane workgroup (9.3 s: one split
-    of min(m, n) > 256 takes 8.5 s), against the exact row-transfer
+    """8 x 8, six-pass FP32 storage, one 1024-lane workgroup (7.9 s, mostly one
+    split of min(m, n) > 256), against the exact row-transfer
     contraction (lnZ 78..111). Measured: 4e-7 (positive), 1.2e-6 (signed)."""
     (lnz, psi), (ref, sg) = _run_grid(catn, gpu_ctx, 8, "f32x6", signed, lanes=1024)
     assert abs(lnz - ref) <= 5e-6 and psi == sg
