@@ -36,11 +36,13 @@ import time
 os.environ.setdefault("OPENBLAS_NUM_THREADS", str(min(os.cpu_count() or 1, 16)))
 # hardware queues of the HIP runtime (read when it initialises; the runtime's
 # default is 4): config 3's batch runs its networks in concurrent groups, one
-# stream each, and a stream per queue: 5.9 s with 8 queues and 6 streams, 7.4 s
-# with 4 and 4, 9.7 s with 6 streams sharing 4 queues
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
-C3_STREAMS = min(6, int(os.environ["GPU_MAX_HW_QUEUES"]))
+# stream each, and a stream per queue. Measured on the MI355X: 4 streams on 4
+# queues 7.4 s (before the Jacobi floor and unrolling); with 16 queues, 6 / 8 /
+# 10 / 12 / 14 / 16 streams 5.17 / 6.08 / 5.43 / 4.72 / 5.38 / 5.02 s (the
+# dealing of the largest lattices into groups moves the time non-monotonically)
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+C3_STREAMS = min(12, int(os.environ["GPU_MAX_HW_QUEUES"]))
 
 import numpy as np
 
@@ -722,8 +724,8 @@ def run_config3(catn, ctx, steps):
     out = {
         "workload": "config 3: 64 open LxL +-J lattices, L uniform in {8,...,24}, h=0, beta "
                     "uniform in [0.2,1.2], Dmax=32, chi=128, FP32 storage with six-pass split-BF16 "
-                    "swap GEMMs, one batch (one 1024-lane workgroup per lattice, 6 concurrent "
-                    "groups of lattices on 6 streams); forward lnZ only",
+                    "swap GEMMs, one batch (one 1024-lane workgroup per lattice, concurrent "
+                    f"groups of lattices on {C3_STREAMS} streams); forward lnZ only",
         "seconds_per_step": float(np.mean(times)),
         "lnz_solves_per_s": 64 / float(np.mean(times)),
         "steps": steps,
