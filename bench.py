@@ -204,8 +204,13 @@ def run_cut_free(catn, ctx, steps, warmup, batch):
     plain_shape = ctx.launch()
 
     def chosen_batch():
-        return catn.ising_network_batch([(n, edges, wp, hp), (n, edges, wm, hm)],
+        # the reverse sweep's tapes and adjoint arenas are allocated here, before
+        # the timed region, as the plain batch's arenas are
+        nets = catn.ising_network_batch([(n, edges, wp, hp), (n, edges, wm, hm)],
                                         [(0, BETA), (1, BETA)] * batch, ctx=ctx, **kw)
+        for t in nets:
+            t.prepare_gradient()
+        return nets
     tuned = catn.autotune(ctx, chosen_batch, run=lambda nets: catn.gradient_batch(nets)[0])
     chosen_shape = (tuned["threads"], tuned["lds_doubles"], tuned["panel"], tuned["jacobi_lanes"])
     log("cut-free chosen autotune: " + ", ".join(

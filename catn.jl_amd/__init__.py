@@ -83,6 +83,7 @@ _SIGS = {
     "catn_net_add_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I32P, _DP, _I32P, C.c_int32]),
     "catn_net_add_tensor_node": (C.c_int, [_P, C.c_int32, C.c_int32, _I64P, _DP, _I32P]),
     "catn_net_upload": (C.c_int, [_P]),
+    "catn_net_prepare_gradient": (C.c_int, [_P, C.c_int32]),
     "catn_net_set_fault": (C.c_int, [_P, C.c_int32]),
     "catn_net_free": (C.c_int, [_P]),
     "catn_contraction": (C.c_int, [_P, _DP, _DP, _DP]),
@@ -414,6 +415,11 @@ class TensorNetwork:
         out = np.zeros(size)
         self.ctx._check(self.ctx.lib.catn_net_site_adjoint(self.h, int(slot), _dptr(out), size))
         return out
+
+    def prepare_gradient(self, tape_cap: int = 0):
+        """Allocate the reverse sweep's tape and adjoint arena now, so that a
+        later gradient_batch (same tape_cap) allocates nothing."""
+        self.ctx._check(self.ctx.lib.catn_net_prepare_gradient(self.h, int(tape_cap)))
 
     def set_fault(self, flags: int):
         """Test hook of tsvd's fallback: flags = 1 makes every SVD of this

@@ -239,10 +239,10 @@ struct catn_net {
   }
 
   // switch on the tape for the coming contraction (before any op has run)
-  void enable_record(int cap) {
+  // the reverse sweep's device memory: a tape of `cap` records and the
+  // adjoint arena (allocated once, kept for a later gradient run)
+  void alloc_record(int cap) {
     ensure_device();
-    if (hs.tape_len != 0 || !pl.prog.empty())
-      throw catn::Error(-5, "gradient: the network has already executed device ops");
     catn::Executor* ex = ctx->ex.get();
     if (!hs.tape || hs.tape_cap < cap) {
       if (hs.tape) ex->dfree(hs.tape);
@@ -250,6 +250,13 @@ struct catn_net {
       hs.tape_cap = cap;
     }
     if (!hs.adj) hs.adj = (double*)ex->dmalloc(sizeof(double) * (size_t)arena_cap);
+  }
+
+  void enable_record(int cap) {
+    ensure_device();
+    if (hs.tape_len != 0 || !pl.prog.empty())
+      throw catn::Error(-5, "gradient: the network has already executed device ops");
+    alloc_record(cap);
     hs.record = 1;
   }
 
@@ -850,6 +857,9 @@ inline bool contract_nets_grouped(Executor* ex, const std::vector<catn_net*>& ne
   return true;
 }
 
+// tape records per network when the caller gives none
+constexpr int kDefaultTapeCap = 1 << 16;
+
 // contraction! with the tape on, then one reverse-sweep launch for all
 inline void gradient_nets(Executor* ex, const std::vector<catn_net*>& nets, int tape_cap) {
   for (catn_net* n : nets) {
@@ -1287,6 +1297,16 @@ int catn_net_set_fault(catn_net* net, int32_t flags) {
   CATN_CATCH
 }
 
+int catn_net_prepare_gradient(catn_net* net, int32_t tape_cap) {
+  CATN_TRY
+  if (tape_cap < 0) throw catn::Error(-2, "prepare_gradient: tape_cap must be >= 0");
+  if (net->cplx)
+    throw catn::Error(-2, "prepare_gradient: the reverse sweep is built for real networks");
+  net->alloc_record(tape_cap > 0 ? tape_cap : catn::kDefaultTapeCap);
+  return 0;
+  CATN_CATCH
+}
+
 int catn_net_upload(catn_net* net) {
   CATN_TRY
   net->ensure_device();
@@ -1339,7 +1359,7 @@ int catn_gradient_batch(catn_ctx* ctx, int32_t count, catn_net* const* nets, int
                         double* lnZ, double* error, double* psi) {
   CATN_TRY
   std::vector<catn_net*> v(nets, nets + count);
-  catn::gradient_nets(ctx->ex.get(), v, tape_cap > 0 ? tape_cap : (1 << 16));
+  catn::gradient_nets(ctx->ex.get(), v, tape_cap > 0 ? tape_cap : catn::kDefaultTapeCap);
   for (int k = 0; k < count; ++k) {
     if (lnZ) lnZ[k] = v[k]->hs.lnZ + v[k]->hs.lntail;
     if (error) error[k] = v[k]->hs.err;

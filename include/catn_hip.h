@@ -141,6 +141,11 @@ int catn_net_add_tensor_node(catn_net* net, int32_t id, int32_t ndim, const int6
 /* Allocate the network's device state and upload its site tensors now
  * (otherwise done lazily by the first call that runs on the device). */
 int catn_net_upload(catn_net* net);
+/* Allocate the network's reverse-sweep memory now: a tape of tape_cap records
+ * (0: the default of catn_gradient_batch) and the adjoint arena, so that a
+ * later catn_gradient_batch with the same tape_cap allocates nothing. No
+ * counterpart in the reference (its AD allocates inside the pullback). */
+int catn_net_prepare_gradient(catn_net* net, int32_t tape_cap);
 /* Test hook of tsvd's fallback (src/linalg_utils.jl:16-21: svd() retried with
  * the QR-iteration driver on a LAPACKException). flags bit 0: every SVD of
  * this network treats its primary path (QR-preconditioned one-sided Jacobi)

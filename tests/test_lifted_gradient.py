@@ -170,3 +170,22 @@ def test_reverse_sweep_on_wide_workgroups(catn, gpu_ctx):
     (l0, w0, h0), (l1, w1, h1) = out[256], out[1024]
     assert abs(l1 - l0) <= 1e-12 * abs(l0)
     assert np.abs(w1 - w0).max() < 1e-10 and np.abs(h1 - h0).max() < 1e-10
+
+
+def test_prepared_gradient_memory_gives_the_same_gradient(catn, emu_ctx):
+    """catn_net_prepare_gradient allocates the tape and adjoint arena ahead of
+    the reverse sweep: the gradient is the same, a prepared network still runs
+    a plain contraction, and a negative tape capacity is refused."""
+    n, edges, w, h = Cf.spin_glass_3reg(12, seed=2)
+    a = catn.ising_network(n, edges, w, h, 1.0, ctx=emu_ctx, **EXACT)
+    b = catn.ising_network(n, edges, w, h, 1.0, ctx=emu_ctx, **EXACT)
+    b.prepare_gradient()
+    la, _, _ = catn.gradient_batch([a])
+    lb, _, _ = catn.gradient_batch([b])
+    assert la[0] == lb[0]
+    assert np.array_equal(catn.ising_gradient(a, len(edges), n)[0], catn.ising_gradient(b, len(edges), n)[0])
+    c = catn.ising_network(n, edges, w, h, 1.0, ctx=emu_ctx, **EXACT)
+    c.prepare_gradient(1000)
+    assert catn.contraction(c)[0] == la[0]
+    with pytest.raises(catn.CatnError, match="tape_cap"):
+        catn.ising_network(n, edges, w, h, 1.0, ctx=emu_ctx, **EXACT).prepare_gradient(-1)
