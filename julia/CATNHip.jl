@@ -17,7 +17,7 @@ using ChainRulesCore
 
 export ising_network, tensor_network, braket_network, contraction!, free_energy,
        magnetization, correlation, ising_lnZ, ising_lnZ_batch, result_tensor, set_launch!,
-       set_batch_streams!
+       set_batch_streams!, prepare_gradient!
 
 const lib = get(ENV, "CATN_HIP_LIB", joinpath(@__DIR__, "..", "catn.jl_amd", "lib", "libcatn_hip.so"))
 
@@ -345,6 +345,11 @@ end
 set_launch!(threads=256, lds=0, panel=0, jacobi_lanes=0) =
     check(ccall((:catn_set_launch, lib), Cint, (Ptr{Cvoid}, Int32, Int32, Int32, Int32), ctx(),
                 threads, lds, panel, jacobi_lanes))
+
+# allocate a network's reverse-sweep memory (tape, adjoint arena) ahead of
+# lnz_and_grad! (tape_cap 0: the library default)
+prepare_gradient!(tn::HipNetwork, tape_cap=0) =
+    check(ccall((:catn_net_prepare_gradient, lib), Cint, (Ptr{Cvoid}, Int32), tn.ptr, tape_cap))
 
 # concurrent groups (streams) of a forward batch of distinct networks, 1..16 (default 4)
 set_batch_streams!(streams=4) =
