@@ -482,6 +482,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--config3-steps", type=int, default=1,
                     help="timed steps of config 3's full-size batched forward (0: skip)")
+    ap.add_argument("--config4", type=int, default=1,
+                    help="run config 4's dense grid at 8x8 as a forward leg (0: skip)")
+    ap.add_argument("--config4-only", action="store_true",
+                    help="only config 4's 8x8 forward leg and its exact comparison, for profiling")
     ap.add_argument("--config3-only", action="store_true",
                     help="only config 3's batched forward and its oracle comparison, for profiling")
     ap.add_argument("--cut-free-batch", type=int, default=128,
@@ -544,6 +548,9 @@ def main():
         print(json.dumps(run_config3(catn, ctx, max(args.config3_steps, 1))),
               flush=True)
         return
+    if args.config4_only:
+        print(json.dumps(run_config4(catn, ctx)), flush=True)
+        return
     if args.cut_free_only:
         print(json.dumps(run_cut_free(catn, ctx, args.steps, args.warmup, args.cut_free_batch)),
               flush=True)
@@ -583,6 +590,8 @@ def main():
         out_cfg["cut_free_comparison"] = run_cut_free(catn, ctx, 3, 1, args.cut_free_batch)
     if args.form == "both" and args.config3_steps > 0:
         out_cfg["config3_forward"] = run_config3(catn, ctx, args.config3_steps)
+    if args.form == "both" and args.config4:
+        out_cfg["config4_forward"] = run_config4(catn, ctx)
 
     # FP64 oracle lnZ of a few instances at the same (Dmax, chi)
     ncheck = 2
@@ -697,6 +706,47 @@ def config3_instances(seed=2025, count=64):
 
 
 C3_ORACLE = os.path.join(ROOT, "tests", "golden", "config3_oracle.json")
+
+
+def run_config4(catn, ctx, L=8):
+    """Config 4's dense-regime network at the largest size that runs: an open
+    L x L grid of bond-4 tensors (positive and signed entries,
+    oracle/configs.py dense_lattice, seeds L and L + 1), Dmax = 256,
+    chi = 1024, FP32 storage with six-pass split-BF16 swap products, one
+    1024-lane workgroup per network. Each network is built and uploaded before
+    its timed contraction; lnZ is compared with the exact row-transfer
+    contraction (oracle/exact.py grid_lnZ), computed outside the timed region.
+    From 10 x 10 a swap reaches the randomized-SVD switch with a sketch wider
+    than theta's rows, which the device refuses."""
+    from oracle import configs as Cf
+    from oracle import exact as X
+    saved = ctx.launch()
+    ctx.set_launch(1024, 20352, 16, 16)
+    rows = []
+    try:
+        for signed in (False, True):
+            ts, ixs = Cf.dense_lattice(L, 4, seed=L + signed, signed=signed)
+            tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=256, chi=1024, dtype="f32x6",
+                                     arena_elems=1 << 26, scratch_elems=1 << 26)
+            _prefetch([tn])
+            t0 = time.perf_counter()
+            lz, err, psi = catn.contraction(tn)
+            dt = time.perf_counter() - t0
+            ref, ph = X.grid_lnZ(ts, L)
+            rows.append({"variant": "signed N(0,1)" if signed else "positive uniform[0,1)",
+                         "seconds": dt, "lnZ": lz, "exact_lnZ": ref, "abs_dlnZ": abs(lz - ref),
+                         "sign_matches": bool(psi == math.copysign(1.0, ph.real))})
+            log(f"config 4 {L}x{L} {rows[-1]['variant']}: {dt:.2f} s, |dlnZ| {abs(lz - ref):.1e}")
+            tn.free()
+    finally:
+        ctx.set_launch(saved[0], saved[1], saved[3], saved[4])
+    return {
+        "workload": f"config 4 at {L}x{L}: open grid, bond 4, Dmax=256, chi=1024, FP32 storage "
+                    "with six-pass split-BF16 swap GEMMs, one 1024-lane workgroup per network; "
+                    "12x12 not reached (randomized-SVD sketch wider than theta from 10x10)",
+        "networks": rows,
+        "max_abs_dlnZ_vs_exact": max(r["abs_dlnZ"] for r in rows),
+    }
 
 
 def run_config3(catn, ctx, steps):

@@ -6,7 +6,10 @@ oracle tests, the HIP parity tests and the benchmark (pure data generation).
   * `spin_glass_3reg`— random 3-regular graph by the pairing model (seeded),
                        w uniform +-1, h ~ N(0, 0.1^2), edges sorted (i<j);
   * `batch_instances`— +-J square-lattice spin glasses, L uniform in {8,...,24},
-                       beta uniform in [0.2, 1.2], h = 0.
+                       beta uniform in [0.2, 1.2], h = 0;
+  * `dense_lattice`  — the dense-regime network: an open LxL grid of tensors
+                       with bond D on every edge (legs up, down, left, right
+                       where present), entries uniform [0, 1) or N(0, 1).
 """
 from __future__ import annotations
 
@@ -74,3 +77,25 @@ def batch_instances(count=64, seed=0, Ls=(8, 10, 12, 14, 16, 18, 20, 22, 24),
         beta = float(rng.uniform(*beta_range))
         out.append((L, L * L, edges, w, np.zeros(L * L), beta))
     return out
+
+
+def dense_lattice(L, D=4, seed=0, signed=False):
+    """(tensors, ixs) of the open L x L dense-regime grid, row-major sites,
+    each with its present legs in the order up, down, left, right."""
+    rng = np.random.default_rng(seed)
+    ts, ixs = [], []
+    for r in range(L):
+        for c in range(L):
+            ix = []
+            if r > 0:
+                ix.append(f"v{r - 1}_{c}")
+            if r < L - 1:
+                ix.append(f"v{r}_{c}")
+            if c > 0:
+                ix.append(f"h{r}_{c - 1}")
+            if c < L - 1:
+                ix.append(f"h{r}_{c}")
+            shape = [D] * len(ix)
+            ts.append(rng.standard_normal(shape) if signed else rng.uniform(0.0, 1.0, shape))
+            ixs.append(ix)
+    return ts, ixs

@@ -8,7 +8,10 @@ Independent of the CATN algorithm; these pin the oracle and the HIP path:
                        arbitrary per-edge couplings and fields (L <= 24);
   * `elimination_lnZ`— variable elimination with a min-fill order on any graph;
   * `exact_contract` — dense einsum of a labelled network (test/exact.jl:4-19);
-  * `brute_observables` — <s_k> and <s_i s_j> by enumeration (test/test_ising.jl:107-118).
+  * `brute_observables` — <s_k> and <s_i s_j> by enumeration (test/test_ising.jl:107-118);
+  * `grid_lnZ`       — exact contraction of an open LxL grid of tensors
+                       (configs.dense_lattice layout) by row transfer, D^(L+1)
+                       entries, for any real or complex entries.
 All sums are done in scaled form (log-sum-exp) so large lattices do not overflow.
 """
 from __future__ import annotations
@@ -177,3 +180,35 @@ def exact_contract(tensors, ixs):
     subs = ",".join("".join(letters[l] for l in ix) for ix in ixs)
     out = "".join(letters[l] for l in orderl if counts[l] == 1)
     return np.einsum(subs + "->" + out, *tensors, optimize="greedy")
+
+
+def grid_lnZ(ts, L):
+    """Exact contraction of an open L x L grid (configs.dense_lattice: site
+    r * L + c, present legs up, down, left, right) by row transfer: the state
+    carries the open vertical bonds below the finished rows and one
+    horizontal bond inside the current row, rescaled per row. Returns
+    (ln|Z|, Z / |Z|)."""
+    psi, lnscale = np.ones(()), 0.0
+    for r in range(L):
+        for c in range(L):
+            ax = [a for a, on in (("u", r > 0), ("d", r < L - 1), ("l", c > 0), ("r", c < L - 1)) if on]
+            nd = c if r < L - 1 else 0          # finished down legs of this row
+            pa, ta = [], []
+            if c > 0:
+                pa.append(nd)
+                ta.append(ax.index("l"))
+            if r > 0:
+                pa.append(nd + (1 if c > 0 else 0))
+                ta.append(ax.index("u"))
+            out = np.tensordot(psi, ts[r * L + c], axes=(pa, ta))
+            rest = [a for i, a in enumerate(ax) if i not in ta]
+            nu = (L - c - 1) if r > 0 else 0    # up legs still open
+            order = list(range(nd))
+            order += [nd + nu + rest.index(a) for a in ("d", "r") if a in rest]
+            order += list(range(nd, nd + nu))
+            psi = np.transpose(out, order)
+        s = float(np.max(np.abs(psi)))
+        psi = psi / s
+        lnscale += math.log(s)
+    v = complex(psi.item()) if np.iscomplexobj(psi) else float(psi)
+    return lnscale + math.log(abs(v)), v / abs(v)

@@ -18,6 +18,9 @@ import math
 import numpy as np
 import pytest
 
+from oracle import configs as Cf
+from oracle import exact as X
+
 
 @pytest.fixture(scope="module", params=["emu", pytest.param("gpu", marks=pytest.mark.gpu)])
 def env(request, catn):
@@ -26,63 +29,32 @@ def env(request, catn):
     return catn, request.getfixturevalue("gpu_ctx")
 
 
-def _grid(L, D, rng, signed):
-    ts, ixs = [], []
-    for r in range(L):
-        for c in range(L):
-            ix = []
-            if r > 0:
-                ix.append(f"v{r - 1}_{c}")
-            if r < L - 1:
-                ix.append(f"v{r}_{c}")
-            if c > 0:
-                ix.append(f"h{r}_{c - 1}")
-            if c < L - 1:
-                ix.append(f"h{r}_{c}")
-            shape = [D] * len(ix)
-            ts.append(rng.standard_normal(shape) if signed else rng.uniform(0.0, 1.0, shape))
-            ixs.append(ix)
-    return ts, ixs
+def _grid(L, D, seed, signed):
+    return Cf.dense_lattice(L, D, seed=seed, signed=signed)
 
 
 def _row_exact(ts, L):
-    """Exact contraction of the _grid network by row transfer: the state
-    carries the open vertical bonds below the finished rows (and one
-    horizontal bond inside the current row); rescaled per row. Returns
-    (ln|Z|, sign Z)."""
-    psi, lnscale = np.ones(()), 0.0
-    for r in range(L):
-        for c in range(L):
-            ax = [a for a, on in (("u", r > 0), ("d", r < L - 1), ("l", c > 0), ("r", c < L - 1)) if on]
-            nd = c if r < L - 1 else 0          # finished down legs of this row
-            pa, ta = [], []
-            if c > 0:
-                pa.append(nd)
-                ta.append(ax.index("l"))
-            if r > 0:
-                pa.append(nd + (1 if c > 0 else 0))
-                ta.append(ax.index("u"))
-            out = np.tensordot(psi, ts[r * L + c], axes=(pa, ta))
-            rest = [a for i, a in enumerate(ax) if i not in ta]
-            nu = (L - c - 1) if r > 0 else 0    # up legs still open
-            order = list(range(nd))
-            order += [nd + nu + rest.index(a) for a in ("d", "r") if a in rest]
-            order += list(range(nd, nd + nu))
-            psi = np.transpose(out, order)
-        s = float(np.max(np.abs(psi)))
-        psi = psi / s
-        lnscale += math.log(s)
-    v = float(psi)
-    return lnscale + math.log(abs(v)), math.copysign(1.0, v)
+    lz, ph = X.grid_lnZ(ts, L)
+    return lz, math.copysign(1.0, ph.real)
 
 
 @pytest.mark.parametrize("L", [3, 4])
 @pytest.mark.parametrize("signed", [False, True])
 def test_row_transfer_is_the_einsum_contraction(L, signed):
-    ts, ixs = _grid(L, 4, np.random.default_rng(L + signed), signed)
+    """The oracle's row transfer (oracle/exact.py grid_lnZ) against the dense
+    einsum at 3 x 3 and 4 x 4, real and complex entries."""
+    ts, ixs = _grid(L, 4, L + signed, signed)
     ref = _exact(ts, ixs)
     lz, sg = _row_exact(ts, L)
     assert abs(lz - math.log(abs(ref))) <= 1e-12 * abs(lz) and sg == math.copysign(1.0, ref)
+    zs = [t * np.exp(0.3j * t) for t in ts]
+    ids = {}
+    args = []
+    for t, ix in zip(zs, ixs):
+        args += [t, [ids.setdefault(l, len(ids)) for l in ix]]
+    zref = complex(np.einsum(*args, [], optimize="greedy"))
+    lzc, ph = X.grid_lnZ(zs, L)
+    assert abs(lzc - math.log(abs(zref))) <= 1e-12 * abs(lzc) and abs(ph - zref / abs(zref)) < 1e-12
 
 
 def _exact(ts, ixs):
@@ -97,7 +69,7 @@ def _exact(ts, ixs):
 @pytest.mark.parametrize("dtype,tol", [("f64", 1e-12), ("f32x6", 2e-6), ("f32", 1e-4)])
 def test_dense_lattice_4x4(env, signed, dtype, tol):
     catn, ctx = env
-    ts, ixs = _grid(4, 4, np.random.default_rng(4 + signed), signed)
+    ts, ixs = _grid(4, 4, 4 + signed, signed)
     ref = _exact(ts, ixs)
     tn = catn.tensor_network(ts, ixs, ctx=ctx, Dmax=256, chi=1024, dtype=dtype)
     lnz, err, psi = catn.contraction(tn)
@@ -106,7 +78,7 @@ def test_dense_lattice_4x4(env, signed, dtype, tol):
 
 
 def _run_grid(catn, ctx, L, dtype, signed, lanes=0):
-    ts, ixs = _grid(L, 4, np.random.default_rng(L + signed), signed)
+    ts, ixs = _grid(L, 4, L + signed, signed)
     saved = ctx.launch()
     if lanes:
         ctx.set_launch(lanes, 20352, 16, 16)
