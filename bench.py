@@ -764,7 +764,7 @@ def run_config3(catn, ctx, steps):
     # on 16 wavefronts than on 4 (26.7 s -> 12.6 s; 512 lanes: 16.5 s)
     ctx.set_launch(1024, 20352, 16, 16)
     ctx.set_batch_streams(C3_STREAMS)
-    times, lz = [], None
+    times, lz, flops = [], None, 0.0
     for k in range(steps):
         nets = catn.ising_network_batch([(n, e, w, h) for (n, e, w, h, _) in inst],
                                         [(q, b) for q, (*_, b) in enumerate(inst)], ctx=ctx,
@@ -774,6 +774,7 @@ def run_config3(catn, ctx, steps):
         lz = catn.contraction_batch(nets)[0]
         times.append(time.perf_counter() - t0)
         log(f"config 3 step {k}: {times[-1]:.2f} s (64 lattices, FP32 storage)")
+        flops = sum(t.work()[0] for t in nets)
         for t in nets:
             t.free()
     out = {
@@ -782,6 +783,7 @@ def run_config3(catn, ctx, steps):
                     "swap GEMMs, one batch (one 1024-lane workgroup per lattice, concurrent "
                     f"groups of lattices on {C3_STREAMS} streams); forward lnZ only",
         "seconds_per_step": float(np.mean(times)),
+        "algorithmic_flops_per_step": float(flops),
         "lnz_solves_per_s": 64 / float(np.mean(times)),
         "steps": steps,
         "chosen_form": "refused by the device (reverse sweep across a Dmax cut)",
