@@ -91,6 +91,7 @@ _SIGS = {
     "catn_gradient_batch": (C.c_int, [_P, C.c_int32, C.POINTER(_P), C.c_int32, _DP, _DP, _DP]),
     "catn_ising_gradient": (C.c_int, [_P, _DP, _DP, _DP]),
     "catn_net_site_adjoint": (C.c_int, [_P, C.c_int32, _DP, C.c_int64]),
+    "catn_net_tensor_adjoint": (C.c_int, [_P, C.c_int32, _DP, C.c_int64]),
     "catn_net_bwd_profile": (C.c_int, [_P, _I64P, _DP, _I32P]),
     "catn_net_tape": (C.c_int, [_P, _I32P, _DP, C.c_int32]),
     "catn_bwd_kernel_stats": (C.c_int, [_P, _I64P, _DP, C.c_int32]),
@@ -416,6 +417,19 @@ class TensorNetwork:
         self.ctx._check(self.ctx.lib.catn_net_site_adjoint(self.h, int(slot), _dptr(out), size))
         return out
 
+    def tensor_gradient(self, k: int) -> np.ndarray:
+        """After gradient_batch on a network from tensor_network: d lnZ / d T
+        for its k-th tensor (0-based), shaped like the tensor, lnZ = log|Z|."""
+        shapes = getattr(self, "tensor_shapes", None)
+        if shapes is None:
+            raise CatnError("tensor_gradient: the network was not built by tensor_network")
+        if not 0 <= k < len(shapes):
+            raise CatnError(f"tensor_gradient: no tensor {k} in this network")
+        size = int(np.prod(shapes[k], dtype=np.int64))
+        out = np.zeros(size)
+        self.ctx._check(self.ctx.lib.catn_net_tensor_adjoint(self.h, int(k), _dptr(out), size))
+        return out.reshape(shapes[k], order="F")
+
     def prepare_gradient(self, tape_cap: int = 0):
         """Allocate the reverse sweep's tape and adjoint arena now, so that a
         later gradient_batch (same tape_cap) allocates nothing."""
@@ -525,6 +539,7 @@ def tensor_network(tensors: Sequence[np.ndarray], ixs: Sequence[Sequence], *,
                                            dm.ctypes.data_as(_I64P), _dptr(dt),
                                            lb.ctypes.data_as(_I64P), C.byref(p), C.byref(out)))
     tn = TensorNetwork(ctx, out, cplx)
+    tn.tensor_shapes = [np.asarray(t).shape for t in tensors]
     # open labels in order of first appearance: the device's sentinels -1, -2, ...
     count = {}
     for ix in ixs:

@@ -398,6 +398,25 @@ CATN_DN void bwd_cano_l(const BwdCtx& c, const TapeRec& rc) {
   g.sync();
 }
 
+// raw2mps step (src/mps_node.jl:31-50): the remaining (dleft, rest) block,
+// as a (dleft di) x (rest / di) matrix, split U | S V^T by tsvd (chi, cutoff);
+// U is the new site, S V^T the rest for the next step
+CATN_DN void bwd_raw_split(const BwdCtx& c, const TapeRec& rc) {
+  Exec& E = *c.E;
+  const WG& g = E.g;
+  const TapeSite &a = rc.in[0], &ou = rc.out[0], &orr = rc.out[1];
+  const int m = ou.dl * ou.d, n = orr.d, k = rc.k;
+  SplitReplay t;
+  if (!split_replay(E, c.x(a), m, n, m, true, 0, rc.val, rc.aux0, rc.path, k, t)) return;
+  note_replay(E, fmax(wg_maxdiff(g, t.L, c.x(ou), (int64_t)m * k),
+                      wg_maxdiff(g, t.R, c.x(orr), (int64_t)k * n)));
+  double* Ab = E.sc.alloc((int64_t)m * n);
+  if (E.sc.fail) { E.status = STATUS_SCRATCH_FULL; return; }
+  if (!split_adjoint(E, t, c.a(ou), c.a(orr), Ab)) return;
+  wg_axpy(g, c.a(a), Ab, 1.0, (int64_t)m * n);
+  g.sync();
+}
+
 // swap! (src/mps_node.jl:126-178): theta = tl * tr, truncated split
 CATN_DN void bwd_swap(const BwdCtx& c, const TapeRec& rc) {
   Exec& E = *c.E;
@@ -807,6 +826,7 @@ CATN_DN int run_backward(const WG& g, InstState* st) {
       case OP_MERGE_FUSE: bwd_merge_fuse(c, rc); break;
       case OP_CUT: bwd_cut(c, rc); break;
       case OP_SCALAR_LOGNORM: bwd_scalar_lognorm(c, rc); break;
+      case OP_RAW_SPLIT: bwd_raw_split(c, rc); break;
       default: E.status = STATUS_UNSUPPORTED; break;
     }
     if (E.sc.fail && E.status == STATUS_OK) E.status = STATUS_SCRATCH_FULL;

@@ -198,6 +198,9 @@ struct catn_net {
   std::vector<double> iw, ih;
   int ipos1 = 0, ival1 = 0, ipos2 = 0, ival2 = 0;
   bool have_adjoint = false;
+  // per tensor added by catn_net_add_tensor_node / catn_tensor_network, in
+  // order: the slot whose initial version holds its data, and its size
+  std::vector<std::pair<int, int64_t>> tensor_slot;
   // final planner after contraction!, shared by the networks that followed one
   // plan (read-only); plan() reads it, own_plan() takes a private copy
   std::shared_ptr<const catn::Planner> done_pl;
@@ -241,20 +244,35 @@ struct catn_net {
   // switch on the tape for the coming contraction (before any op has run)
   // the reverse sweep's device memory: a tape of `cap` records and the
   // adjoint arena (allocated once, kept for a later gradient run)
+  // `cap` records for the contraction behind the build_tape kept ones
   void alloc_record(int cap) {
     ensure_device();
+    alloc_tape(cap + build_tape);
+    if (!hs.adj) hs.adj = (double*)ctx->ex->dmalloc(sizeof(double) * (size_t)arena_cap);
+  }
+
+  // records kept from the construction run (the raw2mps splits of a network
+  // built from tensors) at the start of the tape: the reverse sweep ends with
+  // them and so reaches the input tensors
+  int build_tape = 0;
+  bool tensors_taped = false;   // built by catn_tensor_network, splits on the tape
+
+  // a device tape of at least `total` records that keeps its first build_tape
+  void alloc_tape(int total) {
+    if (hs.tape && hs.tape_cap >= total) return;
     catn::Executor* ex = ctx->ex.get();
-    if (!hs.tape || hs.tape_cap < cap) {
-      if (hs.tape) ex->dfree(hs.tape);
-      hs.tape = (catn::TapeRec*)ex->dmalloc(sizeof(catn::TapeRec) * (size_t)cap);
-      hs.tape_cap = cap;
+    auto* t = (catn::TapeRec*)ex->dmalloc(sizeof(catn::TapeRec) * (size_t)total);
+    if (hs.tape) {
+      if (build_tape > 0) ex->d2d(t, hs.tape, sizeof(catn::TapeRec) * (size_t)build_tape);
+      ex->dfree(hs.tape);
     }
-    if (!hs.adj) hs.adj = (double*)ex->dmalloc(sizeof(double) * (size_t)arena_cap);
+    hs.tape = t;
+    hs.tape_cap = total;
   }
 
   void enable_record(int cap) {
     ensure_device();
-    if (hs.tape_len != 0 || !pl.prog.empty())
+    if (hs.tape_len != build_tape || !pl.prog.empty())
       throw catn::Error(-5, "gradient: the network has already executed device ops");
     alloc_record(cap);
     hs.record = 1;
@@ -627,14 +645,17 @@ inline void add_tensor_node(catn_net* net, int id, int nd, const int64_t* dims, 
   node.nb = nb;
   if (nd == 0) {
     // empty MPS (src/mps_node.jl:34)
+    net->tensor_slot.push_back({-1, 0});
   } else if (nd == 1) {
     const int slot = net->new_slot(1, (int)dims[0], 1, data);
+    net->tensor_slot.push_back({slot, dims[0]});
     node.slot.push_back(slot);
     node.phys.push_back((int)dims[0]);
   } else {
     int64_t total = 1;
     for (int k = 0; k < nd; ++k) total *= dims[k];
     const int rem = net->new_slot(1, (int)total, 1, data);
+    net->tensor_slot.push_back({rem, total});
     for (int k = 0; k < nd - 1; ++k) {
       const int s = net->new_slot(0, 0, 0, nullptr);
       net->pl.emit(OP_RAW_SPLIT, rem, s, (int)dims[k], net->pl.p.chi, 0, 0, net->pl.p.cutoff);
@@ -1098,7 +1119,20 @@ int catn_tensor_network(catn_ctx* ctx, int32_t ntensors, const int32_t* ndims,
   net->num_isolated = iso;
   net->n = ntensors;
   net->beta = 1.0;
-  if (!net->pl.prog.empty()) catn::run_one(net.get());
+  if (!net->pl.prog.empty()) {
+    // a real network records its raw2mps splits now, for a later reverse sweep
+    if (!net->cplx) {
+      net->ensure_device();
+      int nsplit = 0;
+      for (const catn::Op& op : net->pl.prog) nsplit += op.code == catn::OP_RAW_SPLIT;
+      net->alloc_tape(nsplit);
+      net->hs.record = 1;
+    }
+    catn::run_one(net.get());
+    net->hs.record = 0;
+    net->build_tape = net->hs.tape_len;
+  }
+  net->tensors_taped = !net->cplx;
   *out = net.release();
   return 0;
   CATN_CATCH
@@ -1393,6 +1427,26 @@ int catn_net_site_adjoint(catn_net* net, int32_t slot, double* out, int64_t cap)
   if (slot < 0 || slot >= (int)net->off0.size() || net->off0[slot] < 0)
     throw catn::Error(-2, "site adjoint: slot was not an initial site");
   if (cap > 0) catn::exec_of(net)->d2h(out, net->hs.adj + net->off0[slot], sizeof(double) * (size_t)cap);
+  return 0;
+  CATN_CATCH
+}
+
+int catn_net_tensor_adjoint(catn_net* net, int32_t k, double* out, int64_t cap) {
+  CATN_TRY
+  if (!net->have_adjoint) throw catn::Error(-5, "tensor adjoint: run catn_gradient_batch first");
+  if (!net->tensors_taped)
+    throw catn::Error(-2, "tensor adjoint: the network was not built by catn_tensor_network");
+  if (k < 0 || k >= (int)net->tensor_slot.size())
+    throw catn::Error(-2, "tensor adjoint: no tensor " + std::to_string(k) + " in this network");
+  const auto ts = net->tensor_slot[k];
+  if (cap < 0 || cap > ts.second)
+    throw catn::Error(-2, "tensor adjoint: cap exceeds the tensor's " + std::to_string(ts.second) +
+                              " entries");
+  if (cap > 0) {
+    if (ts.first < 0 || ts.first >= (int)net->off0.size() || net->off0[ts.first] < 0)
+      throw catn::Error(-2, "tensor adjoint: the tensor's data slot has no initial version");
+    catn::exec_of(net)->d2h(out, net->hs.adj + net->off0[ts.first], sizeof(double) * (size_t)cap);
+  }
   return 0;
   CATN_CATCH
 }

@@ -3069,7 +3069,10 @@ CATN_DN void op_raw_split(Exec& E, const Op& op) {
   wg_copy(g, E.out(offu), sp.L, (int64_t)m * k);
   wg_copy(g, E.out(offr), sp.R, (int64_t)k * n);
   E.work(0.0, (double)m * n + (double)m * k + (double)k * n);
-  E.rec_begin(OP_RAW_SPLIT); E.rec_commit();
+  E.rec_begin(OP_RAW_SPLIT); E.rec_in(s);
+  E.rec_out(offu, dleft, di, k); E.rec_out(offr, k, n, 1);
+  E.rec_set(k, sp.qr, 1, op.d, op.x);
+  E.rec_commit();
   E.set_site(op.b, offu, dleft, di, k);
   E.set_site(op.a, offr, k, n, 1);
   g.sync();

@@ -146,6 +146,15 @@ int catn_net_upload(catn_net* net);
  * later catn_gradient_batch with the same tape_cap allocates nothing. No
  * counterpart in the reference (its AD allocates inside the pullback). */
 int catn_net_prepare_gradient(catn_net* net, int32_t tape_cap);
+/* After catn_gradient_batch on a network built from tensors
+ * (catn_tensor_network / catn_net_add_tensor_node): d lnZ / d T for its
+ * tensor k (0-based, in the order added), `cap` entries (at most the
+ * tensor's size) in the tensor's column-major layout, lnZ = log|Z| of the
+ * full contraction, so d lnZ / d T = (dZ / dT) / Z. The reverse sweep runs
+ * the raw2mps splits (src/mps_node.jl:31-50) backwards. Real networks. The
+ * reference has no derivative of contraction!; this extends the reverse sweep
+ * of catn_gradient_batch / catn_ising_gradient to tensor networks. */
+int catn_net_tensor_adjoint(catn_net* net, int32_t k, double* out, int64_t cap);
 /* Test hook of tsvd's fallback (src/linalg_utils.jl:16-21: svd() retried with
  * the QR-iteration driver on a LAPACKException). flags bit 0: every SVD of
  * this network treats its primary path (QR-preconditioned one-sided Jacobi)
