@@ -17,7 +17,7 @@ using ChainRulesCore
 
 export ising_network, tensor_network, braket_network, contraction!, free_energy,
        magnetization, correlation, ising_lnZ, ising_lnZ_batch, result_tensor, set_launch!,
-       set_batch_streams!, prepare_gradient!
+       set_batch_streams!, prepare_gradient!, tensor_lnZ
 
 const lib = get(ENV, "CATN_HIP_LIB", joinpath(@__DIR__, "..", "catn.jl_amd", "lib", "libcatn_hip.so"))
 
@@ -305,6 +305,37 @@ function ChainRulesCore.rrule(::typeof(ising_lnZ_batch), instances; lift=nothing
         (NoTangent(), grads)
     end
     [r[1] for r in res], ising_lnZ_batch_pullback
+end
+
+"""
+    tensor_lnZ(tensors, ixs; kw...) -> ln|Z|
+
+ln|Z| of a real tensor network (`TensorNetwork(tensors, ixs)`, then
+`contraction!`). Its rrule's pullback is the reverse sweep through the
+contraction and the raw2mps splits (catn_gradient_batch +
+catn_net_tensor_adjoint): d ln|Z| / d T_k = environment_k / Z per tensor.
+"""
+tensor_lnZ(tensors, ixs; kw...) = first(contraction!(tensor_network(tensors, ixs; kw...)))
+
+function tensor_lnZ_and_grads(tensors, ixs; kw...)
+    tn = tensor_network(tensors, ixs; kw...)
+    lz, err, psi = zeros(1), zeros(1), zeros(1)
+    check(ccall((:catn_gradient_batch, lib), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Int32, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}), ctx(), 1, [tn.ptr], 0, lz, err, psi))
+    grads = map(enumerate(tensors)) do (k, t)
+        g = zeros(Float64, size(t))
+        check(ccall((:catn_net_tensor_adjoint, lib), Cint,
+                    (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64), tn.ptr, k - 1, g, length(g)))
+        convert(Array{eltype(t)}, g)
+    end
+    lz[1], grads
+end
+
+function ChainRulesCore.rrule(::typeof(tensor_lnZ), tensors, ixs; kw...)
+    lz, grads = tensor_lnZ_and_grads(tensors, ixs; kw...)
+    tensor_lnZ_pullback(Δ) = (NoTangent(), [unthunk(Δ) .* g for g in grads], NoTangent())
+    lz, tensor_lnZ_pullback
 end
 
 # magnetization / correlation (src/ising.jl:263-339). method = :pinning runs the
