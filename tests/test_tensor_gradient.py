@@ -102,6 +102,20 @@ def test_grid3_exact(env, signed):
     assert sum(float(np.sum(t * g)) for t, g in zip(ts, got)) == pytest.approx(len(ts), abs=1e-9)
 
 
+@pytest.mark.parametrize("dtype,tol", [("f32x6", 2e-6), ("f32", 5e-5)])
+def test_fp32_storage(env, dtype, tol):
+    """FP32 site storage (plain, and six-pass split-BF16 swap products): the
+    exact-mode gradient to `tol` of the largest entry, lnZ to `tol`."""
+    catn, ctx = env
+    ts, ixs = Cf.dense_lattice(3, D=3, seed=5, signed=True)
+    Z, want = einsum_grads(ts, ixs)
+    lz, psi, got = net_grads(catn, ctx, ts, ixs, dtype=dtype, **EXACT)
+    assert lz == pytest.approx(np.log(abs(Z)), abs=tol)
+    assert psi == np.sign(Z)
+    scale = max(float(np.max(np.abs(w))) for w in want)
+    assert max(float(np.max(np.abs(g - w))) for g, w in zip(got, want)) <= tol * scale
+
+
 def test_einsum_environment_pin():
     """The test's own environment formula, checked against central
     differences of the einsum Z: a transposed or misplaced environment fails."""
