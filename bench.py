@@ -488,7 +488,7 @@ def main():
                     help="only config 4's 8x8 forward leg and its exact comparison, for profiling")
     ap.add_argument("--config3-only", action="store_true",
                     help="only config 3's batched forward and its oracle comparison, for profiling")
-    ap.add_argument("--cut-free-batch", type=int, default=128,
+    ap.add_argument("--cut-free-batch", type=int, default=512,
                     help="instances per chosen-form step of the cut-free comparison (0: skip it)")
     ap.add_argument("--cut-free-only", action="store_true",
                     help="run only the cut-free chosen-vs-plain comparison and print its JSON")
