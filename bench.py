@@ -586,12 +586,15 @@ def main():
         plain = run_plain(catn, ctx, instances[:args.plain_batch], args.plain_steps,
                           min(args.warmup, 1), args.op_profile, not args.no_autotune)
 
-    if refused and args.form == "both" and args.cut_free_batch > 0:
-        out_cfg["cut_free_comparison"] = run_cut_free(catn, ctx, 3, 1, args.cut_free_batch)
+    # config 3 and 4 before the cut-free comparison, whose 1024 recorded
+    # networks and adjoint arenas would otherwise share the device memory
+    # with config 3's first (allocating) step
     if args.form == "both" and args.config3_steps > 0:
         out_cfg["config3_forward"] = run_config3(catn, ctx, args.config3_steps)
     if args.form == "both" and args.config4:
         out_cfg["config4_forward"] = run_config4(catn, ctx)
+    if refused and args.form == "both" and args.cut_free_batch > 0:
+        out_cfg["cut_free_comparison"] = run_cut_free(catn, ctx, 3, 1, args.cut_free_batch)
 
     # FP64 oracle lnZ of a few instances at the same (Dmax, chi)
     ncheck = 2
